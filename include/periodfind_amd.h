@@ -1,0 +1,170 @@
+/*
+ * periodfind_amd — C ABI of the MI355X (gfx950) periodogram engine.
+ *
+ * Evaluates the curve x trial-period x period_dt grid for Conditional
+ * Entropy, AOV, Lomb-Scargle, FPW and Box Least Squares with hand-written
+ * HIP kernels.  Plain C types only: pointers, sizes and status codes.
+ *
+ * Reference interface replaced: the per-algorithm C++ classes that
+ * periodfind's Cython layer binds through `cdef extern from "./cuda/X.h"`:
+ *   ConditionalEntropy::CalcCEValsBatched   periodfind/ce.pyx:25-39,  cuda/ce.h
+ *   AOV::CalcAOVValsBatched                 periodfind/aov.pyx:25-36, cuda/aov.h
+ *   LombScargle::CalcLSBatched              periodfind/ls.pyx:25-35,  cuda/ls.h
+ *   FPW::CalcFPWBatched                     periodfind/fpw.pyx:20-31, cuda/fpw.h
+ *   BLS::CalcBLSBatched                     periodfind/bls.pyx:19-31, cuda/bls.h
+ * Each of those takes vector<float*> times/mags(/errs) + lengths, the trial
+ * grid, and writes float[n_curves][n_periods][n_pdts].  Here the curves are
+ * CSR-packed (offsets[n_curves+1] into flat arrays), errors are status codes
+ * instead of exit() (cuda/errchk.cuh:14-28), and device state persists in a
+ * context instead of per-call cudaMalloc/cudaFree (cuda/ce.cu:481-561).
+ *
+ * Conventions
+ *   - Host pointers are float32 / int64 C-contiguous arrays owned by the caller.
+ *   - Outputs are written in place; every call is synchronous on return.
+ *   - Output layout is out[curve][period][pdt] (row-major), as in the reference.
+ *   - Return value: PF_OK (0) or a negative PF_E_* code; pf_last_error() gives
+ *     the message of the last failing call on the calling thread.
+ *   - A context is used by one thread at a time (calls on one context serialise
+ *     on its stream).
+ */
+#ifndef PERIODFIND_AMD_H
+#define PERIODFIND_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PF_OK = 0,
+    PF_E_NODEVICE = -1,   /* no HIP device / runtime unavailable */
+    PF_E_ARG = -2,        /* invalid argument (sizes, bins, NULL pointers) */
+    PF_E_HIP = -3,        /* a HIP runtime call failed */
+    PF_E_NOMEM = -4,      /* device or host allocation failed */
+    PF_E_LIMIT = -5       /* parameters exceed a kernel limit (e.g. LDS bins) */
+};
+
+/* Output kinds of the grid entry points. */
+enum {
+    PF_OUT_GRID = 0,      /* full periodogram out[c][p][d] */
+    PF_OUT_PEAKS = 1      /* top-N peaks per curve (rust/src/peaks.rs semantics) */
+};
+
+typedef struct pf_ctx pf_ctx;
+
+/* Number of visible HIP devices (0 when the runtime finds none). */
+int pf_device_count(int* count);
+
+/* Library-wide last error message of the calling thread. */
+const char* pf_last_error(void);
+
+/* Create / destroy a context bound to one device.  The context owns a stream,
+ * device buffers that grow on demand and are reused across calls, and the
+ * per-length-bucket tuning table. */
+int pf_ctx_create(pf_ctx** ctx, int device);
+void pf_ctx_destroy(pf_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Plain drop-in grid evaluation (host in, host out).                        */
+/* t/y(/err) hold sum(len) points, curve c owning [offsets[c], offsets[c+1]). */
+/* out: float[n_curves][n_periods][n_pdts].                                   */
+/* ------------------------------------------------------------------------ */
+
+/* ConditionalEntropy(n_phase, n_mag, phase_bin_extent, mag_bin_extent)
+ * .CalcCEValsBatched — cuda/ce.h, periodfind/ce.pyx:25-39; semantics of
+ * rust/src/ce.rs:13-91.  Magnitudes must already be prepared (the Python layer
+ * normalizes them as periodfind/_utils.py:12-47 does). */
+int pf_ce_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* periods, int64_t n_periods,
+               const float* period_dts, int64_t n_pdts, int n_phase, int n_mag,
+               int phase_ext, int mag_ext, float* out);
+
+/* AOV(n_phase, phase_bin_extent).CalcAOVValsBatched — cuda/aov.h,
+ * periodfind/aov.pyx:25-36; semantics of rust/src/aov.rs:12-89. */
+int pf_aov_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                const float* y, const float* periods, int64_t n_periods,
+                const float* period_dts, int64_t n_pdts, int n_phase, int phase_ext,
+                float* out);
+
+/* LombScargle().CalcLSBatched — cuda/ls.h, periodfind/ls.pyx:25-35;
+ * semantics of rust/src/ls.rs:15-79. */
+int pf_ls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* periods, int64_t n_periods,
+               const float* period_dts, int64_t n_pdts, float* out);
+
+/* FPW(n_bins).CalcFPWBatched — cuda/fpw.h, periodfind/fpw.pyx:20-31;
+ * semantics of rust/src/fpw.rs:13-67.  err == NULL means all ones. */
+int pf_fpw_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                const float* y, const float* err, const float* periods, int64_t n_periods,
+                const float* period_dts, int64_t n_pdts, int n_bins, float* out);
+
+/* BLS(n_bins, qmin, qmax).CalcBLSBatched — cuda/bls.h,
+ * periodfind/bls.pyx:19-31; semantics of rust/src/bls.rs:20-134.
+ * err == NULL means all ones. */
+int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                const float* y, const float* err, const float* periods, int64_t n_periods,
+                const float* period_dts, int64_t n_pdts, int n_bins, float qmin, float qmax,
+                float* out);
+
+/* ------------------------------------------------------------------------ */
+/* Device top-N peaks.  Replaces the CPU-only fused entry points             */
+/* calc_{ce,aov,ls,fpw,bls}_peaks_batched (rust/src/lib.rs:559-754) and the  */
+/* standalone find_top_peaks_batched (rust/src/lib.rs:509-555).              */
+/* idx: int64[n_curves][n_peaks] flat (period_idx*n_pdts + pdt_idx), -1 pad;  */
+/* val: float[n_curves][n_peaks], 0 pad; sorted best-first.                  */
+/* use_max is implied by the algorithm (CE minimises, the others maximise).  */
+/* ------------------------------------------------------------------------ */
+
+/* Algorithm selector for the generic entry points below. */
+enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4 };
+
+/* Algorithm parameters; unused fields are ignored. */
+typedef struct pf_params {
+    int algo;        /* PF_ALGO_* */
+    int n_phase;     /* CE / AOV phase bins */
+    int n_mag;       /* CE magnitude bins */
+    int phase_ext;   /* CE / AOV phase_bin_extent */
+    int mag_ext;     /* CE mag_bin_extent */
+    int n_bins;      /* FPW / BLS bins */
+    float qmin;      /* BLS */
+    float qmax;      /* BLS */
+} pf_params;
+
+/* Grid evaluation followed by on-device peak selection (peaks.rs:18-238).
+ * err may be NULL (ones); it is ignored by CE/AOV/LS. */
+int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64_t* offsets,
+             const float* t, const float* y, const float* err, const float* periods,
+             int64_t n_periods, const float* period_dts, int64_t n_pdts, int64_t n_peaks,
+             int64_t min_distance, int64_t* idx, float* val);
+
+/* Standalone peak finder over a host periodogram data[n_curves][T]
+ * (find_top_peaks_batched, rust/src/lib.rs:509-555). */
+int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
+                      int64_t n_peaks, int64_t min_distance, int use_max, int64_t* idx,
+                      float* val);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident path: inputs already in HBM (used by the benchmark and by */
+/* callers that keep curves on the GPU).  All pointers below are DEVICE      */
+/* pointers on the context's device; offsets are a device int64 array.       */
+/* Results are written to device memory; the call is asynchronous on the     */
+/* context stream unless `sync` is nonzero.                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Full grid, device in / device out.  err_d may be NULL (ones).
+ * max_len: the largest curve length (host-known; sizes the launch). */
+int pf_grid_device(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
+                   const int64_t* offsets_d, const float* t_d, const float* y_d,
+                   const float* err_d, int64_t max_len, const float* periods_d,
+                   int64_t n_periods, const float* period_dts_d, int64_t n_pdts,
+                   float* out_d, int sync);
+
+/* Kernel time of the last grid kernel launched on ctx (HIP events), ms. */
+int pf_last_kernel_ms(pf_ctx* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PERIODFIND_AMD_H */

@@ -1,0 +1,206 @@
+"""ctypes wrapper over the CPU oracle (liboracle.so, built from pf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the periodfind_amd package.
+
+Each ``*_grid`` function mirrors one batched Rust entry point
+(rust/src/lib.rs:31-324: calc_{ce,aov,ls,fpw,bls}_batched) and returns a
+float32 array of shape [n_curves, n_periods, n_pdts].  The Python-level
+preprocessing of the reference CPU classes (periodfind/cpu/__init__.py:94-672:
+which algorithms call prepare_magnitudes, errs defaulting to ones) is
+mirrored by :func:`calc_grid`.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_flt = ctypes.c_float
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.or_fold_time.restype = _flt
+        L.or_fold_time.argtypes = [_flt, _flt, _flt]
+        L.or_fold_batch.restype = None
+        L.or_fold_batch.argtypes = [_f32p, _i64, _flt, _flt, _f32p]
+        L.or_ce_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64,
+                                    _int, _int, _int, _int, _f32p, _int]
+        L.or_aov_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64,
+                                     _int, _int, _f32p, _int]
+        L.or_ls_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64,
+                                    _f32p, _int]
+        L.or_fpw_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _i64, _f32p,
+                                     _i64, _int, _f32p, _int]
+        L.or_bls_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _i64, _f32p,
+                                     _i64, _int, _flt, _flt, _f32p, _int]
+        L.or_bls_prepare.restype = None
+        L.or_bls_prepare.argtypes = [_f32p, _f32p, _i64, _f32p, _f32p,
+                                     ctypes.POINTER(ctypes.c_float)]
+        L.or_bls_duration_range.restype = None
+        L.or_bls_duration_range.argtypes = [_int, _flt, _flt, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64)]
+        L.or_find_top_peaks.restype = _i64
+        L.or_find_top_peaks.argtypes = [_f32p, _i64, _i64, _i64, _int, _i64p, _f32p]
+        L.or_find_top_peaks_batched.restype = None
+        L.or_find_top_peaks_batched.argtypes = [_f32p, _i64, _i64, _i64, _i64, _int, _i64p,
+                                                _f32p]
+        L.or_max_threads.restype = _int
+        _lib = L
+    return _lib
+
+
+def pack(arrays):
+    """CSR-pack a list of 1-D float32 arrays -> (offsets int64[n+1], flat float32)."""
+    lens = np.array([len(a) for a in arrays], dtype=np.int64)
+    off = np.zeros(len(arrays) + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    flat = (np.concatenate([np.asarray(a, dtype=np.float32) for a in arrays])
+            if len(arrays) else np.zeros(0, np.float32))
+    return off, np.ascontiguousarray(flat, dtype=np.float32)
+
+
+def _grid_args(times, periods, period_dts):
+    periods = np.ascontiguousarray(periods, dtype=np.float32)
+    period_dts = np.ascontiguousarray(period_dts, dtype=np.float32)
+    off, t = pack(times)
+    out = np.zeros((len(times), len(periods), len(period_dts)), dtype=np.float32)
+    return off, t, periods, period_dts, out
+
+
+def fold_time(t, period, period_dt=0.0):
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    out = np.empty_like(t)
+    lib().or_fold_batch(t, t.size, float(period), float(period_dt), out)
+    return out
+
+
+def ce_grid(times, mags, periods, period_dts, n_phase=10, n_mag=10, p_ext=1, m_ext=1,
+            threads=0):
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    lib().or_ce_batched(len(times), off, t, m, P, P.size, D, D.size, n_phase, n_mag, p_ext,
+                        m_ext, out.reshape(-1), threads)
+    return out
+
+
+def aov_grid(times, mags, periods, period_dts, n_phase=10, ext=1, threads=0):
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    lib().or_aov_batched(len(times), off, t, m, P, P.size, D, D.size, n_phase, ext,
+                         out.reshape(-1), threads)
+    return out
+
+
+def ls_grid(times, mags, periods, period_dts, threads=0):
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    lib().or_ls_batched(len(times), off, t, m, P, P.size, D, D.size, out.reshape(-1), threads)
+    return out
+
+
+def fpw_grid(times, mags, errs, periods, period_dts, n_bins=10, threads=0):
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    _, e = pack(errs)
+    lib().or_fpw_batched(len(times), off, t, m, e, P, P.size, D, D.size, n_bins,
+                         out.reshape(-1), threads)
+    return out
+
+
+def bls_grid(times, mags, errs, periods, period_dts, n_bins=50, qmin=0.01, qmax=0.5,
+             threads=0):
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    _, e = pack(errs)
+    lib().or_bls_batched(len(times), off, t, m, e, P, P.size, D, D.size, n_bins, qmin, qmax,
+                         out.reshape(-1), threads)
+    return out
+
+
+def bls_duration_range(n_bins, qmin, qmax):
+    lo, hi = ctypes.c_int64(), ctypes.c_int64()
+    lib().or_bls_duration_range(n_bins, qmin, qmax, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def find_top_peaks(data, n_peaks=32, min_distance=1, use_max=True):
+    """rust/src/peaks.rs:228-238 on a flat array -> (idx int64[k], val float32[k])."""
+    data = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+    idx = np.zeros(max(n_peaks, 1), np.int64)
+    val = np.zeros(max(n_peaks, 1), np.float32)
+    k = lib().or_find_top_peaks(data, data.size, n_peaks, min_distance, int(bool(use_max)),
+                                idx, val)
+    return idx[:k].copy(), val[:k].copy()
+
+
+def find_top_peaks_batched(data, n_peaks=32, min_distance=1, use_max=True):
+    """rust/src/lib.rs:509-555: data [C, P, D] -> (idx [C, n_peaks] (-1 pad), val (0 pad))."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    C = data.shape[0]
+    T = int(np.prod(data.shape[1:])) if data.ndim > 1 else 0
+    idx = np.zeros((C, n_peaks), np.int64)
+    val = np.zeros((C, n_peaks), np.float32)
+    lib().or_find_top_peaks_batched(data.reshape(-1), C, T, n_peaks, min_distance,
+                                    int(bool(use_max)), idx.reshape(-1), val.reshape(-1))
+    return idx, val
+
+
+USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True}
+
+
+def prepare_magnitudes(mags, center, normalize):
+    """periodfind/_utils.py:12-47 (warning omitted: the oracle never sets both)."""
+    if center:
+        return [m - np.mean(m) for m in mags]
+    if normalize:
+        out = []
+        for m in mags:
+            lo, hi = np.min(m), np.max(m)
+            out.append(((m - lo) / (hi - lo)) * 0.999 + 5e-4)
+        return out
+    return mags
+
+
+def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **params):
+    """Full periodogram with the reference CPU classes' preprocessing defaults.
+
+    CE normalizes (cpu/__init__.py:94), AOV/FPW/BLS pass magnitudes through,
+    LS centers (cpu/__init__.py:318); FPW/BLS default errs to ones (469-470).
+    """
+    if algo == "ce":
+        mm = prepare_magnitudes(mags, params.pop("center", False), params.pop("normalize", True))
+        return ce_grid(times, mm, periods, period_dts, threads=threads, **params)
+    if algo == "aov":
+        mm = prepare_magnitudes(mags, params.pop("center", False), params.pop("normalize", False))
+        return aov_grid(times, mm, periods, period_dts, threads=threads, **params)
+    if algo == "ls":
+        mm = prepare_magnitudes(mags, params.pop("center", True), params.pop("normalize", False))
+        return ls_grid(times, mm, periods, period_dts, threads=threads)
+    if errs is None:
+        errs = [np.ones(len(t), dtype=np.float32) for t in times]
+    if algo == "fpw":
+        return fpw_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
+    if algo == "bls":
+        return bls_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
+    raise ValueError(algo)

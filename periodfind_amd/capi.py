@@ -1,0 +1,223 @@
+"""ctypes view of the C ABI in include/periodfind_amd.h.
+
+The Cython binding (_pfhip.pyx) is the product's Python entry; this module
+exposes the same exported symbols to tests and tools that want to call the
+C ABI directly, with plain numpy arrays.  Loading fails loudly when the
+library has not been built.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfhip.so")
+
+PF_OK = 0
+PF_E_NODEVICE = -1
+PF_E_ARG = -2
+PF_E_HIP = -3
+PF_E_NOMEM = -4
+PF_E_LIMIT = -5
+
+PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
+
+# Every symbol the header declares, with (restype, argtypes).
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_flt = ctypes.c_float
+
+
+class PfParams(ctypes.Structure):
+    _fields_ = [
+        ("algo", ctypes.c_int),
+        ("n_phase", ctypes.c_int),
+        ("n_mag", ctypes.c_int),
+        ("phase_ext", ctypes.c_int),
+        ("mag_ext", ctypes.c_int),
+        ("n_bins", ctypes.c_int),
+        ("qmin", ctypes.c_float),
+        ("qmax", ctypes.c_float),
+    ]
+
+
+def _opt_f32p():
+    """float* that also accepts None (NULL)."""
+
+    class _P:
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return _f32p.from_param(obj)
+
+    return _P
+
+
+_f32p_or_null = _opt_f32p()
+
+SIGNATURES = {
+    "pf_device_count": (_int, [ctypes.POINTER(ctypes.c_int)]),
+    "pf_last_error": (ctypes.c_char_p, []),
+    "pf_ctx_create": (_int, [ctypes.POINTER(_vp), _int]),
+    "pf_ctx_destroy": (None, [_vp]),
+    "pf_ce_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64,
+                          _int, _int, _int, _int, _f32p]),
+    "pf_aov_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64,
+                           _int, _int, _f32p]),
+    "pf_ls_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64, _f32p]),
+    "pf_fpw_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
+                           _i64, _int, _f32p]),
+    "pf_bls_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
+                           _i64, _int, _flt, _flt, _f32p]),
+    "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
+                        _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
+    "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
+    "pf_grid_device": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _vp, _vp, _vp, _vp, _i64,
+                              _vp, _i64, _vp, _i64, _vp, _int]),
+    "pf_last_kernel_ms": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+class PfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"periodfind_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpfhip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing; build it with `python -m periodfind_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != PF_OK:
+        raise PfError(code, lib().pf_last_error().decode())
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().pf_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def pack(arrays):
+    lens = np.array([len(a) for a in arrays], dtype=np.int64)
+    off = np.zeros(len(arrays) + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    flat = (np.concatenate([np.ascontiguousarray(a, dtype=np.float32) for a in arrays])
+            if len(arrays) else np.zeros(0, np.float32))
+    return off, np.ascontiguousarray(flat, dtype=np.float32)
+
+
+class Context:
+    """A pf_ctx on one device."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        check(lib().pf_ctx_create(ctypes.byref(self.h), device))
+
+    def close(self):
+        if self.h:
+            lib().pf_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_float(0.0)
+        check(lib().pf_last_kernel_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def grid(self, algo, times, mags, periods, period_dts, errs=None, **p):
+        """Full periodogram [n_curves, n_periods, n_pdts] through the C ABI."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
+        o = out.reshape(-1)
+        L = lib()
+        C = len(times)
+        if algo == "ce":
+            check(L.pf_ce_grid(self.h, C, off, t, y, P, P.size, D, D.size,
+                               p.get("n_phase", 10), p.get("n_mag", 10), p.get("phase_ext", 1),
+                               p.get("mag_ext", 1), o))
+        elif algo == "aov":
+            check(L.pf_aov_grid(self.h, C, off, t, y, P, P.size, D, D.size,
+                                p.get("n_phase", 10), p.get("phase_ext", 1), o))
+        elif algo == "ls":
+            check(L.pf_ls_grid(self.h, C, off, t, y, P, P.size, D, D.size, o))
+        elif algo == "fpw":
+            check(L.pf_fpw_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
+                                p.get("n_bins", 10), o))
+        elif algo == "bls":
+            check(L.pf_bls_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
+                                p.get("n_bins", 50), p.get("qmin", 0.01), p.get("qmax", 0.5), o))
+        else:
+            raise ValueError(algo)
+        return out
+
+    def peaks(self, algo, times, mags, periods, period_dts, errs=None, n_peaks=32,
+              min_distance=1, **p):
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        prm = make_params(algo, **p)
+        idx = np.zeros((len(times), n_peaks), np.int64)
+        val = np.zeros((len(times), n_peaks), np.float32)
+        check(lib().pf_peaks(self.h, ctypes.byref(prm), len(times), off, t, y, e, P, P.size, D,
+                             D.size, n_peaks, min_distance, idx.reshape(-1), val.reshape(-1)))
+        return idx, val
+
+    def find_top_peaks(self, data, n_peaks=32, min_distance=1, use_max=True):
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        C = data.shape[0]
+        T = int(np.prod(data.shape[1:]))
+        idx = np.zeros((C, n_peaks), np.int64)
+        val = np.zeros((C, n_peaks), np.float32)
+        check(lib().pf_find_top_peaks(self.h, data.reshape(-1), C, T, n_peaks, min_distance,
+                                      int(bool(use_max)), idx.reshape(-1), val.reshape(-1)))
+        return idx, val
+
+
+def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
+                qmax=0.5, ext=None):
+    prm = PfParams()
+    prm.algo = PF_ALGO[algo]
+    if n_phase is None:
+        n_phase = 10
+    prm.n_phase = n_phase
+    prm.n_mag = n_mag
+    prm.phase_ext = ext if ext is not None else phase_ext
+    prm.mag_ext = mag_ext
+    if n_bins is None:
+        n_bins = 50 if algo == "bls" else 10
+    prm.n_bins = n_bins
+    prm.qmin = qmin
+    prm.qmax = qmax
+    return prm

@@ -1,0 +1,109 @@
+// Device-side building blocks shared by every periodogram kernel (gfx950).
+//
+// Decomposition used by all kernels: one wave64 evaluates 64 consecutive
+// trials of one curve, one trial per lane ("trial per lane").  Every lane
+// walks the curve's points in order 0..N-1, so each per-trial accumulation
+// runs in exactly the serial order of the Rust reference (e.g.
+// rust/src/fpw.rs:44-52), and the point stream is wave-uniform: it is read
+// with scalar loads and broadcast to all lanes through SGPR operands.
+// Per-trial bins live in LDS, lane-interleaved ([bin][lane], 4 B words), so
+// a lane's bin update never contends with another lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+constexpr int kWave = 64;
+
+// rust/src/fold.rs:13-15 — pdt correction in f32.
+__device__ __forceinline__ float pdt_correction(float period, float period_dt) {
+    return (period_dt / period) / 2.0f;
+}
+
+// Per-trial constants of the fold.
+struct FoldTrial {
+    double p;      // period (f64)
+    double inv_p;  // RN(1/p)
+    double k;      // pdt correction (f32 widened)
+};
+
+__device__ __forceinline__ FoldTrial make_fold_trial(float period, float period_dt) {
+    FoldTrial f;
+    f.p = static_cast<double>(period);
+    f.inv_p = 1.0 / f.p;  // IEEE double division, once per trial
+    f.k = static_cast<double>(pdt_correction(period, period_dt));
+    return f;
+}
+
+// rust/src/fold.rs:22-26, bit-exact:
+//   t_corr = t - (k*t)*t   (k*t is exact in f64, so (k*t)*t == RN(k*t^2) == RN(k*tt)
+//                           with tt = t*t, itself exact in f64)
+//   ratio  = RN(t_corr / p) via q0 = t_corr*RN(1/p), r = t_corr - q0*p (exact
+//            with FMA), q = q0 + r*RN(1/p) (reciprocal division with one
+//            correction step; reproduces the IEEE quotient)
+//   phase  = f32(|ratio - floor(ratio)|)
+// ratio - floor(ratio) is >= 0 for every ratio, so the abs is a no-op; it is
+// exact for ratio >= 0 and v_fract_f64 computes the same rounded difference
+// for ratio < 0 (its clamp below 1.0 rounds to the same f32 1.0f that the
+// reference returns).  t and tt are wave-uniform (scalar operands); the f64
+// ops run at full rate on CDNA4.
+template <bool HAS_PDT>
+__device__ __forceinline__ float fold_phase(double t, double tt, const FoldTrial& f) {
+    double tc = t;
+    if (HAS_PDT) tc = t - f.k * tt;
+    const double q0 = tc * f.inv_p;
+    const double r = __builtin_fma(-q0, f.p, tc);
+    const double q = __builtin_fma(r, f.inv_p, q0);
+    return static_cast<float>(__builtin_amdgcn_fract(q));
+}
+
+// Exact `(phase / b) as usize` with b = fl(1/n) (rust/src/ce.rs:52,
+// aov.rs:55) for a phase in [0, 1], without a division: q0 = phase*RN(1/b),
+// one FMA correction, truncate.  Checked exhaustively over every f32 phase in
+// [0, 1] against the IEEE quotient for the supported bin counts.
+struct DivBin {
+    float b;      // fl(1/n)
+    float rcp_b;  // RN(1/b)
+};
+
+__device__ __forceinline__ DivBin make_div_bin(int n) {
+    DivBin d;
+    d.b = 1.0f / static_cast<float>(n);
+    d.rcp_b = 1.0f / d.b;
+    return d;
+}
+
+__device__ __forceinline__ uint32_t div_bin(float ph, const DivBin& d) {
+    const float q0 = ph * d.rcp_b;
+    const float r = __builtin_fmaf(-q0, d.b, ph);
+    const float q = __builtin_fmaf(r, d.rcp_b, q0);
+    return static_cast<uint32_t>(q);
+}
+
+// `min((phase * n as f32) as usize, n-1)` (rust/src/fpw.rs:48, bls.rs:79).
+__device__ __forceinline__ uint32_t mul_bin(float ph, float nf, uint32_t nm1) {
+    const uint32_t k = static_cast<uint32_t>(ph * nf);
+    return k < nm1 ? k : nm1;
+}
+
+// Points are consumed in chunks of kChunk: the chunk's wave-uniform loads are
+// issued together (one scalar burst) before any of its LDS updates.
+constexpr int kChunk = 8;
+
+// Map the hardware workgroup id so that consecutive logical blocks land on
+// the same XCD (gfx950 dispatches workgroup i to XCD i % 8; each XCD has its
+// own L2).  Blocks of one curve then share that curve's points in one L2.
+__device__ __forceinline__ uint64_t xcd_remap(uint64_t hw_id, uint64_t n_blocks) {
+    constexpr uint64_t kXcd = 8;
+    uint64_t per = n_blocks / kXcd;
+    uint64_t rem = n_blocks % kXcd;
+    uint64_t xcd = hw_id % kXcd;
+    uint64_t slot = hw_id / kXcd;
+    // The first `rem` XCDs own per+1 logical blocks, the others `per`.
+    uint64_t base = xcd * per + (xcd < rem ? xcd : rem);
+    return base + slot;
+}
+
+}  // namespace pf

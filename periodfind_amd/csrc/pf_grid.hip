@@ -1,0 +1,516 @@
+// Periodogram grid kernels for gfx950: CE, AOV, LS, FPW, BLS.
+//
+// Launch shape (all kernels): one wave64 workgroup per (curve, block of 64
+// consecutive trials); lane = trial.  Trials are the flattened
+// [period][pdt] index, the reference's output order (rust/src/ce.rs:37-39).
+// The workgroup id is remapped so blocks of one curve share an XCD.
+//
+// Every kernel reads, per point, a wave-uniform record (t as f64, optional
+// t*t, and one or two f32 aux values prepared once per curve set) and
+// accumulates into lane-private state: LDS bins for the binned statistics,
+// registers for LS.  Accumulation is serial in point order per trial, as in
+// the Rust reference, so f32 bin sums are bit-identical to it.
+#include "pf_device.hpp"
+#include "pf_kernels.hpp"
+
+namespace pf {
+
+// Resolve (curve, trial) of this lane.  Lanes past the last trial compute
+// the last trial again and do not store.
+struct LaneTrial {
+    int64_t curve;
+    int64_t trial;
+    bool active;
+};
+
+__device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tblocks, int64_t T) {
+    uint64_t L = xcd_remap(blockIdx.x, n_blocks);
+    LaneTrial lt;
+    lt.curve = static_cast<int64_t>(L / static_cast<uint64_t>(tblocks));
+    int64_t tb = static_cast<int64_t>(L % static_cast<uint64_t>(tblocks));
+    int64_t tr = tb * kWave + threadIdx.x;
+    lt.active = tr < T;
+    lt.trial = lt.active ? tr : T - 1;
+    return lt;
+}
+
+// Wrap `k + o` into [0, n) for k <= n and small o (rust `% n`, ce.rs:57-58).
+__device__ __forceinline__ uint32_t wrap_add(uint32_t k, uint32_t o, uint32_t n) {
+    uint32_t v = k + o;
+    while (v >= n) v -= n;
+    return v;
+}
+
+// s_waitcnt lgkmcnt(0) (vmcnt/expcnt left at their maxima), as a real
+// instruction the compiler's wait-count tracking sees, fenced so the
+// scheduler does not move loads across it.
+__device__ __forceinline__ void wait_scalar_loads() {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Walk the curve's points in order.  Points are consumed in chunks of C:
+// the wave-uniform loads of chunk i+1 (times and the statistic's aux record,
+// scalar loads) are issued before chunk i is folded and accumulated, so the
+// single lgkmcnt wait per chunk (scalar loads and LDS updates share that
+// counter) falls after a whole chunk of VALU work.  The prefetch index is
+// clamped to the last full chunk, so it never reads past the curve.
+template <bool HAS_PDT, class Load, class Body>
+__device__ __forceinline__ void scan_points(const double* __restrict__ t64,
+                                            const double* __restrict__ tt, int64_t n,
+                                            const FoldTrial& f, Load load, Body body) {
+    using Rec = decltype(load(int64_t(0)));
+    constexpr int C = HAS_PDT ? kChunk / 2 : kChunk;
+    struct Chunk {
+        double t[C];
+        double t2[C];
+        Rec r[C];
+    };
+    auto fetch = [&](int64_t base, Chunk& ch) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            ch.t[j] = t64[base + j];
+            ch.t2[j] = HAS_PDT ? tt[base + j] : 0.0;
+            ch.r[j] = load(base + j);
+        }
+    };
+    auto run = [&](const Chunk& ch) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) body(fold_phase<HAS_PDT>(ch.t[j], ch.t2[j], f), ch.r[j]);
+    };
+    // Ping-pong over two buffers.  Scalar loads return out of order, so any
+    // wait on them is lgkmcnt(0); the explicit wait sits before each
+    // prefetch, where the only outstanding scalar load is the buffer issued
+    // one whole chunk earlier.  The scheduling barriers keep that order (left
+    // alone, the scheduler sinks loads below the consuming chunk to save
+    // SGPRs, which exposes their latency).
+    int64_t i = 0;
+    if (n >= 2 * C) {
+        const int64_t last = n - C;
+        Chunk a, b;
+        fetch(0, a);
+        for (; i + 2 * C <= n; i += 2 * C) {
+            wait_scalar_loads();
+            fetch(i + C, b);
+            __builtin_amdgcn_sched_barrier(0);
+            run(a);
+            wait_scalar_loads();
+            const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
+            fetch(na, a);
+            __builtin_amdgcn_sched_barrier(0);
+            run(b);
+        }
+    }
+    for (; i < n; ++i) body(fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f), load(i));
+}
+
+struct F2 {
+    float a, b;
+};
+
+// ---------------------------------------------------------------------------
+// Conditional Entropy — rust/src/ce.rs:13-91.
+// LDS: u32 hist[n_phase*n_mag][64].  Per point: fold, exact phase bin,
+// one ds_add_u32 per (phase_ext x mag_ext) cell.  mrow[i] is the byte offset
+// (mag bin % n_mag) * 256 of the point's magnitude column, wave-uniform and
+// prepared per call.
+// ---------------------------------------------------------------------------
+template <bool HAS_PDT, bool EXT1>
+__global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
+    extern __shared__ uint32_t hist[];
+    const int lane = threadIdx.x;
+    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+    // Rows 0..n_phase: row n_phase catches phase == 1.0 (bin n), folded into
+    // row 0 after the scan (counts commute), so the hot loop needs no wrap.
+    const int nrows = EXT1 ? a.n_phase + 1 : a.n_phase;
+    const int ncell = nrows * a.n_mag;
+    for (int b = 0; b < ncell; ++b) hist[b * kWave + lane] = 0u;
+
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const uint32_t* __restrict__ mrow = a.mrow + p0;
+    const DivBin db = make_div_bin(a.n_phase);
+    const uint32_t np_ = static_cast<uint32_t>(a.n_phase);
+    const uint32_t nm = static_cast<uint32_t>(a.n_mag);
+    uint32_t* __restrict__ col = hist + lane;
+    const uint32_t row_bytes = nm * kWave * 4u;
+
+    scan_points<HAS_PDT>(
+        g.t64 + p0, g.tt + p0, n, f,
+        [&](int64_t i) { return mrow[i]; },
+        [&](float ph, uint32_t mb) {
+            const uint32_t k = div_bin(ph, db);
+            if (EXT1) {
+                // mb is the wave-uniform byte offset of the magnitude column.
+                uint32_t* cell = reinterpret_cast<uint32_t*>(
+                    reinterpret_cast<char*>(col) + __umul24(k, row_bytes) + mb);
+                __hip_atomic_fetch_add(cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            } else {
+                const uint32_t mbin = mb / (kWave * 4u);
+                for (int po = 0; po < a.phase_ext; ++po) {
+                    const uint32_t pb = wrap_add(k, po, np_);
+                    for (int mo = 0; mo < a.mag_ext; ++mo) {
+                        const uint32_t m2 = wrap_add(mbin, mo, nm);
+                        __hip_atomic_fetch_add(col + (pb * nm + m2) * kWave, 1u,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                }
+            }
+        });
+    if (EXT1) {
+        for (int i = 0; i < a.n_mag; ++i)
+            col[i * kWave] += col[(a.n_phase * a.n_mag + i) * kWave];
+    }
+
+    // ce.rs:66-86: p = count / (N*ext_p*ext_m); CE = sum_j sum_i p_ij ln(p_j/p_ij).
+    const float norm = static_cast<float>(static_cast<uint64_t>(n) *
+                                          static_cast<uint64_t>(a.phase_ext) *
+                                          static_cast<uint64_t>(a.mag_ext));
+    float ce = 0.0f;
+    for (int j = 0; j < a.n_phase; ++j) {
+        const uint32_t* row = col + j * a.n_mag * kWave;
+        float pj = 0.0f;
+        for (int i = 0; i < a.n_mag; ++i) pj += static_cast<float>(row[i * kWave]) / norm;
+        for (int i = 0; i < a.n_mag; ++i) {
+            const float pij = static_cast<float>(row[i * kWave]) / norm;
+            if (pij != 0.0f) {
+                const float term = pij * logf(pj / pij);
+                ce += term;
+            }
+        }
+    }
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = ce;
+}
+
+// ---------------------------------------------------------------------------
+// AOV — rust/src/aov.rs:12-89.
+// LDS: u32 cnt[nb][64], f32 sum[nb][64], f32 sq[nb][64].  Per point: three
+// ds_add per extent cell.  Per-curve mean is the serial f32 mean (aov.rs:27).
+// ---------------------------------------------------------------------------
+template <bool HAS_PDT, bool EXT1>
+__global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
+    extern __shared__ uint32_t lds[];
+    const int lane = threadIdx.x;
+    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+    const int nb = a.n_bins;
+    uint32_t* cnt = lds + lane;
+    float* sum = reinterpret_cast<float*>(lds + nb * kWave) + lane;
+    float* sq = reinterpret_cast<float*>(lds + 2 * nb * kWave) + lane;
+    for (int b = 0; b < nb; ++b) {
+        cnt[b * kWave] = 0u;
+        sum[b * kWave] = 0.0f;
+        sq[b * kWave] = 0.0f;
+    }
+
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const float* __restrict__ y = a.y + p0;
+    const float* __restrict__ yy = a.yy + p0;
+    const float nf = static_cast<float>(nb);
+    const DivBin db = make_div_bin(nb);
+    const uint32_t nbu = static_cast<uint32_t>(nb);
+
+    scan_points<HAS_PDT>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; },
+        [&](float ph, F2 v) {
+            const uint32_t k = div_bin(ph, db);
+            const int reps = EXT1 ? 1 : a.ext;
+            for (int o = 0; o < reps; ++o) {
+                const uint32_t b = EXT1 ? ((k == nbu) ? 0u : k) : wrap_add(k, o, nbu);
+                __hip_atomic_fetch_add(cnt + b * kWave, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __hip_atomic_fetch_add(sum + b * kWave, v.a, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __hip_atomic_fetch_add(sq + b * kWave, v.b, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        });
+
+    // aov.rs:67-84.
+    const float avg = a.avg[lt.curve];
+    float s1 = 0.0f, s2 = 0.0f;
+    for (int b = 0; b < nb; ++b) {
+        const float c = static_cast<float>(cnt[b * kWave]);
+        if (c != 0.0f) {
+            const float aux = sum[b * kWave] / c;
+            const float res = aux - avg;
+            float t1 = c * res;
+            t1 = t1 * res;
+            s1 += t1;
+            float t2 = c * aux;
+            t2 = t2 * aux;
+            s2 += sq[b * kWave] - t2;
+        }
+    }
+    float F = 0.0f;
+    if (s2 != 0.0f) {
+        const float lhs = (static_cast<float>(n) - nf) / (nf - 1.0f);
+        F = lhs * (s1 / s2);
+    }
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = F;
+}
+
+// ---------------------------------------------------------------------------
+// Lomb-Scargle — rust/src/ls.rs:15-79.
+// Registers only: C = sum y cos, S = sum y sin, CC = sum cos^2, CS = sum cos sin
+// accumulated serially per trial.  FAST uses the hardware v_sin/v_cos, which
+// take the angle in revolutions (the fold phase itself); otherwise sincosf of
+// fl(2pi_f32 * phase) as the reference computes it.
+// ---------------------------------------------------------------------------
+template <bool HAS_PDT, bool FAST>
+__global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
+    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const double* __restrict__ t64 = g.t64 + p0;
+    const double* __restrict__ tt = g.tt + p0;
+    const float* __restrict__ y = a.y + p0;
+    const float two_pi = 2.0f * 3.14159274101257324f;  // 2 * (f32)pi, ls.rs:9
+
+    float mc = 0.0f, ms = 0.0f, cc = 0.0f, cs = 0.0f;
+    scan_points<HAS_PDT>(
+        t64, tt, n, f, [&](int64_t i) { return y[i]; },
+        [&](float ph, float yi) {
+            float s, c;
+            if (FAST) {
+                s = __builtin_amdgcn_sinf(ph);
+                c = __builtin_amdgcn_cosf(ph);
+            } else {
+                sincosf(two_pi * ph, &s, &c);
+            }
+            float p;
+            p = yi * c; mc += p;
+            p = yi * s; ms += p;
+            p = c * c;  cc += p;
+            p = c * s;  cs += p;
+        });
+    // ls.rs:47-77.
+    const float ss = static_cast<float>(n) - cc;
+    const float tau = 0.5f * atan2f(2.0f * cs, cc - ss);
+    float st, ct;
+    sincosf(tau, &st, &ct);
+    float v = ct * mc + st * ms;
+    const float num_l = v * v;
+    v = ct * ms - st * mc;
+    const float num_r = v * v;
+    const float den_l = ct * ct * cc + 2.0f * ct * st * cs + st * st * ss;
+    const float den_r = ct * ct * ss - 2.0f * ct * st * cs + st * st * cc;
+    float P = 0.0f;
+    if (!(den_l == 0.0f || den_r == 0.0f)) P = 0.5f * (num_l / den_l + num_r / den_r);
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = P;
+}
+
+// ---------------------------------------------------------------------------
+// FPW — rust/src/fpw.rs:13-67.
+// LDS: f32 vw[nb][64], vy[nb][64]; w = 1/(e*e), wy = w*y prepared per curve
+// set (fpw.rs:26-27, uncentred).
+// ---------------------------------------------------------------------------
+template <bool HAS_PDT>
+__global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
+    extern __shared__ float fl[];
+    const int lane = threadIdx.x;
+    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+    const int nb = a.n_bins;
+    float* vw = fl + lane;
+    float* vy = fl + nb * kWave + lane;
+    for (int b = 0; b < nb; ++b) {
+        vw[b * kWave] = 0.0f;
+        vy[b * kWave] = 0.0f;
+    }
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const float* __restrict__ w = a.w + p0;
+    const float* __restrict__ wy = a.wy + p0;
+    const float nf = static_cast<float>(nb);
+    const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
+
+    scan_points<HAS_PDT>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], wy[i]}; },
+        [&](float ph, F2 v) {
+            const uint32_t b = mul_bin(ph, nf, nm1);
+            __hip_atomic_fetch_add(vw + b * kWave, v.a, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __hip_atomic_fetch_add(vy + b * kWave, v.b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+        });
+    // fpw.rs:55-60.
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) {
+        const float W = vw[b * kWave];
+        if (W > 0.0f) {
+            const float Y = vy[b * kWave];
+            const float num = Y * Y;
+            const float den = 2.0f * W;
+            s += num / den;
+        }
+    }
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = s;
+}
+
+// ---------------------------------------------------------------------------
+// BLS — rust/src/bls.rs:20-134.
+// LDS: two planes (W, Y) of [nb+1][64] f32.  Binning writes bin k into slot
+// k+1; the prefix pass then turns slot k into prefix[k] in place
+// (bls.rs:85-91).  The (duration, phase) search (bls.rs:100-127) is serial
+// per lane, with uniform loop bounds, so lanes never diverge on them.
+//
+// Exact max: the reference keeps max over fl(num/den).  Rather than one IEEE
+// division per candidate, a candidate is tested against the midpoint M
+// between the current best B and its f32 successor: fl(num/den) > B iff
+// num > M*den (or == with round-to-even going up).  num, den are f32 and M
+// has 25 significant bits, so M*den is exact in f64.  Only when a lane
+// improves is fl(num/den) computed by IEEE division.
+// ---------------------------------------------------------------------------
+// Successor of a finite f32 b >= 0 (best only grows from 0).
+__device__ __forceinline__ float succ_pos(float b) {
+    return __uint_as_float(__float_as_uint(b) + 1u);
+}
+
+__device__ __forceinline__ double bls_midpoint_up(float b) {
+    return 0.5 * (static_cast<double>(b) + static_cast<double>(succ_pos(b)));
+}
+
+template <bool HAS_PDT>
+__global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
+    extern __shared__ float fl[];
+    const int lane = threadIdx.x;
+    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+    const int nb = a.n_bins;
+    float* W = fl + lane;                     // slots 0..nb
+    float* Y = fl + (nb + 1) * kWave + lane;  // slots 0..nb
+    for (int b = 0; b <= nb; ++b) {
+        W[b * kWave] = 0.0f;
+        Y[b * kWave] = 0.0f;
+    }
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const float* __restrict__ w = a.w + p0;
+    const float* __restrict__ yw = a.yw + p0;
+    const float nf = static_cast<float>(nb);
+    const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
+    float* W1 = W + kWave;
+    float* Y1 = Y + kWave;
+
+    scan_points<HAS_PDT>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; },
+        [&](float ph, F2 v) {
+            const uint32_t b = mul_bin(ph, nf, nm1);
+            __hip_atomic_fetch_add(W1 + b * kWave, v.a, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __hip_atomic_fetch_add(Y1 + b * kWave, v.b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+        });
+    // Prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
+    {
+        float pw = 0.0f, py = 0.0f;
+        for (int k = 0; k < nb; ++k) {
+            pw = pw + W1[k * kWave];
+            py = py + Y1[k * kWave];
+            W1[k * kWave] = pw;
+            Y1[k * kWave] = py;
+        }
+    }
+    const float wt = W[nb * kWave];
+    float best = 0.0f;
+    if (wt > 0.0f) {
+        double mid = bls_midpoint_up(best);
+        for (int p = 0; p < nb; ++p) {
+            const float w0 = W[p * kWave];
+            const float y0 = Y[p * kWave];
+            const float wtail = wt - w0;                  // wp[nb] - wp[phi]
+            const float ytail = Y[nb * kWave] - y0;       // yp[nb] - yp[phi]
+            for (int d = a.d_lo; d <= a.d_hi; ++d) {
+                const int e = p + d;
+                float r, s;
+                if (e <= nb) {
+                    r = W[e * kWave] - w0;
+                    s = Y[e * kWave] - y0;
+                } else {
+                    const int wr = e - nb;
+                    r = wtail + W[wr * kWave];
+                    s = ytail + Y[wr * kWave];
+                }
+                // 0 < r/wt < 1  <=>  0 < r < wt  for finite wt > 0.
+                if (r > 0.0f && r < wt) {
+                    const float num = s * s;
+                    const float den = r * (wt - r);
+                    const double lhs = static_cast<double>(num);
+                    const double rhs = mid * static_cast<double>(den);
+                    bool up = lhs > rhs;
+                    if (lhs == rhs) {
+                        up = (__float_as_uint(succ_pos(best)) & 1u) == 0u;
+                    }
+                    if (!(den > 0.0f)) up = (num / den) > best;
+                    if (up) {
+                        const float v = num / den;
+                        if (v > best) {
+                            best = v;
+                            mid = bls_midpoint_up(best);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = best;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+// ---------------------------------------------------------------------------
+static inline dim3 grid_of(const GridArgs& g) { return dim3(static_cast<uint32_t>(g.n_blocks)); }
+
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, hipStream_t s) {
+    const bool ext1 = a.phase_ext == 1 && a.mag_ext == 1;
+    if (g.has_pdt) {
+        if (ext1) hipLaunchKernelGGL((ce_kernel<true, true>), grid_of(g), dim3(kWave), lds, s, g, a);
+        else hipLaunchKernelGGL((ce_kernel<true, false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    } else {
+        if (ext1) hipLaunchKernelGGL((ce_kernel<false, true>), grid_of(g), dim3(kWave), lds, s, g, a);
+        else hipLaunchKernelGGL((ce_kernel<false, false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, hipStream_t s) {
+    const bool ext1 = a.ext == 1;
+    if (g.has_pdt) {
+        if (ext1) hipLaunchKernelGGL((aov_kernel<true, true>), grid_of(g), dim3(kWave), lds, s, g, a);
+        else hipLaunchKernelGGL((aov_kernel<true, false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    } else {
+        if (ext1) hipLaunchKernelGGL((aov_kernel<false, true>), grid_of(g), dim3(kWave), lds, s, g, a);
+        else hipLaunchKernelGGL((aov_kernel<false, false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s) {
+    if (g.has_pdt) {
+        if (a.fast) hipLaunchKernelGGL((ls_kernel<true, true>), grid_of(g), dim3(kWave), 0, s, g, a);
+        else hipLaunchKernelGGL((ls_kernel<true, false>), grid_of(g), dim3(kWave), 0, s, g, a);
+    } else {
+        if (a.fast) hipLaunchKernelGGL((ls_kernel<false, true>), grid_of(g), dim3(kWave), 0, s, g, a);
+        else hipLaunchKernelGGL((ls_kernel<false, false>), grid_of(g), dim3(kWave), 0, s, g, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, hipStream_t s) {
+    if (g.has_pdt) hipLaunchKernelGGL((fpw_kernel<true>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((fpw_kernel<false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, hipStream_t s) {
+    if (g.has_pdt) hipLaunchKernelGGL((bls_kernel<true>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((bls_kernel<false>), grid_of(g), dim3(kWave), lds, s, g, a);
+    return hipGetLastError();
+}
+
+}  // namespace pf

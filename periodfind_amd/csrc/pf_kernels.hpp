@@ -1,0 +1,86 @@
+// Kernel argument blocks and launcher declarations (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+constexpr int kWaveHost = 64;  // wave64: trials per workgroup
+
+// Shared by every grid kernel: the curve set (CSR) and the trial grid.
+struct GridArgs {
+    const int64_t* off;     // [n_curves+1] point offsets
+    const double* t64;      // [npts] times widened to f64
+    const double* tt;       // [npts] t*t in f64 (read only when has_pdt)
+    const float* periods;   // [np]
+    const float* pdts;      // [nd]
+    int64_t nd;             // n_pdts
+    int64_t T;              // trials per curve = np*nd
+    int64_t tblocks;        // ceil(T/64)
+    uint64_t n_blocks;      // n_curves*tblocks workgroups
+    float* out;             // [n_curves][T]
+    int has_pdt;            // any nonzero period_dt
+};
+
+struct CeArgs {
+    const uint32_t* mrow;   // [npts] ((mag bin) % n_mag) * 256: byte offset of the column
+    int n_phase, n_mag, phase_ext, mag_ext;
+};
+
+struct AovArgs {
+    const float* y;         // [npts]
+    const float* yy;        // [npts] y*y
+    const float* avg;       // [n_curves] serial f32 mean
+    int n_bins, ext;
+};
+
+struct LsArgs {
+    const float* y;         // [npts]
+    int fast;               // hardware sin/cos on revolutions
+};
+
+struct FpwArgs {
+    const float* w;         // [npts] 1/(e*e)
+    const float* wy;        // [npts] w*y
+    int n_bins;
+};
+
+struct BlsArgs {
+    const float* w;         // [npts] 1/(e*e)
+    const float* yw;        // [npts] w*(y - weighted mean)
+    int n_bins, d_lo, d_hi;
+};
+
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, hipStream_t s);
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, hipStream_t s);
+hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s);
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, hipStream_t s);
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, hipStream_t s);
+
+// LDS bytes per wave of each binned kernel.
+inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }
+inline size_t lds_aov(int nb) { return size_t(3) * nb * 64 * 4; }
+inline size_t lds_fpw(int nb) { return size_t(2) * nb * 64 * 4; }
+inline size_t lds_bls(int nb) { return size_t(2) * (nb + 1) * 64 * 4; }
+
+// Per-point / per-curve preparation kernels (pf_prep.hip).
+hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t npts,
+                              hipStream_t s);
+hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, int64_t npts, int n_mag,
+                              hipStream_t s);
+hipError_t launch_square(const float* y, float* yy, int64_t npts, hipStream_t s);
+hipError_t launch_inv_var(const float* e, float* w, int64_t npts, hipStream_t s);
+hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, hipStream_t s);
+// Serial per-curve reductions in the reference's order (one lane per curve).
+hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int64_t n_curves,
+                             hipStream_t s);
+hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,
+                              float* yw, int64_t n_curves, int64_t max_len, hipStream_t s);
+
+// Peak selection (pf_peaks.hip).
+hipError_t launch_peaks(const float* data, int64_t n_curves, int64_t T, int64_t n_peaks,
+                        int64_t min_distance, int use_max, int64_t* idx, float* val,
+                        hipStream_t s);
+
+}  // namespace pf

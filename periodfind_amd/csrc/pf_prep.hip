@@ -1,0 +1,171 @@
+// Per-point and per-curve preparation of a curve set (gfx950).
+//
+// Element-wise conversions run one thread per point.  The per-curve serial
+// reductions (AOV mean, BLS weighted mean) must reproduce the reference's
+// serial f32 summation order, so they run one lane per curve; a curve set
+// computes them once and caches the result.
+#include "pf_device.hpp"
+#include "pf_kernels.hpp"
+
+namespace pf {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline dim3 blocks_for(int64_t n) {
+    return dim3(static_cast<uint32_t>((n + kBlock - 1) / kBlock));
+}
+
+__global__ void widen_times_kernel(const float* __restrict__ t, double* __restrict__ t64,
+                                   double* __restrict__ tt, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double d = static_cast<double>(t[i]);
+    t64[i] = d;
+    if (tt) tt[i] = d * d;  // exact: 24x24-bit product
+}
+
+// `(m / fl(1/n_mag)) as usize` (rust/src/ce.rs:53; saturating, NaN -> 0),
+// reduced mod n_mag so that (mb + mo) % n_mag can be formed by subtraction,
+// stored as the byte offset of that column in the CE kernel's lane-interleaved
+// histogram (one 64-lane row of u32 per bin).
+__global__ void ce_mag_rows_kernel(const float* __restrict__ y, uint32_t* __restrict__ mrow,
+                                   int64_t n, int n_mag) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float bsz = 1.0f / static_cast<float>(n_mag);
+    const float q = y[i] / bsz;
+    uint64_t mb;
+    if (!(q > 0.0f)) mb = 0;
+    else if (q >= 18446744073709551616.0f) mb = UINT64_MAX;
+    else mb = static_cast<uint64_t>(q);
+    mrow[i] = static_cast<uint32_t>(mb % static_cast<uint64_t>(n_mag)) * (kWave * 4u);
+}
+
+__global__ void square_kernel(const float* __restrict__ y, float* __restrict__ yy, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    yy[i] = y[i] * y[i];
+}
+
+// w = 1/(e*e) (rust/src/fpw.rs:26, bls.rs:34); e == NULL means ones.
+__global__ void inv_var_kernel(const float* __restrict__ e, float* __restrict__ w, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (e) {
+        const float ee = e[i] * e[i];
+        w[i] = 1.0f / ee;
+    } else {
+        w[i] = 1.0f;
+    }
+}
+
+__global__ void mul_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                           float* __restrict__ out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    out[i] = a[i] * b[i];
+}
+
+// rust/src/aov.rs:27: sum::<f32>() / length, serial.
+__global__ void curve_mean_kernel(const int64_t* __restrict__ off, const float* __restrict__ y,
+                                  float* __restrict__ avg, int64_t n_curves) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (c >= n_curves) return;
+    const int64_t a = off[c], b = off[c + 1];
+    float s = 0.0f;
+    for (int64_t i = a; i < b; ++i) s += y[i];
+    avg[c] = s / static_cast<float>(b - a);
+}
+
+// rust/src/bls.rs:35-47: total_w and sum(w*y) serial; mean; yw = w*(y - mean).
+__global__ void bls_mean_kernel(const int64_t* __restrict__ off, const float* __restrict__ y,
+                                const float* __restrict__ w, float* __restrict__ mean,
+                                int64_t n_curves) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (c >= n_curves) return;
+    const int64_t a = off[c], b = off[c + 1];
+    float tw = 0.0f, sy = 0.0f;
+    for (int64_t i = a; i < b; ++i) {
+        tw += w[i];
+        const float p = w[i] * y[i];
+        sy += p;
+    }
+    mean[c] = (tw > 0.0f) ? sy / tw : 0.0f;
+}
+
+// `bx` blocks per curve: yw[i] = w[i] * (y[i] - mean[c]).
+__global__ void bls_center_kernel(const int64_t* __restrict__ off, const float* __restrict__ y,
+                                  const float* __restrict__ w, const float* __restrict__ mean,
+                                  float* __restrict__ yw, int64_t bx) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) / bx;
+    const int64_t part = static_cast<int64_t>(blockIdx.x) % bx;
+    const int64_t a = off[c], b = off[c + 1];
+    const float m = mean[c];
+    for (int64_t i = a + part * kBlock + threadIdx.x; i < b; i += bx * kBlock) {
+        const float d = y[i] - m;
+        yw[i] = w[i] * d;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t npts,
+                              hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_times_kernel, blocks_for(npts), dim3(kBlock), 0, s, t, t64, tt, npts);
+    return hipGetLastError();
+}
+
+hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, int64_t npts, int n_mag,
+                              hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ce_mag_rows_kernel, blocks_for(npts), dim3(kBlock), 0, s, y, mrow, npts,
+                       n_mag);
+    return hipGetLastError();
+}
+
+hipError_t launch_square(const float* y, float* yy, int64_t npts, hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(square_kernel, blocks_for(npts), dim3(kBlock), 0, s, y, yy, npts);
+    return hipGetLastError();
+}
+
+hipError_t launch_inv_var(const float* e, float* w, int64_t npts, hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(inv_var_kernel, blocks_for(npts), dim3(kBlock), 0, s, e, w, npts);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mul_kernel, blocks_for(npts), dim3(kBlock), 0, s, a, b, out, npts);
+    return hipGetLastError();
+}
+
+hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int64_t n_curves,
+                             hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(curve_mean_kernel, dim3(static_cast<uint32_t>((n_curves + 63) / 64)),
+                       dim3(64), 0, s, off, y, avg, n_curves);
+    return hipGetLastError();
+}
+
+// Weighted mean per curve (serial) into `mean`, then yw = w*(y - mean).
+hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,
+                              float* yw, int64_t n_curves, int64_t max_len, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bls_mean_kernel, dim3(static_cast<uint32_t>((n_curves + 63) / 64)),
+                       dim3(64), 0, s, off, y, w, mean, n_curves);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int64_t bx = (max_len + kBlock - 1) / kBlock;
+    if (bx < 1) bx = 1;
+    if (bx > 64) bx = 64;
+    hipLaunchKernelGGL(bls_center_kernel, dim3(static_cast<uint32_t>(bx * n_curves)),
+                       dim3(kBlock), 0, s, off, y, w, mean, yw, bx);
+    return hipGetLastError();
+}
+
+}  // namespace pf
