@@ -1,0 +1,207 @@
+# cython: language_level=3, boundscheck=False, wraparound=False
+"""Cython binding over the C ABI of libpfhip.so (include/periodfind_amd.h).
+
+Counterpart of the reference's per-algorithm `cdef extern from "./cuda/X.h"`
+bindings (periodfind/ce.pyx:25-39 and siblings): this module calls the
+C entry points with the GIL released, on CSR-packed float32 buffers, and
+turns status codes into RuntimeError.  Validation, magnitude preparation and
+result objects live in periodfind_amd.gpu.
+"""
+
+import threading
+
+import numpy as np
+
+cimport numpy as cnp
+from libc.stdint cimport int64_t
+
+cnp.import_array()
+
+cdef extern from "periodfind_amd.h":
+    ctypedef struct pf_ctx:
+        pass
+    ctypedef struct pf_params:
+        int algo
+        int n_phase
+        int n_mag
+        int phase_ext
+        int mag_ext
+        int n_bins
+        float qmin
+        float qmax
+
+    int PF_OK
+    int pf_device_count(int* count)
+    const char* pf_last_error()
+    int pf_ctx_create(pf_ctx** ctx, int device)
+    void pf_ctx_destroy(pf_ctx* ctx) nogil
+    int pf_ce_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* periods, int64_t n_periods,
+                   const float* period_dts, int64_t n_pdts, int n_phase, int n_mag,
+                   int phase_ext, int mag_ext, float* out) nogil
+    int pf_aov_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                    const float* y, const float* periods, int64_t n_periods,
+                    const float* period_dts, int64_t n_pdts, int n_phase, int phase_ext,
+                    float* out) nogil
+    int pf_ls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* periods, int64_t n_periods,
+                   const float* period_dts, int64_t n_pdts, float* out) nogil
+    int pf_fpw_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                    const float* y, const float* err, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_bins,
+                    float* out) nogil
+    int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                    const float* y, const float* err, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_bins,
+                    float qmin, float qmax, float* out) nogil
+    int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
+                 const int64_t* offsets, const float* t, const float* y, const float* err,
+                 const float* periods, int64_t n_periods, const float* period_dts,
+                 int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                 float* val) nogil
+    int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
+                          int64_t n_peaks, int64_t min_distance, int use_max, int64_t* idx,
+                          float* val) nogil
+    int pf_last_kernel_ms(pf_ctx* ctx, float* ms)
+
+ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
+
+
+def device_count():
+    cdef int n = 0
+    pf_device_count(&n)
+    return n
+
+
+cdef _raise(int code):
+    msg = pf_last_error().decode("utf-8", "replace")
+    raise RuntimeError(f"periodfind_amd: {msg} (code {code})")
+
+
+cdef const float* _fptr(cnp.ndarray a):
+    if a is None:
+        return NULL
+    return <const float*> cnp.PyArray_DATA(a)
+
+
+cdef class Context:
+    """A device context (stream, reusable device buffers) on one GPU.
+    Calls on one Context are serialised by a lock."""
+
+    cdef pf_ctx* ctx
+    cdef object lock
+    cdef readonly int device
+
+    def __cinit__(self, int device=0):
+        self.ctx = NULL
+        self.device = device
+        self.lock = threading.Lock()
+        cdef int rc = pf_ctx_create(&self.ctx, device)
+        if rc != 0:
+            _raise(rc)
+
+    def __dealloc__(self):
+        if self.ctx != NULL:
+            with nogil:
+                pf_ctx_destroy(self.ctx)
+            self.ctx = NULL
+
+    def last_kernel_ms(self):
+        cdef float ms = 0.0
+        pf_last_kernel_ms(self.ctx, &ms)
+        return ms
+
+    def grid(self, str algo, dict p, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+             cnp.ndarray periods, cnp.ndarray pdts, cnp.ndarray out):
+        """Full periodogram into out[n_curves, n_periods, n_pdts] (float32)."""
+        cdef int64_t C = off.shape[0] - 1
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef float* op = <float*> cnp.PyArray_DATA(out)
+        cdef int rc = 0
+        cdef int a = ALGOS[algo]
+        cdef int i0 = 0, i1 = 0, i2 = 0, i3 = 0
+        cdef float f0 = 0.0, f1 = 0.0
+        if a == 0:
+            i0, i1, i2, i3 = p["n_phase"], p["n_mag"], p["phase_ext"], p["mag_ext"]
+        elif a == 1:
+            i0, i1 = p["n_phase"], p["phase_ext"]
+        elif a == 3:
+            i0 = p["n_bins"]
+        elif a == 4:
+            i0, f0, f1 = p["n_bins"], p["qmin"], p["qmax"]
+        with self.lock:
+            with nogil:
+                if a == 0:
+                    rc = pf_ce_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, i0, i1, i2, i3, op)
+                elif a == 1:
+                    rc = pf_aov_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, i0, i1, op)
+                elif a == 2:
+                    rc = pf_ls_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, op)
+                elif a == 3:
+                    rc = pf_fpw_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
+                else:
+                    rc = pf_bls_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, f0, f1,
+                                     op)
+        if rc != 0:
+            _raise(rc)
+        return out
+
+    def peaks(self, str algo, dict p, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+              cnp.ndarray periods, cnp.ndarray pdts, int64_t n_peaks, int64_t min_distance):
+        """Device top-N peaks -> (idx int64[C, n_peaks], val float32[C, n_peaks])."""
+        cdef int64_t C = off.shape[0] - 1
+        cdef pf_params prm
+        prm.algo = ALGOS[algo]
+        prm.n_phase = p.get("n_phase", 10)
+        prm.n_mag = p.get("n_mag", 10)
+        prm.phase_ext = p.get("phase_ext", 1)
+        prm.mag_ext = p.get("mag_ext", 1)
+        prm.n_bins = p.get("n_bins", 10)
+        prm.qmin = p.get("qmin", 0.01)
+        prm.qmax = p.get("qmax", 0.5)
+        idx = np.full((C, n_peaks), -1, dtype=np.int64)
+        val = np.zeros((C, n_peaks), dtype=np.float32)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_peaks(self.ctx, &prm, C, o, tp, yp, ep, pp, NP, dp, ND, n_peaks,
+                              min_distance, ip, vp)
+        if rc != 0:
+            _raise(rc)
+        return idx, val
+
+    def find_top_peaks(self, cnp.ndarray data, int64_t n_peaks, int64_t min_distance,
+                       bint use_max):
+        """Peaks of a host periodogram data[C, ...] (flattened per curve)."""
+        cdef int64_t C = data.shape[0]
+        cdef int64_t T = data.size // C if C > 0 else 0
+        idx = np.full((C, n_peaks), -1, dtype=np.int64)
+        val = np.zeros((C, n_peaks), dtype=np.float32)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef const float* dp = _fptr(data)
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_find_top_peaks(self.ctx, dp, C, T, n_peaks, min_distance,
+                                       1 if use_max else 0, ip, vp)
+        if rc != 0:
+            _raise(rc)
+        return idx, val

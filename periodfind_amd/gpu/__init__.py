@@ -1,0 +1,244 @@
+"""GPU (HIP, MI355X) implementations of the period-finding algorithms.
+
+Drop-in counterparts of periodfind.gpu.{ConditionalEntropy, AOV, LombScargle,
+FPW, BoxLeastSquares} (periodfind/{ce,aov,ls,fpw,bls}.pyx), with the CPU
+classes' superset signature (periodfind/cpu/__init__.py:73-672): besides
+output='stats' | 'periodogram', output='peaks' returns the top `n_peaks`
+local extrema per curve (peaks.rs semantics), selected on the device.
+
+Each calc() validates inputs with the reference's messages, prepares
+magnitudes as the reference does (CE normalizes, LS centers), CSR-packs the
+curves and calls the HIP library through the Cython binding.
+"""
+
+import threading
+
+import numpy as np
+
+from periodfind_amd import _pfhip
+from periodfind_amd._utils import ensure_float32, prepare_magnitudes, validate_errs, validate_inputs
+from periodfind_amd.results import Periodogram, Statistics
+
+N_PEAKS_DEFAULT = 32
+MIN_DISTANCE_DEFAULT = 1
+
+_ctx_lock = threading.Lock()
+_contexts = {}
+
+
+def context(device=0):
+    """The process-wide HIP context of `device` (created on first use)."""
+    with _ctx_lock:
+        c = _contexts.get(device)
+        if c is None:
+            c = _pfhip.Context(device)
+            _contexts[device] = c
+        return c
+
+
+def pack(arrays):
+    """CSR-pack a list of 1-D float32 arrays -> (offsets int64[n+1], flat)."""
+    n = len(arrays)
+    off = np.zeros(n + 1, dtype=np.int64)
+    if n:
+        np.cumsum([a.shape[0] for a in arrays], out=off[1:])
+        flat = np.concatenate(arrays).astype(np.float32, copy=False)
+    else:
+        flat = np.zeros(0, np.float32)
+    return off, np.ascontiguousarray(flat)
+
+
+def _grid_arrays(periods, period_dts):
+    P = np.ascontiguousarray(periods, dtype=np.float32).reshape(-1)
+    D = np.ascontiguousarray(period_dts, dtype=np.float32).reshape(-1)
+    return P, D
+
+
+def _unravel_peaks(idx, val, periods, period_dts):
+    """Flat peak indices -> list (per curve) of Statistics, stopping at the
+    first -1 (periodfind/cpu/__init__.py:46-70)."""
+    n_pdts = len(period_dts)
+    out = []
+    for ci in range(idx.shape[0]):
+        row = []
+        for k in range(idx.shape[1]):
+            f = int(idx[ci, k])
+            if f < 0:
+                break
+            row.append(Statistics(params=[float(periods[f // n_pdts]), float(period_dts[f % n_pdts])],
+                                  value=float(val[ci, k]), mean=0.0, std=1.0, median=0.0, mad=1.0))
+        out.append(row)
+    return out
+
+
+class _Algorithm:
+    """Shared calc() flow; subclasses set _algo, _use_max and _params()."""
+
+    _algo = None
+    _use_max = True
+    _takes_errs = False
+    device = 0
+
+    def _params(self):
+        return {}
+
+    def _run(self, times, mags_use, errs, periods, period_dts, output, n_stats,
+             significance_type, n_peaks, min_distance):
+        P, D = _grid_arrays(periods, period_dts)
+        off, t = pack(times)
+        _, y = pack(mags_use)
+        e = pack(errs)[1] if errs is not None else None
+        ctx = context(self.device)
+        if output == "peaks":
+            idx, val = ctx.peaks(self._algo, self._params(), off, t, y, e, P, D, int(n_peaks),
+                                 int(min_distance))
+            return _unravel_peaks(idx, val, periods, period_dts)
+        if output not in ("stats", "periodogram"):
+            raise NotImplementedError(
+                f'Output type "{output}" is not implemented. '
+                f'Use "stats", "periodogram", or "peaks".')
+        out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
+        ctx.grid(self._algo, self._params(), off, t, y, e, P, D, out)
+        params = [periods, period_dts]
+        if output == "stats":
+            return [Statistics.statistics_from_data(out[i], params, self._use_max, n=n_stats,
+                                                    significance_type=significance_type)
+                    for i in range(len(times))]
+        return [Periodogram(d, params, self._use_max) for d in out]
+
+
+class ConditionalEntropy(_Algorithm):
+    """Conditional Entropy (periodfind/ce.pyx:41-80; rust/src/ce.rs).  Lower is better.
+
+    n_phase, n_mag: histogram bins; phase_bin_extent, mag_bin_extent: each
+    point also counts in the next extent-1 bins, wrapping."""
+
+    _algo = "ce"
+    _use_max = False
+
+    def __init__(self, n_phase=10, n_mag=10, phase_bin_extent=1, mag_bin_extent=1, device=0):
+        self.n_phase = n_phase
+        self.n_mag = n_mag
+        self.phase_bin_extent = phase_bin_extent
+        self.mag_bin_extent = mag_bin_extent
+        self.device = device
+
+    def _params(self):
+        return dict(n_phase=int(self.n_phase), n_mag=int(self.n_mag),
+                    phase_ext=int(self.phase_bin_extent), mag_ext=int(self.mag_bin_extent))
+
+    def calc(self, times, mags, periods, period_dts, output="stats", normalize=True,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
+class AOV(_Algorithm):
+    """Analysis of Variance F-statistic (periodfind/aov.pyx; rust/src/aov.rs).  Higher is better."""
+
+    _algo = "aov"
+
+    def __init__(self, n_phase=10, phase_bin_extent=1, device=0):
+        self.n_phase = n_phase
+        self.phase_bin_extent = phase_bin_extent
+        self.device = device
+
+    def _params(self):
+        return dict(n_phase=int(self.n_phase), phase_ext=int(self.phase_bin_extent))
+
+    def calc(self, times, mags, periods, period_dts, output="stats", normalize=False,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
+class LombScargle(_Algorithm):
+    """Lomb-Scargle power (periodfind/ls.pyx; rust/src/ls.rs).  Higher is better;
+    magnitudes are centred by default."""
+
+    _algo = "ls"
+
+    def __init__(self, device=0):
+        self.device = device
+
+    def calc(self, times, mags, periods, period_dts, output="stats", normalize=False,
+             center=True, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
+class FPW(_Algorithm):
+    """Fast Phase-folding Weighted statistic (periodfind/fpw.pyx; rust/src/fpw.rs).
+    Higher is better; errs default to ones; magnitudes are used as given."""
+
+    _algo = "fpw"
+
+    def __init__(self, n_bins=10, device=0):
+        self.n_bins = n_bins
+        self.device = device
+
+    def _params(self):
+        return dict(n_bins=int(self.n_bins))
+
+    def calc(self, times, mags, periods, period_dts, errs=None, output="stats", normalize=False,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        errs = validate_errs(errs, times)
+        return self._run(times, mags, errs, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
+class BoxLeastSquares(_Algorithm):
+    """Box Least Squares (periodfind/bls.pyx; rust/src/bls.rs).  Higher is better;
+    errs default to ones; magnitudes are used as given."""
+
+    _algo = "bls"
+
+    def __init__(self, n_bins=50, qmin=0.01, qmax=0.5, device=0):
+        self.n_bins = n_bins
+        self.qmin = qmin
+        self.qmax = qmax
+        self.device = device
+
+    def _params(self):
+        return dict(n_bins=int(self.n_bins), qmin=float(self.qmin), qmax=float(self.qmax))
+
+    def calc(self, times, mags, periods, period_dts, errs=None, output="stats", normalize=False,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        errs = validate_errs(errs, times)
+        return self._run(times, mags, errs, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
+def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTANCE_DEFAULT,
+                           use_max=True, device=0):
+    """Peaks of host periodograms data[n_curves, n_periods, n_pdts] on the device
+    (rust/src/lib.rs:509-555) -> (idx int64[C, n_peaks], val float32[C, n_peaks])."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    return context(device).find_top_peaks(data, int(n_peaks), int(min_distance), bool(use_max))
+
+
+__all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares",
+           "find_top_peaks_batched", "context"]
