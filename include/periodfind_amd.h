@@ -145,20 +145,51 @@ int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t 
                       float* val);
 
 /* ------------------------------------------------------------------------ */
-/* Device-resident path: inputs already in HBM (used by the benchmark and by */
-/* callers that keep curves on the GPU).  All pointers below are DEVICE      */
-/* pointers on the context's device; offsets are a device int64 array.       */
-/* Results are written to device memory; the call is asynchronous on the     */
-/* context stream unless `sync` is nonzero.                                  */
+/* Resident curve sets: upload once, evaluate many times.  No counterpart in  */
+/* the reference (which copies every curve on every call, cuda/ce.cu:513-518);*/
+/* used by callers that keep curves on the GPU and by the benchmark.          */
+/* The per-point arrays a statistic needs (f64 times, inverse variances,      */
+/* magnitude-bin offsets, per-curve serial means) are built on first use and  */
+/* cached with the set.                                                       */
 /* ------------------------------------------------------------------------ */
+typedef struct pf_curves pf_curves;
 
-/* Full grid, device in / device out.  err_d may be NULL (ones).
- * max_len: the largest curve length (host-known; sizes the launch). */
-int pf_grid_device(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
-                   const int64_t* offsets_d, const float* t_d, const float* y_d,
-                   const float* err_d, int64_t max_len, const float* periods_d,
-                   int64_t n_periods, const float* period_dts_d, int64_t n_pdts,
-                   float* out_d, int sync);
+/* Copy a CSR curve set (host arrays) to ctx's device.  err may be NULL (ones). */
+int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves, const int64_t* offsets,
+                     const float* t, const float* y, const float* err);
+void pf_curves_destroy(pf_curves* cv);
+
+/* Full grid over a resident set.  The periodogram stays in device memory
+ * (pf_curves_result) and is also copied to `out` when out != NULL. */
+int pf_curves_grid(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                   int64_t n_periods, const float* period_dts, int64_t n_pdts, float* out);
+
+/* Device pointer and shape of the last periodogram computed on cv. */
+int pf_curves_result(pf_curves* cv, const float** out_d, int64_t* n_curves, int64_t* T);
+
+/* Grid + on-device peak selection over a resident set (see pf_peaks). */
+int pf_curves_peaks(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts,
+                    int64_t n_peaks, int64_t min_distance, int64_t* idx, float* val);
+
+/* ------------------------------------------------------------------------ */
+/* Device 'stats' output.  Replaces the per-curve host loop over             */
+/* Statistics.statistics_from_data (periodfind/ce.pyx:189-202,               */
+/* periodfind/__init__.py:337-427): per curve, summary[c][4] = {mean, std,   */
+/* median, mad} of the whole [period][pdt] grid with numpy float32 semantics, */
+/* and the n_stats best trials (flat index, value), best first; equal values */
+/* keep index order.  fallback[c] = 1 marks a curve whose periodogram holds  */
+/* NaN: its entries are not written and the caller computes them from the   */
+/* full periodogram.  Needs n_periods*n_pdts <= 8192 (PF_E_LIMIT otherwise). */
+/* ------------------------------------------------------------------------ */
+int pf_stats(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64_t* offsets,
+             const float* t, const float* y, const float* err, const float* periods,
+             int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
+             float* summary, int64_t* idx, float* val, int* fallback);
+
+int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
+                    float* summary, int64_t* idx, float* val, int* fallback);
 
 /* Kernel time of the last grid kernel launched on ctx (HIP events), ms. */
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms);

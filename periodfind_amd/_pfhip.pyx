@@ -62,6 +62,11 @@ cdef extern from "periodfind_amd.h":
     int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
                           int64_t n_peaks, int64_t min_distance, int use_max, int64_t* idx,
                           float* val) nogil
+    int pf_stats(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
+                 const int64_t* offsets, const float* t, const float* y, const float* err,
+                 const float* periods, int64_t n_periods, const float* period_dts,
+                 int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
+                 int* fallback) nogil
     int pf_last_kernel_ms(pf_ctx* ctx, float* ms)
 
 ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
@@ -82,6 +87,19 @@ cdef const float* _fptr(cnp.ndarray a):
     if a is None:
         return NULL
     return <const float*> cnp.PyArray_DATA(a)
+
+
+cdef pf_params _params(str algo, dict p):
+    cdef pf_params prm
+    prm.algo = ALGOS[algo]
+    prm.n_phase = p.get("n_phase", 10)
+    prm.n_mag = p.get("n_mag", 10)
+    prm.phase_ext = p.get("phase_ext", 1)
+    prm.mag_ext = p.get("mag_ext", 1)
+    prm.n_bins = p.get("n_bins", 10)
+    prm.qmin = p.get("qmin", 0.01)
+    prm.qmax = p.get("qmax", 0.5)
+    return prm
 
 
 cdef class Context:
@@ -153,19 +171,42 @@ cdef class Context:
             _raise(rc)
         return out
 
+    def stats(self, str algo, dict p, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+              cnp.ndarray periods, cnp.ndarray pdts, int n_stats):
+        """Device 'stats': (summary float32[C, 4] = mean, std, median, mad;
+        idx int64[C, n_stats]; val float32[C, n_stats]; fallback int32[C])."""
+        cdef int64_t C = off.shape[0] - 1
+        cdef pf_params prm = _params(algo, p)
+        summary = np.zeros((C, 4), dtype=np.float32)
+        idx = np.zeros((C, n_stats), dtype=np.int64)
+        val = np.zeros((C, n_stats), dtype=np.float32)
+        fb = np.zeros(C, dtype=np.int32)
+        cdef float* sp = <float*> cnp.PyArray_DATA(summary)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef int* fp = <int*> cnp.PyArray_DATA(fb)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_stats(self.ctx, &prm, C, o, tp, yp, ep, pp, NP, dp, ND, n_stats, sp,
+                              ip, vp, fp)
+        if rc != 0:
+            _raise(rc)
+        return summary, idx, val, fb
+
     def peaks(self, str algo, dict p, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
               cnp.ndarray periods, cnp.ndarray pdts, int64_t n_peaks, int64_t min_distance):
         """Device top-N peaks -> (idx int64[C, n_peaks], val float32[C, n_peaks])."""
         cdef int64_t C = off.shape[0] - 1
-        cdef pf_params prm
-        prm.algo = ALGOS[algo]
-        prm.n_phase = p.get("n_phase", 10)
-        prm.n_mag = p.get("n_mag", 10)
-        prm.phase_ext = p.get("phase_ext", 1)
-        prm.mag_ext = p.get("mag_ext", 1)
-        prm.n_bins = p.get("n_bins", 10)
-        prm.qmin = p.get("qmin", 0.01)
-        prm.qmax = p.get("qmax", 0.5)
+        cdef pf_params prm = _params(algo, p)
         idx = np.full((C, n_peaks), -1, dtype=np.int64)
         val = np.zeros((C, n_peaks), dtype=np.float32)
         cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)

@@ -26,6 +26,7 @@ PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
 # Every symbol the header declares, with (restype, argtypes).
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -77,8 +78,19 @@ SIGNATURES = {
     "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
                         _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
     "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
-    "pf_grid_device": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _vp, _vp, _vp, _vp, _i64,
-                              _vp, _i64, _vp, _i64, _vp, _int]),
+    "pf_curves_create": (_int, [_vp, ctypes.POINTER(_vp), _i64, _i64p, _f32p, _f32p,
+                                _f32p_or_null]),
+    "pf_curves_destroy": (None, [_vp]),
+    "pf_curves_grid": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
+                              _f32p_or_null]),
+    "pf_curves_result": (_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64)]),
+    "pf_curves_peaks": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
+                               _i64, _i64, _i64p, _f32p]),
+    "pf_stats": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p, _f32p_or_null,
+                        _f32p, _i64, _f32p, _i64, _int, _f32p, _i64p, _f32p, _i32p]),
+    "pf_curves_stats": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
+                               _int, _f32p, _i64p, _f32p, _i32p]),
     "pf_last_kernel_ms": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
 }
 
@@ -193,6 +205,23 @@ class Context:
         check(lib().pf_peaks(self.h, ctypes.byref(prm), len(times), off, t, y, e, P, P.size, D,
                              D.size, n_peaks, min_distance, idx.reshape(-1), val.reshape(-1)))
         return idx, val
+
+    def stats(self, algo, times, mags, periods, period_dts, errs=None, n_stats=1, **p):
+        """Device 'stats' -> (summary [C,4], idx [C,n], val [C,n], fallback [C])."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        prm = make_params(algo, **p)
+        C = len(times)
+        summ = np.zeros((C, 4), np.float32)
+        idx = np.zeros((C, n_stats), np.int64)
+        val = np.zeros((C, n_stats), np.float32)
+        fb = np.zeros(C, np.int32)
+        check(lib().pf_stats(self.h, ctypes.byref(prm), C, off, t, y, e, P, P.size, D, D.size,
+                             n_stats, summ.reshape(-1), idx.reshape(-1), val.reshape(-1), fb))
+        return summ, idx, val, fb
 
     def find_top_peaks(self, data, n_peaks=32, min_distance=1, use_max=True):
         data = np.ascontiguousarray(data, dtype=np.float32)

@@ -78,6 +78,13 @@ hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int
 hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,
                               float* yw, int64_t n_curves, int64_t max_len, hipStream_t s);
 
+// Per-curve summary statistics and top-n (pf_stats.hip).
+constexpr int64_t kStatsMaxT = 8192;
+size_t stats_lds_bytes(int64_t T);
+hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_top, int use_max,
+                        float* summary, int64_t* top_idx, float* top_val, int* flags,
+                        hipStream_t s);
+
 // Peak selection (pf_peaks.hip).
 hipError_t launch_peaks(const float* data, int64_t n_curves, int64_t T, int64_t n_peaks,
                         int64_t min_distance, int use_max, int64_t* idx, float* val,

@@ -57,7 +57,7 @@ int ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return PF_OK;
     if (b.p) {
-        hipFree(b.p);
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -75,25 +75,72 @@ int ensure(DevBuf& b, size_t bytes) {
 template <typename T>
 T* as(DevBuf& b) { return static_cast<T*>(b.p); }
 
+// A curve set on the device: owned CSR storage plus the derived per-point
+// arrays each statistic needs, built on first use and kept until the inputs
+// change.  The drop-in calls reuse one set owned by the context; resident
+// sets (pf_curves) own theirs.
+struct CurveSet {
+    int64_t n_curves = 0;
+    int64_t npts = 0;
+    int64_t max_len = 0;
+    bool has_err = false;
+    DevBuf off, t, y, e;
+    // Derived, with validity flags.
+    DevBuf t64, tt;
+    bool t64_ok = false, tt_ok = false;
+    DevBuf ce_mrow;
+    int ce_nmag = -1;
+    DevBuf aov_yy, aov_avg;
+    bool aov_ok = false;
+    DevBuf w;  // 1/(e*e) (FPW and BLS)
+    bool w_ok = false;
+    DevBuf fpw_wy;
+    bool fpw_ok = false;
+    DevBuf bls_yw, bls_mean;
+    bool bls_ok = false;
+    // Last periodogram.
+    DevBuf out;
+    int64_t out_T = 0;
+
+    void invalidate() {
+        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = false;
+        ce_nmag = -1;
+        out_T = 0;
+    }
+    void release() {
+        DevBuf* bufs[] = {&off, &t, &y, &e, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
+                          &fpw_wy, &bls_yw, &bls_mean, &out};
+        for (DevBuf* b : bufs)
+            if (b->p) {
+                (void)hipFree(b->p);
+                b->p = nullptr;
+                b->cap = 0;
+            }
+        invalidate();
+    }
+};
+
 }  // namespace
 
 struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
     float last_ms = 0.0f;
     size_t lds_max = 65536;
-    // Inputs.
-    DevBuf off, t, y, e;
-    // Derived per-point arrays.
-    DevBuf t64, tt, a0, a1, mrow;
-    // Per-curve scalars.
-    DevBuf cs0;
+    CurveSet scratch;  // curve set of the drop-in calls
     // Trial grid (cached by content).
     DevBuf periods, pdts;
     std::vector<float> periods_h, pdts_h;
-    // Output staging.
-    DevBuf out, pidx, pval;
+    bool has_pdt = false;
+    // Peak / stats outputs.
+    DevBuf pidx, pval, summ, flags;
+};
+
+struct pf_curves {
+    pf_ctx* ctx = nullptr;
+    CurveSet cs;
 };
 
 namespace {
@@ -154,16 +201,21 @@ void bls_duration_range(int nb, float qmin, float qmax, int* lo, int* hi) {
     *hi = static_cast<int>(b);
 }
 
-int upload_grid(pf_ctx* ctx, const float* periods, int64_t np, const float* pdts, int64_t nd,
-                bool* has_pdt) {
+int check_grid(const float* periods, int64_t np, const float* pdts, int64_t nd) {
+    if (np < 0 || nd < 0) return fail(PF_E_ARG, "negative grid size");
+    if ((np > 0 && !periods) || (nd > 0 && !pdts)) return fail(PF_E_ARG, "grid pointer is NULL");
+    return PF_OK;
+}
+
+int upload_grid(pf_ctx* ctx, const float* periods, int64_t np, const float* pdts, int64_t nd) {
     bool same = static_cast<int64_t>(ctx->periods_h.size()) == np &&
                 static_cast<int64_t>(ctx->pdts_h.size()) == nd &&
                 std::memcmp(ctx->periods_h.data(), periods, np * sizeof(float)) == 0 &&
                 std::memcmp(ctx->pdts_h.data(), pdts, nd * sizeof(float)) == 0;
-    *has_pdt = false;
-    for (int64_t i = 0; i < nd; ++i)
-        if (pdts[i] != 0.0f) *has_pdt = true;
     if (same) return PF_OK;
+    ctx->has_pdt = false;
+    for (int64_t i = 0; i < nd; ++i)
+        if (pdts[i] != 0.0f) ctx->has_pdt = true;
     PF_TRY(ensure(ctx->periods, np * sizeof(float)));
     PF_TRY(ensure(ctx->pdts, nd * sizeof(float)));
     PF_HIP(hipMemcpyAsync(ctx->periods.p, periods, np * sizeof(float), hipMemcpyHostToDevice,
@@ -176,17 +228,6 @@ int upload_grid(pf_ctx* ctx, const float* periods, int64_t np, const float* pdts
     return PF_OK;
 }
 
-// Curve set resident on the device for one call.
-struct Curves {
-    int64_t n_curves = 0;
-    int64_t npts = 0;
-    int64_t max_len = 0;
-    const int64_t* off = nullptr;
-    const float* t = nullptr;
-    const float* y = nullptr;
-    const float* e = nullptr;  // may be null (ones)
-};
-
 int validate_offsets(int64_t n_curves, const int64_t* off, int64_t* npts, int64_t* max_len) {
     if (n_curves < 0) return fail(PF_E_ARG, "n_curves < 0");
     if (!off) return fail(PF_E_ARG, "offsets is NULL");
@@ -194,8 +235,9 @@ int validate_offsets(int64_t n_curves, const int64_t* off, int64_t* npts, int64_
     int64_t mx = 0;
     for (int64_t c = 0; c < n_curves; ++c) {
         int64_t len = off[c + 1] - off[c];
-        if (len < 0) return fail(PF_E_ARG, "offsets must be non-decreasing (curve %lld)",
-                                 static_cast<long long>(c));
+        if (len < 0)
+            return fail(PF_E_ARG, "offsets must be non-decreasing (curve %lld)",
+                        static_cast<long long>(c));
         if (len > mx) mx = len;
     }
     *npts = off[n_curves];
@@ -203,124 +245,167 @@ int validate_offsets(int64_t n_curves, const int64_t* off, int64_t* npts, int64_
     return PF_OK;
 }
 
-int upload_curves(pf_ctx* ctx, int64_t n_curves, const int64_t* off, const float* t,
-                  const float* y, const float* e, Curves* cv) {
-    PF_TRY(validate_offsets(n_curves, off, &cv->npts, &cv->max_len));
-    if (cv->npts > 0 && (!t || !y)) return fail(PF_E_ARG, "t and y must not be NULL");
-    cv->n_curves = n_curves;
-    const size_t nb = static_cast<size_t>(cv->npts) * sizeof(float);
-    PF_TRY(ensure(ctx->off, (n_curves + 1) * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->t, nb));
-    PF_TRY(ensure(ctx->y, nb));
-    PF_HIP(hipMemcpyAsync(ctx->off.p, off, (n_curves + 1) * sizeof(int64_t),
-                          hipMemcpyHostToDevice, ctx->stream));
+// Copy host CSR arrays into cs (device), invalidating derived arrays.
+int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* off,
+                  const float* t, const float* y, const float* e) {
+    int64_t npts = 0, mx = 0;
+    PF_TRY(validate_offsets(n_curves, off, &npts, &mx));
+    if (npts > 0 && (!t || !y)) return fail(PF_E_ARG, "t and y must not be NULL");
+    cs.invalidate();
+    cs.n_curves = n_curves;
+    cs.npts = npts;
+    cs.max_len = mx;
+    cs.has_err = e != nullptr;
+    const size_t nb = static_cast<size_t>(npts) * sizeof(float);
+    PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
+    PF_TRY(ensure(cs.t, nb));
+    PF_TRY(ensure(cs.y, nb));
+    PF_HIP(hipMemcpyAsync(cs.off.p, off, (n_curves + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                          ctx->stream));
     if (nb) {
-        PF_HIP(hipMemcpyAsync(ctx->t.p, t, nb, hipMemcpyHostToDevice, ctx->stream));
-        PF_HIP(hipMemcpyAsync(ctx->y.p, y, nb, hipMemcpyHostToDevice, ctx->stream));
+        PF_HIP(hipMemcpyAsync(cs.t.p, t, nb, hipMemcpyHostToDevice, ctx->stream));
+        PF_HIP(hipMemcpyAsync(cs.y.p, y, nb, hipMemcpyHostToDevice, ctx->stream));
     }
-    cv->off = as<int64_t>(ctx->off);
-    cv->t = as<float>(ctx->t);
-    cv->y = as<float>(ctx->y);
-    cv->e = nullptr;
     if (e) {
-        PF_TRY(ensure(ctx->e, nb));
-        if (nb) PF_HIP(hipMemcpyAsync(ctx->e.p, e, nb, hipMemcpyHostToDevice, ctx->stream));
-        cv->e = as<float>(ctx->e);
+        PF_TRY(ensure(cs.e, nb));
+        if (nb) PF_HIP(hipMemcpyAsync(cs.e.p, e, nb, hipMemcpyHostToDevice, ctx->stream));
     }
     return PF_OK;
 }
 
-// Prepare derived arrays and launch the grid kernel; out_d is [n_curves][T].
-int run_grid(pf_ctx* ctx, const Params& p, const Curves& cv, int64_t np, int64_t nd,
-             bool has_pdt, float* out_d) {
+// Build (or reuse) the derived arrays params.algo needs.
+int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = cs.npts;
+    const float* y = as<float>(cs.y);
+    const float* e = cs.has_err ? as<float>(cs.e) : nullptr;
+    const int64_t* off = as<int64_t>(cs.off);
+    if (!cs.t64_ok || (has_pdt && !cs.tt_ok)) {
+        PF_TRY(ensure(cs.t64, n * sizeof(double)));
+        if (has_pdt) PF_TRY(ensure(cs.tt, n * sizeof(double)));
+        PF_HIP(pf::launch_widen_times(as<float>(cs.t), as<double>(cs.t64),
+                                      has_pdt ? as<double>(cs.tt) : nullptr, n, s));
+        cs.t64_ok = true;
+        cs.tt_ok = cs.tt_ok || has_pdt;
+    }
+    switch (p.algo) {
+        case PF_ALGO_CE:
+            if (cs.ce_nmag != p.n_mag) {
+                PF_TRY(ensure(cs.ce_mrow, n * sizeof(uint32_t)));
+                PF_HIP(pf::launch_ce_mag_rows(y, as<uint32_t>(cs.ce_mrow), n, p.n_mag, s));
+                cs.ce_nmag = p.n_mag;
+            }
+            break;
+        case PF_ALGO_AOV:
+            if (!cs.aov_ok) {
+                PF_TRY(ensure(cs.aov_yy, n * sizeof(float)));
+                PF_TRY(ensure(cs.aov_avg, cs.n_curves * sizeof(float)));
+                PF_HIP(pf::launch_square(y, as<float>(cs.aov_yy), n, s));
+                PF_HIP(pf::launch_curve_mean(off, y, as<float>(cs.aov_avg), cs.n_curves, s));
+                cs.aov_ok = true;
+            }
+            break;
+        case PF_ALGO_FPW:
+        case PF_ALGO_BLS:
+            if (!cs.w_ok) {
+                PF_TRY(ensure(cs.w, n * sizeof(float)));
+                PF_HIP(pf::launch_inv_var(e, as<float>(cs.w), n, s));
+                cs.w_ok = true;
+            }
+            if (p.algo == PF_ALGO_FPW && !cs.fpw_ok) {
+                PF_TRY(ensure(cs.fpw_wy, n * sizeof(float)));
+                PF_HIP(pf::launch_mul(as<float>(cs.w), y, as<float>(cs.fpw_wy), n, s));
+                cs.fpw_ok = true;
+            }
+            if (p.algo == PF_ALGO_BLS && !cs.bls_ok) {
+                PF_TRY(ensure(cs.bls_yw, n * sizeof(float)));
+                PF_TRY(ensure(cs.bls_mean, cs.n_curves * sizeof(float)));
+                PF_HIP(pf::launch_bls_prepare(off, y, as<float>(cs.w), as<float>(cs.bls_mean),
+                                              as<float>(cs.bls_yw), cs.n_curves, cs.max_len, s));
+                cs.bls_ok = true;
+            }
+            break;
+        default:
+            break;
+    }
+    return PF_OK;
+}
+
+// Launch the grid kernel of params.algo over cs into cs.out ([n_curves][T]).
+// The kernel is bracketed by the context's timing events.
+int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd) {
     PF_TRY(check_params(p, ctx->lds_max));
     const int64_t T = np * nd;
-    if (cv.n_curves == 0 || T == 0) return PF_OK;
+    ctx->timed = false;
+    if (cs.n_curves == 0 || T == 0) return PF_OK;
+    const bool has_pdt = ctx->has_pdt;
+    PF_TRY(prepare(ctx, cs, p, has_pdt));
+    PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
+    cs.out_T = T;
     hipStream_t s = ctx->stream;
-    const int64_t n = cv.npts;
-
-    PF_TRY(ensure(ctx->t64, n * sizeof(double)));
-    if (has_pdt) PF_TRY(ensure(ctx->tt, n * sizeof(double)));
-    PF_HIP(pf::launch_widen_times(cv.t, as<double>(ctx->t64), has_pdt ? as<double>(ctx->tt) : nullptr,
-                                  n, s));
 
     pf::GridArgs g;
-    g.off = cv.off;
-    g.t64 = as<double>(ctx->t64);
-    g.tt = has_pdt ? as<double>(ctx->tt) : as<double>(ctx->t64);
+    g.off = as<int64_t>(cs.off);
+    g.t64 = as<double>(cs.t64);
+    g.tt = has_pdt ? as<double>(cs.tt) : as<double>(cs.t64);
     g.periods = as<float>(ctx->periods);
     g.pdts = as<float>(ctx->pdts);
     g.nd = nd;
     g.T = T;
     g.tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
-    g.n_blocks = static_cast<uint64_t>(cv.n_curves) * static_cast<uint64_t>(g.tblocks);
-    g.out = out_d;
+    g.n_blocks = static_cast<uint64_t>(cs.n_curves) * static_cast<uint64_t>(g.tblocks);
+    g.out = as<float>(cs.out);
     g.has_pdt = has_pdt ? 1 : 0;
     if (g.n_blocks > 0xffffffffull)
         return fail(PF_E_LIMIT, "grid of %llu workgroups exceeds one launch",
                     static_cast<unsigned long long>(g.n_blocks));
 
+    PF_HIP(hipEventRecord(ctx->ev0, s));
     hipError_t le = hipSuccess;
     switch (p.algo) {
         case PF_ALGO_CE: {
-            PF_TRY(ensure(ctx->mrow, n * sizeof(uint32_t)));
-            PF_HIP(pf::launch_ce_mag_rows(cv.y, as<uint32_t>(ctx->mrow), n, p.n_mag, s));
-            pf::CeArgs a{as<uint32_t>(ctx->mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            PF_HIP(hipEventRecord(ctx->ev0, s));
+            pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
             le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), s);
             break;
         }
         case PF_ALGO_AOV: {
-            PF_TRY(ensure(ctx->a0, n * sizeof(float)));
-            PF_TRY(ensure(ctx->cs0, cv.n_curves * sizeof(float)));
-            PF_HIP(pf::launch_square(cv.y, as<float>(ctx->a0), n, s));
-            PF_HIP(pf::launch_curve_mean(cv.off, cv.y, as<float>(ctx->cs0), cv.n_curves, s));
-            pf::AovArgs a{cv.y, as<float>(ctx->a0), as<float>(ctx->cs0), p.n_phase, p.phase_ext};
-            PF_HIP(hipEventRecord(ctx->ev0, s));
+            pf::AovArgs a{as<float>(cs.y), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
+                          p.n_phase, p.phase_ext};
             le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), s);
             break;
         }
         case PF_ALGO_LS: {
-            pf::LsArgs a{cv.y, std::getenv("PF_LS_PRECISE") ? 0 : 1};
-            PF_HIP(hipEventRecord(ctx->ev0, s));
+            pf::LsArgs a{as<float>(cs.y), std::getenv("PF_LS_PRECISE") ? 0 : 1};
             le = pf::launch_ls(g, a, s);
             break;
         }
         case PF_ALGO_FPW: {
-            PF_TRY(ensure(ctx->a0, n * sizeof(float)));
-            PF_TRY(ensure(ctx->a1, n * sizeof(float)));
-            PF_HIP(pf::launch_inv_var(cv.e, as<float>(ctx->a0), n, s));
-            PF_HIP(pf::launch_mul(as<float>(ctx->a0), cv.y, as<float>(ctx->a1), n, s));
-            pf::FpwArgs a{as<float>(ctx->a0), as<float>(ctx->a1), p.n_bins};
-            PF_HIP(hipEventRecord(ctx->ev0, s));
+            pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
             le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), s);
             break;
         }
         case PF_ALGO_BLS: {
-            PF_TRY(ensure(ctx->a0, n * sizeof(float)));
-            PF_TRY(ensure(ctx->a1, n * sizeof(float)));
-            PF_TRY(ensure(ctx->cs0, cv.n_curves * sizeof(float)));
-            PF_HIP(pf::launch_inv_var(cv.e, as<float>(ctx->a0), n, s));
-            PF_HIP(pf::launch_bls_prepare(cv.off, cv.y, as<float>(ctx->a0), as<float>(ctx->cs0),
-                                          as<float>(ctx->a1), cv.n_curves, cv.max_len, s));
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
-            pf::BlsArgs a{as<float>(ctx->a0), as<float>(ctx->a1), p.n_bins, lo, hi};
-            PF_HIP(hipEventRecord(ctx->ev0, s));
+            pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi};
             le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), s);
             break;
         }
     }
-    if (le != hipSuccess) return fail(PF_E_HIP, "grid kernel launch failed: %s", hipGetErrorString(le));
+    if (le != hipSuccess)
+        return fail(PF_E_HIP, "grid kernel launch failed: %s", hipGetErrorString(le));
     PF_HIP(hipEventRecord(ctx->ev1, s));
+    ctx->timed = true;
     return PF_OK;
 }
 
-int finish_timing(pf_ctx* ctx) {
-    PF_HIP(hipEventSynchronize(ctx->ev1));
-    float ms = 0.0f;
-    PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    ctx->last_ms = ms;
+int finish(pf_ctx* ctx) {
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->timed) {
+        float ms = 0.0f;
+        PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_ms = ms;
+    }
     return PF_OK;
 }
 
@@ -337,34 +422,99 @@ Params to_params(const pf_params* p) {
     return q;
 }
 
-int check_grid(const float* periods, int64_t np, const float* pdts, int64_t nd) {
-    if (np < 0 || nd < 0) return fail(PF_E_ARG, "negative grid size");
-    if ((np > 0 && !periods) || (nd > 0 && !pdts)) return fail(PF_E_ARG, "grid pointer is NULL");
-    return PF_OK;
+// Grid (+ optional copy-out) over a set already on the device.
+int grid_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, int64_t np,
+            const float* pdts, int64_t nd, float* out) {
+    PF_TRY(check_params(p, ctx->lds_max));
+    PF_TRY(check_grid(periods, np, pdts, nd));
+    if (cs.n_curves > 0 && np * nd > 0) PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
+    PF_TRY(run_grid(ctx, cs, p, np, nd));
+    const size_t ob = static_cast<size_t>(cs.n_curves) * np * nd * sizeof(float);
+    if (out && ob)
+        PF_HIP(hipMemcpyAsync(out, cs.out.p, ob, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
 }
 
-// Host in -> host out grid evaluation shared by the per-algorithm entry points.
+// Peaks over a set already on the device.
+int peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, int64_t np,
+             const float* pdts, int64_t nd, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+             float* val) {
+    PF_TRY(check_params(p, ctx->lds_max));
+    PF_TRY(check_grid(periods, np, pdts, nd));
+    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
+    if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
+    if (cs.n_curves > 0 && n_peaks > 0 && (!idx || !val)) return fail(PF_E_ARG, "idx/val is NULL");
+    const size_t pk = static_cast<size_t>(cs.n_curves) * n_peaks;
+    if (pk == 0) return PF_OK;
+    const int64_t T = np * nd;
+    if (T == 0) {
+        for (size_t i = 0; i < pk; ++i) {
+            idx[i] = -1;
+            val[i] = 0.0f;
+        }
+        return PF_OK;
+    }
+    PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
+    PF_TRY(run_grid(ctx, cs, p, np, nd));
+    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
+    PF_HIP(pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
+                            as<int64_t>(ctx->pidx), as<float>(ctx->pval), ctx->stream));
+    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return finish(ctx);
+}
+
+// 'stats' over a set already on the device.
+int stats_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, int64_t np,
+             const float* pdts, int64_t nd, int n_stats, float* summary, int64_t* idx,
+             float* val, int* fallback) {
+    PF_TRY(check_params(p, ctx->lds_max));
+    PF_TRY(check_grid(periods, np, pdts, nd));
+    const int64_t T = np * nd;
+    if (cs.n_curves == 0) return PF_OK;
+    if (T < 1 || T > pf::kStatsMaxT)
+        return fail(PF_E_LIMIT, "device stats need 1 <= n_periods*n_pdts <= %lld",
+                    static_cast<long long>(pf::kStatsMaxT));
+    if (n_stats < 1 || n_stats > T) return fail(PF_E_ARG, "n_stats must be in [1, T]");
+    if (!summary || !idx || !val || !fallback) return fail(PF_E_ARG, "NULL output");
+    if (pf::stats_lds_bytes(T) > ctx->lds_max) return fail(PF_E_LIMIT, "stats exceed LDS");
+    PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
+    PF_TRY(run_grid(ctx, cs, p, np, nd));
+    const size_t C = static_cast<size_t>(cs.n_curves);
+    PF_TRY(ensure(ctx->summ, C * 4 * sizeof(float)));
+    PF_TRY(ensure(ctx->pidx, C * n_stats * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->pval, C * n_stats * sizeof(float)));
+    PF_TRY(ensure(ctx->flags, C * sizeof(int)));
+    const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
+    PF_HIP(pf::launch_stats(as<float>(cs.out), cs.n_curves, T, n_stats, use_max,
+                            as<float>(ctx->summ), as<int64_t>(ctx->pidx), as<float>(ctx->pval),
+                            as<int>(ctx->flags), ctx->stream));
+    PF_HIP(hipMemcpyAsync(summary, ctx->summ.p, C * 4 * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, C * n_stats * sizeof(int64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, C * n_stats * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PF_HIP(hipMemcpyAsync(fallback, ctx->flags.p, C * sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return finish(ctx);
+}
+
+// Host in -> host out (the drop-in calls): upload into the context's set.
 int grid_host(pf_ctx* ctx, const Params& p, int64_t n_curves, const int64_t* off, const float* t,
               const float* y, const float* e, const float* periods, int64_t np,
               const float* pdts, int64_t nd, float* out) {
     if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
     PF_TRY(check_params(p, ctx->lds_max));
     PF_TRY(check_grid(periods, np, pdts, nd));
-    if (n_curves > 0 && !out) return fail(PF_E_ARG, "out is NULL");
+    if (n_curves > 0 && np * nd > 0 && !out) return fail(PF_E_ARG, "out is NULL");
     PF_HIP(hipSetDevice(ctx->device));
-    Curves cv;
-    PF_TRY(upload_curves(ctx, n_curves, off, t, y, e, &cv));
-    const int64_t T = np * nd;
-    if (n_curves == 0) return PF_OK;
-    if (T == 0) return PF_OK;
-    bool has_pdt = false;
-    PF_TRY(upload_grid(ctx, periods, np, pdts, nd, &has_pdt));
-    const size_t ob = static_cast<size_t>(n_curves) * T * sizeof(float);
-    PF_TRY(ensure(ctx->out, ob));
-    PF_TRY(run_grid(ctx, p, cv, np, nd, has_pdt, as<float>(ctx->out)));
-    PF_HIP(hipMemcpyAsync(out, ctx->out.p, ob, hipMemcpyDeviceToHost, ctx->stream));
-    PF_HIP(hipStreamSynchronize(ctx->stream));
-    return finish_timing(ctx);
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, off, t, y, e));
+    return grid_on(ctx, ctx->scratch, p, periods, np, pdts, nd, out);
 }
 
 }  // namespace
@@ -376,8 +526,7 @@ const char* pf_last_error(void) { return g_err.c_str(); }
 int pf_device_count(int* count) {
     if (!count) return fail(PF_E_ARG, "count is NULL");
     int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess) n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     *count = n;
     return PF_OK;
 }
@@ -407,16 +556,16 @@ int pf_ctx_create(pf_ctx** out, int device) {
 
 void pf_ctx_destroy(pf_ctx* ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->off, &ctx->t, &ctx->y, &ctx->e, &ctx->t64, &ctx->tt, &ctx->a0,
-                      &ctx->a1, &ctx->mrow, &ctx->cs0, &ctx->periods, &ctx->pdts, &ctx->out,
-                      &ctx->pidx, &ctx->pval};
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->pidx, &ctx->pval, &ctx->summ,
+                      &ctx->flags};
     for (DevBuf* b : bufs)
-        if (b->p) hipFree(b->p);
-    if (ctx->ev0) hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) hipEventDestroy(ctx->ev1);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (b->p) (void)hipFree(b->p);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -468,37 +617,14 @@ int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64
              int64_t n_periods, const float* period_dts, int64_t n_pdts, int64_t n_peaks,
              int64_t min_distance, int64_t* idx, float* val) {
     if (!ctx || !params) return fail(PF_E_ARG, "ctx/params is NULL");
-    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
-    if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
     Params p = to_params(params);
     PF_TRY(check_params(p, ctx->lds_max));
     PF_TRY(check_grid(periods, n_periods, period_dts, n_pdts));
-    if (n_curves > 0 && n_peaks > 0 && (!idx || !val)) return fail(PF_E_ARG, "idx/val is NULL");
+    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
     PF_HIP(hipSetDevice(ctx->device));
-    Curves cv;
-    PF_TRY(upload_curves(ctx, n_curves, offsets, t, y, err, &cv));
-    if (n_curves == 0 || n_peaks == 0) return PF_OK;
-    const int64_t T = n_periods * n_pdts;
-    const size_t pk = static_cast<size_t>(n_curves) * n_peaks;
-    if (T == 0) {
-        for (size_t i = 0; i < pk; ++i) { idx[i] = -1; val[i] = 0.0f; }
-        return PF_OK;
-    }
-    bool has_pdt = false;
-    PF_TRY(upload_grid(ctx, periods, n_periods, period_dts, n_pdts, &has_pdt));
-    PF_TRY(ensure(ctx->out, static_cast<size_t>(n_curves) * T * sizeof(float)));
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
-    PF_TRY(run_grid(ctx, p, cv, n_periods, n_pdts, has_pdt, as<float>(ctx->out)));
-    const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
-    PF_HIP(pf::launch_peaks(as<float>(ctx->out), n_curves, T, n_peaks, min_distance, use_max,
-                            as<int64_t>(ctx->pidx), as<float>(ctx->pval), ctx->stream));
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipStreamSynchronize(ctx->stream));
-    return finish_timing(ctx);
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
+    return peaks_on(ctx, ctx->scratch, p, periods, n_periods, period_dts, n_pdts, n_peaks,
+                    min_distance, idx, val);
 }
 
 int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
@@ -510,39 +636,107 @@ int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t 
     if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
     if (n_curves == 0 || n_peaks == 0) return PF_OK;
     if (!idx || !val || (T > 0 && !data)) return fail(PF_E_ARG, "NULL pointer");
-    PF_HIP(hipSetDevice(ctx->device));
     const size_t pk = static_cast<size_t>(n_curves) * n_peaks;
-    const size_t db = static_cast<size_t>(n_curves) * T * sizeof(float);
-    PF_TRY(ensure(ctx->out, db));
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
-    if (db) PF_HIP(hipMemcpyAsync(ctx->out.p, data, db, hipMemcpyHostToDevice, ctx->stream));
     if (T == 0) {
-        for (size_t i = 0; i < pk; ++i) { idx[i] = -1; val[i] = 0.0f; }
+        for (size_t i = 0; i < pk; ++i) {
+            idx[i] = -1;
+            val[i] = 0.0f;
+        }
         return PF_OK;
     }
-    PF_HIP(pf::launch_peaks(as<float>(ctx->out), n_curves, T, n_peaks, min_distance,
+    PF_HIP(hipSetDevice(ctx->device));
+    CurveSet& cs = ctx->scratch;
+    cs.invalidate();
+    const size_t db = static_cast<size_t>(n_curves) * T * sizeof(float);
+    PF_TRY(ensure(cs.out, db));
+    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    PF_HIP(hipMemcpyAsync(cs.out.p, data, db, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(pf::launch_peaks(as<float>(cs.out), n_curves, T, n_peaks, min_distance,
                             use_max ? 1 : 0, as<int64_t>(ctx->pidx), as<float>(ctx->pval),
                             ctx->stream));
     PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->stream));
-    PF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timed = false;
+    return finish(ctx);
+}
+
+int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves, const int64_t* offsets,
+                     const float* t, const float* y, const float* err) {
+    if (!ctx || !out) return fail(PF_E_ARG, "ctx/out is NULL");
+    *out = nullptr;
+    PF_HIP(hipSetDevice(ctx->device));
+    pf_curves* cv = new pf_curves();
+    cv->ctx = ctx;
+    int rc = upload_curves(ctx, cv->cs, n_curves, offsets, t, y, err);
+    if (rc == PF_OK) rc = finish(ctx);
+    if (rc != PF_OK) {
+        cv->cs.release();
+        delete cv;
+        return rc;
+    }
+    *out = cv;
     return PF_OK;
 }
 
-int pf_grid_device(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
-                   const int64_t* offsets_d, const float* t_d, const float* y_d,
-                   const float* err_d, int64_t max_len, const float* periods_d,
-                   int64_t n_periods, const float* period_dts_d, int64_t n_pdts, float* out_d,
-                   int sync) {
-    (void)max_len;
-    (void)periods_d;
-    (void)period_dts_d;
-    (void)ctx; (void)params; (void)n_curves; (void)offsets_d; (void)t_d; (void)y_d;
-    (void)err_d; (void)n_periods; (void)n_pdts; (void)out_d; (void)sync;
-    return fail(PF_E_ARG, "pf_grid_device: not implemented");
+void pf_curves_destroy(pf_curves* cv) {
+    if (!cv) return;
+    (void)hipSetDevice(cv->ctx->device);
+    (void)hipStreamSynchronize(cv->ctx->stream);
+    cv->cs.release();
+    delete cv;
+}
+
+int pf_curves_grid(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                   int64_t n_periods, const float* period_dts, int64_t n_pdts, float* out) {
+    if (!ctx || !cv || !params) return fail(PF_E_ARG, "NULL argument");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    return grid_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts, n_pdts, out);
+}
+
+int pf_curves_result(pf_curves* cv, const float** out_d, int64_t* n_curves, int64_t* T) {
+    if (!cv || !out_d || !n_curves || !T) return fail(PF_E_ARG, "NULL argument");
+    *out_d = as<float>(cv->cs.out);
+    *n_curves = cv->cs.n_curves;
+    *T = cv->cs.out_T;
+    return PF_OK;
+}
+
+int pf_curves_peaks(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts,
+                    int64_t n_peaks, int64_t min_distance, int64_t* idx, float* val) {
+    if (!ctx || !cv || !params) return fail(PF_E_ARG, "NULL argument");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    return peaks_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts, n_pdts,
+                    n_peaks, min_distance, idx, val);
+}
+
+int pf_stats(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64_t* offsets,
+             const float* t, const float* y, const float* err, const float* periods,
+             int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
+             float* summary, int64_t* idx, float* val, int* fallback) {
+    if (!ctx || !params) return fail(PF_E_ARG, "ctx/params is NULL");
+    Params p = to_params(params);
+    PF_TRY(check_params(p, ctx->lds_max));
+    PF_TRY(check_grid(periods, n_periods, period_dts, n_pdts));
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
+    return stats_on(ctx, ctx->scratch, p, periods, n_periods, period_dts, n_pdts, n_stats,
+                    summary, idx, val, fallback);
+}
+
+int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,
+                    int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
+                    float* summary, int64_t* idx, float* val, int* fallback) {
+    if (!ctx || !cv || !params) return fail(PF_E_ARG, "NULL argument");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    return stats_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts, n_pdts,
+                    n_stats, summary, idx, val, fallback);
 }
 
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms) {

@@ -1,0 +1,255 @@
+// On-device 'stats' output: per curve, the summary numpy computes in
+// Statistics.statistics_from_data (periodfind/__init__.py:337-427) — mean,
+// std, median, MAD over the whole [period][pdt] grid — and the top-n trials.
+//
+// Bit-compatibility with numpy float32:
+//   * np.sum / np.mean: pairwise summation (8 accumulators over leaves of at
+//     most 128 elements, halving splits rounded down to a multiple of 8)
+//     within blocks of 8192 elements, blocks accumulated in order; mean is
+//     fl(sum / n).
+//   * np.std: x = a - mean, x*x, the same summation, / n, sqrt.
+//   * np.median: middle element, or fl(fl(a + b) / 2) of the two middle ones;
+//     MAD = median(|a - median|).
+//   * top-n: the n best values; among equal values the lower flat index wins
+//     (numpy's choice among exact ties is unspecified).
+// One workgroup per curve, T <= kStatsMaxT values staged in LDS; a curve
+// containing NaN is flagged so the caller can defer to numpy.
+#include "pf_device.hpp"
+#include "pf_kernels.hpp"
+
+namespace pf {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ uint32_t order_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float key_value(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+// numpy's pairwise summation of x[0..n) (serial, one thread).
+__device__ float pairwise_sum(const float* x, int n) {
+    // Iterative post-order walk of the split tree with an explicit stack.
+    struct Frame {
+        int start, n, state;
+        float left;
+    };
+    Frame st[16];
+    int sp = 0;
+    st[0] = {0, n, 0, 0.0f};
+    float ret = 0.0f;
+    while (true) {
+        Frame& f = st[sp];
+        if (f.n <= 128) {
+            float res;
+            if (f.n < 8) {
+                res = -0.0f;
+                for (int i = 0; i < f.n; ++i) res += x[f.start + i];
+            } else {
+                float r[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) r[j] = x[f.start + j];
+                int i = 8;
+                for (; i < f.n - (f.n % 8); i += 8) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) r[j] += x[f.start + i + j];
+                }
+                res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+                for (; i < f.n; ++i) res += x[f.start + i];
+            }
+            ret = res;
+            if (sp == 0) return ret;
+            --sp;
+            continue;  // parent consumes `ret`
+        }
+        int n2 = f.n / 2;
+        n2 -= n2 % 8;
+        if (f.state == 0) {
+            f.state = 1;
+            st[sp + 1] = {f.start, n2, 0, 0.0f};
+            ++sp;
+        } else if (f.state == 1) {
+            f.left = ret;
+            f.state = 2;
+            st[sp + 1] = {f.start + n2, f.n - n2, 0, 0.0f};
+            ++sp;
+        } else {
+            ret = f.left + ret;
+            if (sp == 0) return ret;
+            --sp;
+        }
+    }
+}
+
+// np.add.reduce over a contiguous float32 run: pairwise within 8192 blocks.
+__device__ float numpy_sum(const float* x, int n) {
+    float s = 0.0f;
+    for (int b = 0; b < n; b += 8192) {
+        const int m = (n - b) < 8192 ? (n - b) : 8192;
+        s += pairwise_sum(x + b, m);
+    }
+    return s;
+}
+
+__device__ void bitonic_sort(uint32_t* k, int n2) {
+    for (int size = 2; size <= n2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += kThreads) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint32_t a = k[i], b = k[j];
+                    if ((a > b) == up) {
+                        k[i] = b;
+                        k[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Median of n sorted keys, numpy style.
+__device__ float sorted_median(const uint32_t* k, int n) {
+    if (n & 1) return key_value(k[n / 2]);
+    const float a = key_value(k[n / 2 - 1]);
+    const float b = key_value(k[n / 2]);
+    float s = -0.0f;
+    s += a;
+    s += b;
+    return s / 2.0f;
+}
+
+__global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict__ data,
+                                                         int64_t T, int n_top, int use_max,
+                                                         float* __restrict__ summary,
+                                                         int64_t* __restrict__ top_idx,
+                                                         float* __restrict__ top_val,
+                                                         int* __restrict__ flags) {
+    extern __shared__ uint32_t sm[];
+    const int n = static_cast<int>(T);
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    float* a = reinterpret_cast<float*>(sm);  // original order [n]
+    uint32_t* k = sm + n;                     // keys [n2]
+    float* d = reinterpret_cast<float*>(k);   // scratch aliasing keys
+    __shared__ float s_mean, s_med;
+    __shared__ int s_nan;
+    const int64_t c = blockIdx.x;
+    const float* row = data + c * T;
+    if (threadIdx.x == 0) s_nan = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const float v = row[i];
+        a[i] = v;
+        if (v != v) s_nan = 1;
+    }
+    __syncthreads();
+    if (s_nan) {
+        if (threadIdx.x == 0) flags[c] = 1;
+        return;
+    }
+    // mean
+    if (threadIdx.x == 0) {
+        s_mean = numpy_sum(a, n) / static_cast<float>(n);
+        flags[c] = 0;
+    }
+    __syncthreads();
+    const float mean = s_mean;
+    // std: x = a - mean; x = x * x; sum; / n; sqrt.
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const float x = a[i] - mean;
+        d[i] = x * x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float var = numpy_sum(d, n) / static_cast<float>(n);
+        summary[c * 4 + 0] = mean;
+        summary[c * 4 + 1] = __fsqrt_rn(var);
+    }
+    __syncthreads();
+    // median
+    for (int i = threadIdx.x; i < n2; i += kThreads) k[i] = i < n ? order_key(a[i]) : 0xffffffffu;
+    __syncthreads();
+    bitonic_sort(k, n2);
+    if (threadIdx.x == 0) s_med = sorted_median(k, n);
+    __syncthreads();
+    const float med = s_med;
+    // top-n threshold from the sorted keys: the n_top-th best value.
+    uint32_t thr_key = 0;
+    if (n_top > 0) thr_key = use_max ? k[n - n_top] : k[n_top - 1];
+    __syncthreads();
+    // MAD
+    for (int i = threadIdx.x; i < n2; i += kThreads)
+        k[i] = i < n ? order_key(__builtin_fabsf(a[i] - med)) : 0xffffffffu;
+    __syncthreads();
+    bitonic_sort(k, n2);
+    if (threadIdx.x == 0) {
+        summary[c * 4 + 2] = med;
+        summary[c * 4 + 3] = sorted_median(k, n);
+    }
+    // Top-n (thread 0): strictly better than the threshold value, then ties in
+    // index order, then best-first insertion sort (ties keep index order).
+    if (threadIdx.x == 0 && n_top > 0) {
+        const float thr = key_value(thr_key);
+        int64_t* oi = top_idx + c * n_top;
+        float* ov = top_val + c * n_top;
+        int cnt = 0;
+        for (int i = 0; i < n && cnt < n_top; ++i) {
+            const float v = a[i];
+            if (use_max ? (v > thr) : (v < thr)) {
+                oi[cnt] = i;
+                ov[cnt] = v;
+                ++cnt;
+            }
+        }
+        for (int i = 0; i < n && cnt < n_top; ++i) {
+            if (a[i] == thr) {
+                oi[cnt] = i;
+                ov[cnt] = a[i];
+                ++cnt;
+            }
+        }
+        for (int x = 1; x < cnt; ++x) {
+            const float v = ov[x];
+            const int64_t id = oi[x];
+            int y = x - 1;
+            while (y >= 0 && (use_max ? (ov[y] < v || (ov[y] == v && oi[y] > id))
+                                      : (ov[y] > v || (ov[y] == v && oi[y] > id)))) {
+                ov[y + 1] = ov[y];
+                oi[y + 1] = oi[y];
+                --y;
+            }
+            ov[y + 1] = v;
+            oi[y + 1] = id;
+        }
+    }
+}
+
+}  // namespace
+
+size_t stats_lds_bytes(int64_t T) {
+    int64_t n2 = 1;
+    while (n2 < T) n2 <<= 1;
+    return static_cast<size_t>(T + n2) * 4;
+}
+
+hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_top, int use_max,
+                        float* summary, int64_t* top_idx, float* top_val, int* flags,
+                        hipStream_t s) {
+    if (n_curves <= 0 || T <= 0) return hipSuccess;
+    if (T > kStatsMaxT || n_top > T) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stats_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kThreads),
+                       stats_lds_bytes(T), s, data, T, n_top, use_max, summary, top_idx, top_val,
+                       flags);
+    return hipGetLastError();
+}
+
+}  // namespace pf

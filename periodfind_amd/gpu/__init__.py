@@ -21,6 +21,7 @@ from periodfind_amd.results import Periodogram, Statistics
 
 N_PEAKS_DEFAULT = 32
 MIN_DISTANCE_DEFAULT = 1
+DEVICE_STATS_MAX_T = 8192  # trials per curve the device 'stats' kernel stages in LDS
 
 _ctx_lock = threading.Lock()
 _contexts = {}
@@ -97,14 +98,42 @@ class _Algorithm:
             raise NotImplementedError(
                 f'Output type "{output}" is not implemented. '
                 f'Use "stats", "periodogram", or "peaks".')
+        params = [periods, period_dts]
+        T = P.size * D.size
+        if output == "stats" and 1 <= n_stats <= T <= DEVICE_STATS_MAX_T and len(times) > 0:
+            return self._device_stats(ctx, off, t, y, e, P, D, params, n_stats,
+                                      significance_type, times, mags_use, errs)
         out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
         ctx.grid(self._algo, self._params(), off, t, y, e, P, D, out)
-        params = [periods, period_dts]
         if output == "stats":
             return [Statistics.statistics_from_data(out[i], params, self._use_max, n=n_stats,
                                                     significance_type=significance_type)
                     for i in range(len(times))]
         return [Periodogram(d, params, self._use_max) for d in out]
+
+
+    def _device_stats(self, ctx, off, t, y, e, P, D, params, n_stats, significance_type,
+                      times, mags_use, errs):
+        """'stats' computed on the device (numpy float32 semantics); curves whose
+        periodogram holds NaN are summarised by numpy from the full grid."""
+        summ, idx, val, fb = ctx.stats(self._algo, self._params(), off, t, y, e, P, D,
+                                       int(n_stats))
+        nd = D.size
+        grid = None
+        res = []
+        for c in range(len(times)):
+            if fb[c]:
+                if grid is None:
+                    grid = np.zeros((len(times), P.size, D.size), dtype=np.float32)
+                    ctx.grid(self._algo, self._params(), off, t, y, e, P, D, grid)
+                res.append(Statistics.statistics_from_data(
+                    grid[c], params, self._use_max, n=n_stats, significance_type=significance_type))
+                continue
+            mean, std, med, mad = summ[c]
+            best = [Statistics([params[0][int(i) // nd], params[1][int(i) % nd]], v, mean, std, med,
+                               mad, significance_type) for i, v in zip(idx[c], val[c])]
+            res.append(best[0] if n_stats == 1 else best)
+        return res
 
 
 class ConditionalEntropy(_Algorithm):

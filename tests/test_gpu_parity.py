@@ -189,3 +189,35 @@ def test_find_top_peaks_device(ctx):
                 gi, gv = ctx.find_top_peaks(data, n_peaks, md, use_max)
                 np.testing.assert_array_equal(gi, ri)
                 np.testing.assert_array_equal(gv, rv)
+
+
+@pytest.mark.parametrize("algo,n_stats,shape", [
+    ("ce", 1, (1000, 1)), ("bls", 3, (1000, 1)), ("ls", 5, (333, 3)), ("fpw", 1, (4097, 2)),
+    ("aov", 2, (128, 1)),
+])
+def test_device_stats_match_numpy(ctx, algo, n_stats, shape):
+    """Device 'stats' (mean, std, median, MAD, top-n) vs numpy on the same grid."""
+    from periodfind_amd.results import Statistics
+
+    t, m, e = _curves([300, 1000, 77, 2, 513], seed=16)
+    m = _prep(algo, m)
+    P = np.linspace(0.5, 10.0, shape[0]).astype(np.float32)
+    D = (np.arange(shape[1]) * 1e-5).astype(np.float32)
+    prm = PARAMS[algo] if algo != "aov" else dict(n_phase=15)
+    grid = _gpu(ctx, algo, t, m, e, P, D, **prm)
+    summ, idx, val, fb = ctx.stats(algo, t, m, P, D, errs=e if algo in ("fpw", "bls") else None,
+                                   n_stats=n_stats, **prm)
+    use_max = algo != "ce"
+    for c in range(len(t)):
+        g = grid[c]
+        if np.isnan(g).any():
+            assert fb[c] == 1
+            continue
+        assert fb[c] == 0
+        ref = Statistics.statistics_from_data(g, [P, D], use_max, n=n_stats)
+        ref = ref if isinstance(ref, list) else [ref]
+        assert summ[c, 0] == ref[0].mean and summ[c, 1] == ref[0].std
+        assert summ[c, 2] == ref[0].median and summ[c, 3] == ref[0].mad
+        np.testing.assert_array_equal(val[c], np.array([s.value for s in ref], np.float32))
+        flat = g.reshape(-1)
+        np.testing.assert_array_equal(flat[idx[c]], val[c])
