@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark: batched periodogram throughput on one MI355X.
+
+Workload (one "step"): the five statistics CE(10,10), AOV(15), LS, FPW(10)
+and BLS(50, 0.01, 0.5), each over 100 curves at each of 1024, 4096 and 8192
+points (mixed sinusoid / box-eclipse / noise curves, times uniform over 1000
+days), 1000 trial periods linspace(0.5, 10), period_dts [0], output 'stats'.
+Inputs are resident in HBM before the timed region (resident curve sets);
+a step evaluates the grid and the per-curve statistics on the device and
+copies the statistics back.
+
+Metric: pts/sec @1000 periods = (curves x points, summed over the 5
+statistics and 3 sizes) / wall seconds per step.  Also reported: point-period
+evaluations per second, the per-kernel roofline fraction (HIP events on the
+kernel's stream), the drop-in calc() rate from host arrays, and the CPU
+oracle (C++/OpenMP restatement of the Rust path) timed on this host.
+
+Prints ONE JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SIZES = (1024, 4096, 8192)
+N_CURVES = 100
+ALGOS = ("ce", "aov", "ls", "fpw", "bls")
+PARAMS = {
+    "ce": dict(n_phase=10, n_mag=10),
+    "aov": dict(n_phase=15),
+    "ls": dict(),
+    "fpw": dict(n_bins=10),
+    "bls": dict(n_bins=50, qmin=0.01, qmax=0.5),
+}
+# Algorithmic flops per point-period evaluation (inner loop) and per trial
+# (epilogue): fold 6, bin/accumulate as counted per statistic.
+FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0}
+FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0}
+PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
+# Published 1x P100 pts/sec at 8192 points (benchmarks/throughput_results_1gpu.csv:49-81).
+P100_8192 = {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510}
+
+
+def prep_mags(algo, mags):
+    from periodfind_amd._utils import prepare_magnitudes
+
+    if algo == "ce":
+        return prepare_magnitudes(mags, False, True)
+    if algo == "ls":
+        return prepare_magnitudes(mags, True, False)
+    return mags
+
+
+class Resident:
+    """A resident curve set (pf_curves) for one statistic and one size."""
+
+    def __init__(self, ctx, algo, times, mags, errs, periods, pdts):
+        from periodfind_amd import capi
+
+        self.capi = capi
+        self.ctx = ctx
+        self.algo = algo
+        self.h = ctypes.c_void_p()
+        off, t = capi.pack(times)
+        _, y = capi.pack(prep_mags(algo, mags))
+        e = capi.pack(errs)[1] if algo in ("fpw", "bls") else None
+        capi.check(capi.lib().pf_curves_create(ctx.h, ctypes.byref(self.h), len(times), off, t,
+                                               y, e))
+        self.prm = capi.make_params(algo, **PARAMS[algo])
+        self.P = np.ascontiguousarray(periods, np.float32)
+        self.D = np.ascontiguousarray(pdts, np.float32)
+        C = len(times)
+        self.summ = np.zeros((C, 4), np.float32)
+        self.idx = np.zeros((C, 1), np.int64)
+        self.val = np.zeros((C, 1), np.float32)
+        self.fb = np.zeros(C, np.int32)
+
+    def step(self):
+        """Grid + device stats; returns the grid kernel's time in ms."""
+        L = self.capi.lib()
+        self.capi.check(L.pf_curves_stats(self.ctx.h, self.h, ctypes.byref(self.prm), self.P,
+                                          self.P.size, self.D, self.D.size, 1,
+                                          self.summ.reshape(-1), self.idx.reshape(-1),
+                                          self.val.reshape(-1), self.fb))
+        return self.ctx.last_kernel_ms()
+
+    def close(self):
+        if self.h:
+            self.capi.lib().pf_curves_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+def cpu_baseline(datasets, budget_s):
+    """Oracle (C++/OpenMP restatement of the Rust path) on a bounded sample:
+    the first `k` curves of each size, all five statistics."""
+    from oracle import oracle as O
+
+    threads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        threads = min(threads, int(omp))
+    # Calibrate: one small run, then size the sample to ~budget_s.
+    t, m, e, P, D = datasets[SIZES[0]]
+    t0 = time.perf_counter()
+    O.calc_grid("fpw", t[:2], m[:2], P, D, errs=e[:2], threads=threads)
+    rate = 2 * SIZES[0] * len(P) / max(time.perf_counter() - t0, 1e-6)  # evals/s, rough
+    evals_per_curve_set = sum(SIZES) * len(P) * len(ALGOS) * 1.6  # LS/BLS are slower
+    k = int(max(1, min(N_CURVES, budget_s * rate / evals_per_curve_set)))
+    pts = 0
+    t0 = time.perf_counter()
+    for n in SIZES:
+        t, m, e, P, D = datasets[n]
+        for algo in ALGOS:
+            kw = dict(PARAMS[algo])
+            O.calc_grid(algo, t[:k], m[:k], P, D, errs=e[:k] if algo in ("fpw", "bls") else None,
+                        threads=threads, **kw)
+            pts += k * n
+    dt = time.perf_counter() - t0
+    return {
+        "value": pts / dt,
+        "unit": "pts/sec @1000 periods",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {k} of the 100 curves at each of 1024/4096/8192 points, all five "
+                  f"statistics, 1000 periods ({dt:.1f} s)",
+        "evals_per_s": pts * 1000 / dt,
+    }
+
+
+def drop_in_rate(datasets, reps=2):
+    """Reference-style wall time: calc(..., output='stats') from host arrays."""
+    import periodfind_amd.gpu as G
+
+    cls = {"ce": G.ConditionalEntropy, "aov": G.AOV, "ls": G.LombScargle, "fpw": G.FPW,
+           "bls": G.BoxLeastSquares}
+    res = {}
+    for algo in ALGOS:
+        obj = cls[algo](**PARAMS[algo] if algo != "aov" else {"n_phase": 15})
+        t, m, e, P, D = datasets[8192]
+        kw = {"errs": e} if algo in ("fpw", "bls") else {}
+        obj.calc(t, m, P, D, output="stats", **kw)
+        best = 1e9
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            obj.calc(t, m, P, D, output="stats", **kw)
+            best = min(best, time.perf_counter() - t0)
+        res[algo] = len(t) * 8192 / best
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate CPU-baseline budget (0 disables)")
+    ap.add_argument("--no-drop-in", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(curve sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from periodfind_amd import capi, synth
+
+    print("bench: building workload", file=sys.stderr, flush=True)
+    datasets = {n: synth.config2(n, N_CURVES) for n in SIZES}
+    ctx = capi.Context(0)
+    sets = {}
+    for n in SIZES:
+        t, m, e, P, D = datasets[n]
+        for algo in ALGOS:
+            sets[(algo, n)] = Resident(ctx, algo, t, m, e, P, D)
+
+    print("bench: warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        for r in sets.values():
+            r.step()
+
+    print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
+    kms = {k: [] for k in sets}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for k, r in sets.items():
+            kms[k].append(r.step())
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    pts_per_step = sum(N_CURVES * n for (_, n) in sets)
+    value = pts_per_step / (wall / args.steps)
+
+    detail = {}
+    best = None
+    for (algo, n), v in kms.items():
+        kt = float(np.mean(v)) * 1e-3
+        evals = N_CURVES * n * 1000
+        fpe = FLOPS_INNER[algo] + FLOPS_TRIAL[algo] / n
+        achieved = evals * fpe / kt / 1e12
+        detail[f"{algo}@{n}"] = {"kernel_ms": round(kt * 1e3, 4),
+                                 "evals_per_s": evals / kt,
+                                 "tflops": round(achieved, 3),
+                                 "frac": round(achieved / PEAK_VALU_TFLOPS, 4)}
+        if best is None or kt > best[1]:
+            best = (f"{algo}@{n}", kt, achieved)
+    dom, kt, ach = best
+
+    result = {
+        "metric": "pts/sec @1000 periods per algo",
+        "value": value,
+        "unit": "pts/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (fold in f64)",
+        "data": "synthetic",
+        "config": {"workload": "5 statistics x 100 curves x {1024,4096,8192} points, "
+                               "1000 periods, period_dts=[0], output=stats",
+                   "n_curves": N_CURVES, "sizes": list(SIZES), "n_periods": 1000},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(ach, 3),
+                     "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": None},
+        "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
+        "kernels": detail,
+    }
+    for r in sets.values():
+        r.close()
+    ctx.close()
+
+    if not args.no_drop_in:
+        print("bench: drop-in calc() from host arrays", file=sys.stderr, flush=True)
+        di = drop_in_rate(datasets)
+        result["drop_in_pts_per_s_8192"] = {a: round(v) for a, v in di.items()}
+        result["drop_in_vs_p100_published_8192"] = {a: round(di[a] / P100_8192[a], 1)
+                                                    for a in ALGOS}
+    if args.cpu_seconds > 0:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(datasets, args.cpu_seconds)
+        result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,8 +6,9 @@
 //   * np.sum / np.mean: pairwise summation (8 accumulators over leaves of at
 //     most 128 elements, halving splits rounded down to a multiple of 8)
 //     within blocks of 8192 elements, blocks accumulated in order; mean is
-//     fl(sum / n).
-//   * np.std: x = a - mean, x*x, the same summation, / n, sqrt.
+//     f32(f64(sum) / n) (numpy divides by an intp count, which promotes the
+//     division to float64 before the float32 cast).
+//   * np.std: x = a - mean, x*x, the same summation, f32(f64 / n), sqrt.
 //   * np.median: middle element, or fl(fl(a + b) / 2) of the two middle ones;
 //     MAD = median(|a - median|).
 //   * top-n: the n best values; among equal values the lower flat index wins
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
     }
     // mean
     if (threadIdx.x == 0) {
-        s_mean = numpy_sum(a, n) / static_cast<float>(n);
+        s_mean = static_cast<float>(static_cast<double>(numpy_sum(a, n)) / static_cast<double>(n));
         flags[c] = 0;
     }
     __syncthreads();
@@ -170,7 +171,8 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float var = numpy_sum(d, n) / static_cast<float>(n);
+        const float var =
+            static_cast<float>(static_cast<double>(numpy_sum(d, n)) / static_cast<double>(n));
         summary[c * 4 + 0] = mean;
         summary[c * 4 + 1] = __fsqrt_rn(var);
     }

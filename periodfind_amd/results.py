@@ -44,9 +44,13 @@ class Statistics:
                              n=1, significance_type="stdmean"):
         """Top-`n` parameters of a periodogram as Statistics, best first
         (periodfind/__init__.py:337-427): argpartition for the top set,
-        numpy mean/std/median/MAD over the whole grid, sorted by value."""
+        numpy mean/std/median/MAD over the whole grid, sorted by value.
+
+        The partition pivot is data.size - n.  The reference uses len(data)
+        (the number of periods), which for use_max on a multi-pdt grid picks n
+        arbitrary elements above the pivot instead of the n largest."""
         if use_max:
-            part = np.argpartition(data, len(data) - n, axis=None)[-n:]
+            part = np.argpartition(data, data.size - n, axis=None)[-n:]
         else:
             part = np.argpartition(data, n, axis=None)[:n]
         idxs = np.unravel_index(part, data.shape)
