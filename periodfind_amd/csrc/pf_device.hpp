@@ -88,6 +88,79 @@ __device__ __forceinline__ uint32_t mul_bin(float ph, float nf, uint32_t nm1) {
     return k < nm1 ? k : nm1;
 }
 
+// Lane-private bin update without LDS atomics.  A lane owns its bins, so no
+// arbitration is needed, and the hardware's f32 LDS atomic is far slower
+// than a plain read and write.  Bins are laid out [bin][plane][lane] (4 B
+// words): the A planes of one bin (e.g. sum w, sum w*y) sit 256 B apart, so
+// one address per point serves every plane through the instruction's
+// immediate offset.
+//
+// G points are applied at once: their G bins are read, a point whose bin
+// repeats an earlier one in the group takes that point's running value from
+// registers instead of the (stale) read, and the G results are written back
+// in point order (a repeated bin ends with its last value).  Each bin thus
+// receives its contributions one by one in point order, exactly as the
+// serial reference adds them.  G trades VALU (compares/selects) for fewer
+// exposed LDS round trips; G = 1 is a plain read-add-write per point.
+// col: this lane's column (plane 0, bin 0); off[j]: byte offset of point j's
+// bin, bin * A * 256.
+constexpr uint32_t kPlaneBytes = kWave * 4;
+
+template <typename T>
+__device__ __forceinline__ T* lds_at(T* col, uint32_t off, int plane) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(col) + off + plane * kPlaneBytes);
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_rmw(T* col, const uint32_t (&off)[1], const T (&val)[A][1]) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        T* p = lds_at(col, off[0], a);
+        *p = *p + val[a][0];
+    }
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_rmw(T* col, const uint32_t (&off)[2], const T (&val)[A][2]) {
+    const bool e01 = off[0] == off[1];
+    T r[A][2];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        r[a][0] = *lds_at(col, off[0], a);
+        r[a][1] = *lds_at(col, off[1], a);
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const T s0 = r[a][0] + val[a][0];
+        const T s1 = (e01 ? s0 : r[a][1]) + val[a][1];
+        *lds_at(col, off[0], a) = s0;
+        *lds_at(col, off[1], a) = s1;
+    }
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_rmw(T* col, const uint32_t (&off)[4], const T (&val)[A][4]) {
+    const bool e01 = off[0] == off[1];
+    const bool e02 = off[0] == off[2], e12 = off[1] == off[2];
+    const bool e03 = off[0] == off[3], e13 = off[1] == off[3], e23 = off[2] == off[3];
+    T r[A][4];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[a][j] = *lds_at(col, off[j], a);
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const T s0 = r[a][0] + val[a][0];
+        const T s1 = (e01 ? s0 : r[a][1]) + val[a][1];
+        const T s2 = (e12 ? s1 : (e02 ? s0 : r[a][2])) + val[a][2];
+        const T s3 = (e23 ? s2 : (e13 ? s1 : (e03 ? s0 : r[a][3]))) + val[a][3];
+        *lds_at(col, off[0], a) = s0;
+        *lds_at(col, off[1], a) = s1;
+        *lds_at(col, off[2], a) = s2;
+        *lds_at(col, off[3], a) = s3;
+    }
+}
+
 // Points are consumed in chunks of kChunk: the chunk's wave-uniform loads are
 // issued together (one scalar burst) before any of its LDS updates.
 constexpr int kChunk = 8;

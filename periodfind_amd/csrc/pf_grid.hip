@@ -7,9 +7,16 @@
 //
 // Every kernel reads, per point, a wave-uniform record (t as f64, optional
 // t*t, and one or two f32 aux values prepared once per curve set) and
-// accumulates into lane-private state: LDS bins for the binned statistics,
-// registers for LS.  Accumulation is serial in point order per trial, as in
-// the Rust reference, so f32 bin sums are bit-identical to it.
+// accumulates into lane-private state: LDS bins for the binned statistics
+// (updated by lds_rmw, no atomics), registers for LS.  Accumulation is serial
+// in point order per trial, as in the Rust reference, so f32 bin sums are
+// bit-identical to it.
+//
+// Template parameters: HAS_PDT (any nonzero period_dt: the fold subtracts
+// k*t^2), EXT1 (CE/AOV bin extents of 1), G (points per lds_rmw group, a
+// tuning knob: 1, 2 or 4).
+#include <type_traits>
+
 #include "pf_device.hpp"
 #include "pf_kernels.hpp"
 
@@ -50,18 +57,25 @@ __device__ __forceinline__ void wait_scalar_loads() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// Walk the curve's points in order.  Points are consumed in chunks of C:
-// the wave-uniform loads of chunk i+1 (times and the statistic's aux record,
+// Number of points in a body group (the extent of its phase array).
+template <class A>
+constexpr int group_size_v = static_cast<int>(std::extent<std::remove_reference_t<A>>::value);
+
+// Walk the curve's points in order.  Points are loaded in chunks of C: the
+// wave-uniform loads of chunk i+1 (times and the statistic's aux record,
 // scalar loads) are issued before chunk i is folded and accumulated, so the
-// single lgkmcnt wait per chunk (scalar loads and LDS updates share that
+// single lgkmcnt wait per chunk (scalar loads and LDS traffic share that
 // counter) falls after a whole chunk of VALU work.  The prefetch index is
 // clamped to the last full chunk, so it never reads past the curve.
-template <bool HAS_PDT, class Load, class Body>
+// `body(ph[G], rec[G])` receives the folded phases of G consecutive points
+// (G inside full chunks, 1 for the remainder).
+template <bool HAS_PDT, int G, class Load, class Body>
 __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
                                             const double* __restrict__ tt, int64_t n,
                                             const FoldTrial& f, Load load, Body body) {
     using Rec = decltype(load(int64_t(0)));
     constexpr int C = HAS_PDT ? kChunk / 2 : kChunk;
+    static_assert(C % G == 0, "chunk must hold whole groups");
     struct Chunk {
         double t[C];
         double t2[C];
@@ -77,7 +91,16 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
     };
     auto run = [&](const Chunk& ch) {
 #pragma unroll
-        for (int j = 0; j < C; ++j) body(fold_phase<HAS_PDT>(ch.t[j], ch.t2[j], f), ch.r[j]);
+        for (int gi = 0; gi < C; gi += G) {
+            float ph[G];
+            Rec rc[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                ph[j] = fold_phase<HAS_PDT>(ch.t[gi + j], ch.t2[gi + j], f);
+                rc[j] = ch.r[gi + j];
+            }
+            body(ph, rc);
+        }
     };
     // Ping-pong over two buffers.  Scalar loads return out of order, so any
     // wait on them is lgkmcnt(0); the explicit wait sits before each
@@ -102,7 +125,11 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
             run(b);
         }
     }
-    for (; i < n; ++i) body(fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f), load(i));
+    for (; i < n; ++i) {
+        float ph[1] = {fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
+        Rec rc[1] = {load(i)};
+        body(ph, rc);
+    }
 }
 
 struct F2 {
@@ -111,18 +138,16 @@ struct F2 {
 
 // ---------------------------------------------------------------------------
 // Conditional Entropy — rust/src/ce.rs:13-91.
-// LDS: u32 hist[n_phase*n_mag][64].  Per point: fold, exact phase bin,
-// one ds_add_u32 per (phase_ext x mag_ext) cell.  mrow[i] is the byte offset
-// (mag bin % n_mag) * 256 of the point's magnitude column, wave-uniform and
-// prepared per call.
+// LDS: u32 hist[n_phase+1][n_mag][64].  Row n_phase catches phase == 1.0
+// (bin n) and is folded into row 0 after the scan (counts commute), so the
+// hot loop needs no wrap.  mrow[i] is the wave-uniform byte offset
+// (mag bin % n_mag) * 256 of the point's magnitude column.
 // ---------------------------------------------------------------------------
-template <bool HAS_PDT, bool EXT1>
+template <bool HAS_PDT, bool EXT1, int G>
 __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
     extern __shared__ uint32_t hist[];
     const int lane = threadIdx.x;
     const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
-    // Rows 0..n_phase: row n_phase catches phase == 1.0 (bin n), folded into
-    // row 0 after the scan (counts commute), so the hot loop needs no wrap.
     const int nrows = EXT1 ? a.n_phase + 1 : a.n_phase;
     const int ncell = nrows * a.n_mag;
     for (int b = 0; b < ncell; ++b) hist[b * kWave + lane] = 0u;
@@ -135,26 +160,34 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
     const uint32_t np_ = static_cast<uint32_t>(a.n_phase);
     const uint32_t nm = static_cast<uint32_t>(a.n_mag);
     uint32_t* __restrict__ col = hist + lane;
-    const uint32_t row_bytes = nm * kWave * 4u;
+    const uint32_t row_bytes = nm * kPlaneBytes;
 
-    scan_points<HAS_PDT>(
-        g.t64 + p0, g.tt + p0, n, f,
-        [&](int64_t i) { return mrow[i]; },
-        [&](float ph, uint32_t mb) {
-            const uint32_t k = div_bin(ph, db);
+    scan_points<HAS_PDT, EXT1 ? G : 1>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return mrow[i]; },
+        [&](const auto& ph, const auto& mb) {
+            constexpr int K = group_size_v<decltype(ph)>;
             if (EXT1) {
-                // mb is the wave-uniform byte offset of the magnitude column.
-                uint32_t* cell = reinterpret_cast<uint32_t*>(
-                    reinterpret_cast<char*>(col) + __umul24(k, row_bytes) + mb);
-                __hip_atomic_fetch_add(cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                uint32_t off[K];
+                uint32_t one[1][K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    off[j] = __umul24(div_bin(ph[j], db), row_bytes) + mb[j];
+                    one[0][j] = 1u;
+                }
+                lds_rmw(col, off, one);
             } else {
-                const uint32_t mbin = mb / (kWave * 4u);
-                for (int po = 0; po < a.phase_ext; ++po) {
-                    const uint32_t pb = wrap_add(k, po, np_);
-                    for (int mo = 0; mo < a.mag_ext; ++mo) {
-                        const uint32_t m2 = wrap_add(mbin, mo, nm);
-                        __hip_atomic_fetch_add(col + (pb * nm + m2) * kWave, 1u,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const uint32_t k = div_bin(ph[j], db);
+                    const uint32_t mbin = mb[j] / kPlaneBytes;
+                    for (int po = 0; po < a.phase_ext; ++po) {
+                        const uint32_t pb = wrap_add(k, po, np_);
+                        for (int mo = 0; mo < a.mag_ext; ++mo) {
+                            const uint32_t m2 = wrap_add(mbin, mo, nm);
+                            const uint32_t off1[1] = {(pb * nm + m2) * kPlaneBytes};
+                            const uint32_t one1[1][1] = {{1u}};
+                            lds_rmw(col, off1, one1);
+                        }
                     }
                 }
             }
@@ -186,23 +219,18 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
 
 // ---------------------------------------------------------------------------
 // AOV — rust/src/aov.rs:12-89.
-// LDS: u32 cnt[nb][64], f32 sum[nb][64], f32 sq[nb][64].  Per point: three
-// ds_add per extent cell.  Per-curve mean is the serial f32 mean (aov.rs:27).
+// LDS: per bin three planes (count, sum y, sum y^2), [bin][3][64] f32; the
+// count is an exact f32 integer (Rust's u32 count converted to f32 in the
+// epilogue, aov.rs:68).  Per-curve mean is the serial f32 mean (aov.rs:27).
 // ---------------------------------------------------------------------------
-template <bool HAS_PDT, bool EXT1>
+template <bool HAS_PDT, bool EXT1, int G>
 __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ float lds[];
     const int lane = threadIdx.x;
     const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
-    uint32_t* cnt = lds + lane;
-    float* sum = reinterpret_cast<float*>(lds + nb * kWave) + lane;
-    float* sq = reinterpret_cast<float*>(lds + 2 * nb * kWave) + lane;
-    for (int b = 0; b < nb; ++b) {
-        cnt[b * kWave] = 0u;
-        sum[b * kWave] = 0.0f;
-        sq[b * kWave] = 0.0f;
-    }
+    float* col = lds + lane;
+    for (int b = 0; b < 3 * nb; ++b) col[b * kWave] = 0.0f;
 
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
@@ -212,20 +240,34 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
     const float nf = static_cast<float>(nb);
     const DivBin db = make_div_bin(nb);
     const uint32_t nbu = static_cast<uint32_t>(nb);
+    constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
 
-    scan_points<HAS_PDT>(
+    scan_points<HAS_PDT, EXT1 ? G : 1>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; },
-        [&](float ph, F2 v) {
-            const uint32_t k = div_bin(ph, db);
-            const int reps = EXT1 ? 1 : a.ext;
-            for (int o = 0; o < reps; ++o) {
-                const uint32_t b = EXT1 ? ((k == nbu) ? 0u : k) : wrap_add(k, o, nbu);
-                __hip_atomic_fetch_add(cnt + b * kWave, 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
-                __hip_atomic_fetch_add(sum + b * kWave, v.a, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
-                __hip_atomic_fetch_add(sq + b * kWave, v.b, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+        [&](const auto& ph, const auto& v) {
+            constexpr int K = group_size_v<decltype(ph)>;
+            if (EXT1) {
+                uint32_t off[K];
+                float val[3][K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const uint32_t k = div_bin(ph[j], db);
+                    off[j] = ((k == nbu) ? 0u : k) * kBinBytes;
+                    val[0][j] = 1.0f;
+                    val[1][j] = v[j].a;
+                    val[2][j] = v[j].b;
+                }
+                lds_rmw(col, off, val);
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const uint32_t k = div_bin(ph[j], db);
+                    for (int o = 0; o < a.ext; ++o) {
+                        const uint32_t off1[1] = {wrap_add(k, o, nbu) * kBinBytes};
+                        const float val1[3][1] = {{1.0f}, {v[j].a}, {v[j].b}};
+                        lds_rmw(col, off1, val1);
+                    }
+                }
             }
         });
 
@@ -233,16 +275,16 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
     const float avg = a.avg[lt.curve];
     float s1 = 0.0f, s2 = 0.0f;
     for (int b = 0; b < nb; ++b) {
-        const float c = static_cast<float>(cnt[b * kWave]);
+        const float c = col[(3 * b) * kWave];
         if (c != 0.0f) {
-            const float aux = sum[b * kWave] / c;
+            const float aux = col[(3 * b + 1) * kWave] / c;
             const float res = aux - avg;
             float t1 = c * res;
             t1 = t1 * res;
             s1 += t1;
             float t2 = c * aux;
             t2 = t2 * aux;
-            s2 += sq[b * kWave] - t2;
+            s2 += col[(3 * b + 2) * kWave] - t2;
         }
     }
     float F = 0.0f;
@@ -266,27 +308,30 @@ __global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
     const int64_t n = g.off[lt.curve + 1] - p0;
-    const double* __restrict__ t64 = g.t64 + p0;
-    const double* __restrict__ tt = g.tt + p0;
     const float* __restrict__ y = a.y + p0;
     const float two_pi = 2.0f * 3.14159274101257324f;  // 2 * (f32)pi, ls.rs:9
 
     float mc = 0.0f, ms = 0.0f, cc = 0.0f, cs = 0.0f;
-    scan_points<HAS_PDT>(
-        t64, tt, n, f, [&](int64_t i) { return y[i]; },
-        [&](float ph, float yi) {
-            float s, c;
-            if (FAST) {
-                s = __builtin_amdgcn_sinf(ph);
-                c = __builtin_amdgcn_cosf(ph);
-            } else {
-                sincosf(two_pi * ph, &s, &c);
+    scan_points<HAS_PDT, 4>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return y[i]; },
+        [&](const auto& ph, const auto& yv) {
+            constexpr int K = group_size_v<decltype(ph)>;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                float s, c;
+                if (FAST) {
+                    s = __builtin_amdgcn_sinf(ph[j]);
+                    c = __builtin_amdgcn_cosf(ph[j]);
+                } else {
+                    sincosf(two_pi * ph[j], &s, &c);
+                }
+                const float yi = yv[j];
+                float p;
+                p = yi * c; mc += p;
+                p = yi * s; ms += p;
+                p = c * c;  cc += p;
+                p = c * s;  cs += p;
             }
-            float p;
-            p = yi * c; mc += p;
-            p = yi * s; ms += p;
-            p = c * c;  cc += p;
-            p = c * s;  cs += p;
         });
     // ls.rs:47-77.
     const float ss = static_cast<float>(n) - cc;
@@ -306,21 +351,17 @@ __global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
 
 // ---------------------------------------------------------------------------
 // FPW — rust/src/fpw.rs:13-67.
-// LDS: f32 vw[nb][64], vy[nb][64]; w = 1/(e*e), wy = w*y prepared per curve
-// set (fpw.rs:26-27, uncentred).
+// LDS: [bin][2][64] f32 (sum w, sum w*y); w = 1/(e*e), wy = w*y prepared per
+// curve set (fpw.rs:26-27, uncentred).
 // ---------------------------------------------------------------------------
-template <bool HAS_PDT>
+template <bool HAS_PDT, int G>
 __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
     extern __shared__ float fl[];
     const int lane = threadIdx.x;
     const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
-    float* vw = fl + lane;
-    float* vy = fl + nb * kWave + lane;
-    for (int b = 0; b < nb; ++b) {
-        vw[b * kWave] = 0.0f;
-        vy[b * kWave] = 0.0f;
-    }
+    float* col = fl + lane;
+    for (int b = 0; b < 2 * nb; ++b) col[b * kWave] = 0.0f;
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
     const int64_t n = g.off[lt.curve + 1] - p0;
@@ -329,21 +370,26 @@ __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
     const float nf = static_cast<float>(nb);
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
 
-    scan_points<HAS_PDT>(
+    scan_points<HAS_PDT, G>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], wy[i]}; },
-        [&](float ph, F2 v) {
-            const uint32_t b = mul_bin(ph, nf, nm1);
-            __hip_atomic_fetch_add(vw + b * kWave, v.a, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
-            __hip_atomic_fetch_add(vy + b * kWave, v.b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+        [&](const auto& ph, const auto& v) {
+            constexpr int K = group_size_v<decltype(ph)>;
+            uint32_t off[K];
+            float val[2][K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
+                val[0][j] = v[j].a;
+                val[1][j] = v[j].b;
+            }
+            lds_rmw(col, off, val);
         });
     // fpw.rs:55-60.
     float s = 0.0f;
     for (int b = 0; b < nb; ++b) {
-        const float W = vw[b * kWave];
+        const float W = col[(2 * b) * kWave];
         if (W > 0.0f) {
-            const float Y = vy[b * kWave];
+            const float Y = col[(2 * b + 1) * kWave];
             const float num = Y * Y;
             const float den = 2.0f * W;
             s += num / den;
@@ -354,10 +400,10 @@ __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
 
 // ---------------------------------------------------------------------------
 // BLS — rust/src/bls.rs:20-134.
-// LDS: two planes (W, Y) of [nb+1][64] f32.  Binning writes bin k into slot
-// k+1; the prefix pass then turns slot k into prefix[k] in place
-// (bls.rs:85-91).  The (duration, phase) search (bls.rs:100-127) is serial
-// per lane, with uniform loop bounds, so lanes never diverge on them.
+// LDS: [slot 0..nb][2][64] f32 (W, Y).  Binning adds bin k into slot k+1; the
+// prefix pass then turns slot k into prefix[k] in place (bls.rs:85-91).  The
+// (duration, phase) search (bls.rs:100-127) is serial per lane, with uniform
+// loop bounds, so lanes never diverge on them.
 //
 // Exact max: the reference keeps max over fl(num/den).  Rather than one IEEE
 // division per candidate, a candidate is tested against the midpoint M
@@ -375,18 +421,14 @@ __device__ __forceinline__ double bls_midpoint_up(float b) {
     return 0.5 * (static_cast<double>(b) + static_cast<double>(succ_pos(b)));
 }
 
-template <bool HAS_PDT>
+template <bool HAS_PDT, int G>
 __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
     extern __shared__ float fl[];
     const int lane = threadIdx.x;
     const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
-    float* W = fl + lane;                     // slots 0..nb
-    float* Y = fl + (nb + 1) * kWave + lane;  // slots 0..nb
-    for (int b = 0; b <= nb; ++b) {
-        W[b * kWave] = 0.0f;
-        Y[b * kWave] = 0.0f;
-    }
+    float* col = fl + lane;  // slot s: W at plane 0, Y at plane 1
+    for (int b = 0; b < 2 * (nb + 1); ++b) col[b * kWave] = 0.0f;
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
     const int64_t n = g.off[lt.curve + 1] - p0;
@@ -394,47 +436,53 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
     const float* __restrict__ yw = a.yw + p0;
     const float nf = static_cast<float>(nb);
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
-    float* W1 = W + kWave;
-    float* Y1 = Y + kWave;
+    constexpr uint32_t kSlotBytes = 2 * kPlaneBytes;
+    float* col1 = col + 2 * kWave;  // slot 1
 
-    scan_points<HAS_PDT>(
+    scan_points<HAS_PDT, G>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; },
-        [&](float ph, F2 v) {
-            const uint32_t b = mul_bin(ph, nf, nm1);
-            __hip_atomic_fetch_add(W1 + b * kWave, v.a, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
-            __hip_atomic_fetch_add(Y1 + b * kWave, v.b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+        [&](const auto& ph, const auto& v) {
+            constexpr int K = group_size_v<decltype(ph)>;
+            uint32_t off[K];
+            float val[2][K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                off[j] = mul_bin(ph[j], nf, nm1) * kSlotBytes;
+                val[0][j] = v[j].a;
+                val[1][j] = v[j].b;
+            }
+            lds_rmw(col1, off, val);
         });
     // Prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
     {
         float pw = 0.0f, py = 0.0f;
-        for (int k = 0; k < nb; ++k) {
-            pw = pw + W1[k * kWave];
-            py = py + Y1[k * kWave];
-            W1[k * kWave] = pw;
-            Y1[k * kWave] = py;
+        for (int k = 1; k <= nb; ++k) {
+            pw = pw + col[(2 * k) * kWave];
+            py = py + col[(2 * k + 1) * kWave];
+            col[(2 * k) * kWave] = pw;
+            col[(2 * k + 1) * kWave] = py;
         }
     }
-    const float wt = W[nb * kWave];
+    const float wt = col[(2 * nb) * kWave];
+    const float ytot = col[(2 * nb + 1) * kWave];
     float best = 0.0f;
     if (wt > 0.0f) {
         double mid = bls_midpoint_up(best);
         for (int p = 0; p < nb; ++p) {
-            const float w0 = W[p * kWave];
-            const float y0 = Y[p * kWave];
-            const float wtail = wt - w0;                  // wp[nb] - wp[phi]
-            const float ytail = Y[nb * kWave] - y0;       // yp[nb] - yp[phi]
+            const float w0 = col[(2 * p) * kWave];
+            const float y0 = col[(2 * p + 1) * kWave];
+            const float wtail = wt - w0;    // wp[nb] - wp[phi]
+            const float ytail = ytot - y0;  // yp[nb] - yp[phi]
             for (int d = a.d_lo; d <= a.d_hi; ++d) {
                 const int e = p + d;
                 float r, s;
                 if (e <= nb) {
-                    r = W[e * kWave] - w0;
-                    s = Y[e * kWave] - y0;
+                    r = col[(2 * e) * kWave] - w0;
+                    s = col[(2 * e + 1) * kWave] - y0;
                 } else {
                     const int wr = e - nb;
-                    r = wtail + W[wr * kWave];
-                    s = ytail + Y[wr * kWave];
+                    r = wtail + col[(2 * wr) * kWave];
+                    s = ytail + col[(2 * wr + 1) * kWave];
                 }
                 // 0 < r/wt < 1  <=>  0 < r < wt  for finite wt > 0.
                 if (r > 0.0f && r < wt) {
@@ -443,9 +491,7 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
                     const double lhs = static_cast<double>(num);
                     const double rhs = mid * static_cast<double>(den);
                     bool up = lhs > rhs;
-                    if (lhs == rhs) {
-                        up = (__float_as_uint(succ_pos(best)) & 1u) == 0u;
-                    }
+                    if (lhs == rhs) up = (__float_as_uint(succ_pos(best)) & 1u) == 0u;
                     if (!(den > 0.0f)) up = (num / den) > best;
                     if (up) {
                         const float v = num / den;
@@ -462,31 +508,51 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers.
+// Host-side launchers.  `grp` selects the lds_rmw group size (1, 2 or 4).
 // ---------------------------------------------------------------------------
 static inline dim3 grid_of(const GridArgs& g) { return dim3(static_cast<uint32_t>(g.n_blocks)); }
 
-hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, hipStream_t s) {
+template <bool P, bool E>
+void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s) {
+    if (!E) hipLaunchKernelGGL((ce_kernel<P, false, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp >= 4) hipLaunchKernelGGL((ce_kernel<P, E, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp == 2) hipLaunchKernelGGL((ce_kernel<P, E, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((ce_kernel<P, E, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+}
+
+template <bool P, bool E>
+void aov_g(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s) {
+    if (!E) hipLaunchKernelGGL((aov_kernel<P, false, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp >= 4) hipLaunchKernelGGL((aov_kernel<P, E, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp == 2) hipLaunchKernelGGL((aov_kernel<P, E, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((aov_kernel<P, E, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+}
+
+template <bool P>
+void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s) {
+    if (grp >= 4) hipLaunchKernelGGL((fpw_kernel<P, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp == 2) hipLaunchKernelGGL((fpw_kernel<P, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((fpw_kernel<P, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+}
+
+template <bool P>
+void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s) {
+    if (grp >= 4) hipLaunchKernelGGL((bls_kernel<P, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp == 2) hipLaunchKernelGGL((bls_kernel<P, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else hipLaunchKernelGGL((bls_kernel<P, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+}
+
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s) {
     const bool ext1 = a.phase_ext == 1 && a.mag_ext == 1;
-    if (g.has_pdt) {
-        if (ext1) hipLaunchKernelGGL((ce_kernel<true, true>), grid_of(g), dim3(kWave), lds, s, g, a);
-        else hipLaunchKernelGGL((ce_kernel<true, false>), grid_of(g), dim3(kWave), lds, s, g, a);
-    } else {
-        if (ext1) hipLaunchKernelGGL((ce_kernel<false, true>), grid_of(g), dim3(kWave), lds, s, g, a);
-        else hipLaunchKernelGGL((ce_kernel<false, false>), grid_of(g), dim3(kWave), lds, s, g, a);
-    }
+    if (g.has_pdt) ext1 ? ce_g<true, true>(g, a, lds, grp, s) : ce_g<true, false>(g, a, lds, grp, s);
+    else ext1 ? ce_g<false, true>(g, a, lds, grp, s) : ce_g<false, false>(g, a, lds, grp, s);
     return hipGetLastError();
 }
 
-hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, hipStream_t s) {
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s) {
     const bool ext1 = a.ext == 1;
-    if (g.has_pdt) {
-        if (ext1) hipLaunchKernelGGL((aov_kernel<true, true>), grid_of(g), dim3(kWave), lds, s, g, a);
-        else hipLaunchKernelGGL((aov_kernel<true, false>), grid_of(g), dim3(kWave), lds, s, g, a);
-    } else {
-        if (ext1) hipLaunchKernelGGL((aov_kernel<false, true>), grid_of(g), dim3(kWave), lds, s, g, a);
-        else hipLaunchKernelGGL((aov_kernel<false, false>), grid_of(g), dim3(kWave), lds, s, g, a);
-    }
+    if (g.has_pdt) ext1 ? aov_g<true, true>(g, a, lds, grp, s) : aov_g<true, false>(g, a, lds, grp, s);
+    else ext1 ? aov_g<false, true>(g, a, lds, grp, s) : aov_g<false, false>(g, a, lds, grp, s);
     return hipGetLastError();
 }
 
@@ -501,15 +567,13 @@ hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, hipStream_t s) {
-    if (g.has_pdt) hipLaunchKernelGGL((fpw_kernel<true>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((fpw_kernel<false>), grid_of(g), dim3(kWave), lds, s, g, a);
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s) {
+    g.has_pdt ? fpw_g<true>(g, a, lds, grp, s) : fpw_g<false>(g, a, lds, grp, s);
     return hipGetLastError();
 }
 
-hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, hipStream_t s) {
-    if (g.has_pdt) hipLaunchKernelGGL((bls_kernel<true>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((bls_kernel<false>), grid_of(g), dim3(kWave), lds, s, g, a);
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s) {
+    g.has_pdt ? bls_g<true>(g, a, lds, grp, s) : bls_g<false>(g, a, lds, grp, s);
     return hipGetLastError();
 }
 

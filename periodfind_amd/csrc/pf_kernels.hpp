@@ -52,11 +52,12 @@ struct BlsArgs {
     int n_bins, d_lo, d_hi;
 };
 
-hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, hipStream_t s);
-hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, hipStream_t s);
+// `grp`: points per LDS read-modify-write group (1, 2 or 4), a tuning knob.
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s);
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s);
 hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s);
-hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, hipStream_t s);
-hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, hipStream_t s);
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s);
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s);
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }

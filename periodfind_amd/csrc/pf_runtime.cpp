@@ -129,6 +129,7 @@ struct pf_ctx {
     bool timed = false;
     float last_ms = 0.0f;
     size_t lds_max = 65536;
+    int rmw_group = 4;  // LDS read-modify-write group size of the binned kernels
     CurveSet scratch;  // curve set of the drop-in calls
     // Trial grid (cached by content).
     DevBuf periods, pdts;
@@ -365,13 +366,13 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     switch (p.algo) {
         case PF_ALGO_CE: {
             pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), s);
+            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->rmw_group, s);
             break;
         }
         case PF_ALGO_AOV: {
             pf::AovArgs a{as<float>(cs.y), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
                           p.n_phase, p.phase_ext};
-            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), s);
+            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), ctx->rmw_group, s);
             break;
         }
         case PF_ALGO_LS: {
@@ -381,14 +382,14 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
         }
         case PF_ALGO_FPW: {
             pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
-            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), s);
+            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), ctx->rmw_group, s);
             break;
         }
         case PF_ALGO_BLS: {
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
             pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi};
-            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), s);
+            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), ctx->rmw_group, s);
             break;
         }
     }
@@ -545,6 +546,10 @@ int pf_ctx_create(pf_ctx** out, int device) {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
             hipSuccess && lds > 0)
         ctx->lds_max = static_cast<size_t>(lds);
+    if (const char* g = std::getenv("PF_RMW_GROUP")) {
+        const int v = std::atoi(g);
+        ctx->rmw_group = (v >= 4) ? 4 : (v == 2 ? 2 : 1);
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         pf_ctx_destroy(ctx);

@@ -174,7 +174,9 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
         const float var =
             static_cast<float>(static_cast<double>(numpy_sum(d, n)) / static_cast<double>(n));
         summary[c * 4 + 0] = mean;
-        summary[c * 4 + 1] = __fsqrt_rn(var);
+        // f64 sqrt then f32: correctly rounded (__fsqrt_rn maps to the native
+        // approximate sqrt unless OCML's rounded operations are enabled).
+        summary[c * 4 + 1] = static_cast<float>(__builtin_sqrt(static_cast<double>(var)));
     }
     __syncthreads();
     // median
