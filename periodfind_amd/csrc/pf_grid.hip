@@ -14,7 +14,8 @@
 //
 // Template parameters: HAS_PDT (any nonzero period_dt: the fold subtracts
 // k*t^2), EXT1 (CE/AOV bin extents of 1), G (points per lds_rmw group, a
-// tuning knob: 1, 2 or 4).
+// tuning knob: 1, 2 or 4; for CE, G = 0 selects non-returning LDS u32
+// atomics, which the hardware executes fast, unlike its f32 LDS atomics).
 #include <type_traits>
 
 #include "pf_device.hpp"
@@ -162,11 +163,19 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
     uint32_t* __restrict__ col = hist + lane;
     const uint32_t row_bytes = nm * kPlaneBytes;
 
-    scan_points<HAS_PDT, EXT1 ? G : 1>(
+    scan_points<HAS_PDT, (EXT1 && G > 0) ? G : 1>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return mrow[i]; },
         [&](const auto& ph, const auto& mb) {
             constexpr int K = group_size_v<decltype(ph)>;
-            if (EXT1) {
+            if (EXT1 && G == 0) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    uint32_t* cell = reinterpret_cast<uint32_t*>(
+                        reinterpret_cast<char*>(col) + __umul24(div_bin(ph[j], db), row_bytes) +
+                        mb[j]);
+                    __hip_atomic_fetch_add(cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
+            } else if (EXT1) {
                 uint32_t off[K];
                 uint32_t one[1][K];
 #pragma unroll
@@ -468,14 +477,30 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
     float best = 0.0f;
     if (wt > 0.0f) {
         double mid = bls_midpoint_up(best);
+        // One candidate window [p, p+d): Rust's window sums (bls.rs:104-117),
+        // then the exact-max test.  The filter num >= M*den is a superset of
+        // fl(num/den) > best (it also admits ties and den == 0); admitted
+        // candidates are decided exactly by the IEEE quotient, as in Rust.
+        auto consider = [&](float r, float s) {
+            if (r > 0.0f && r < wt) {  // 0 < r/wt < 1 for finite wt > 0
+                const float num = s * s;
+                const float den = r * (wt - r);
+                if (static_cast<double>(num) >= mid * static_cast<double>(den)) {
+                    const float v = num / den;
+                    if (v > best) {
+                        best = v;
+                        mid = bls_midpoint_up(best);
+                    }
+                }
+            }
+        };
         for (int p = 0; p < nb; ++p) {
             const float w0 = col[(2 * p) * kWave];
             const float y0 = col[(2 * p + 1) * kWave];
             const float wtail = wt - w0;    // wp[nb] - wp[phi]
             const float ytail = ytot - y0;  // yp[nb] - yp[phi]
-            for (int d = a.d_lo; d <= a.d_hi; ++d) {
+            auto window = [&](int d, float& r, float& s) {
                 const int e = p + d;
-                float r, s;
                 if (e <= nb) {
                     r = col[(2 * e) * kWave] - w0;
                     s = col[(2 * e + 1) * kWave] - y0;
@@ -484,23 +509,25 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
                     r = wtail + col[(2 * wr) * kWave];
                     s = ytail + col[(2 * wr + 1) * kWave];
                 }
-                // 0 < r/wt < 1  <=>  0 < r < wt  for finite wt > 0.
-                if (r > 0.0f && r < wt) {
-                    const float num = s * s;
-                    const float den = r * (wt - r);
-                    const double lhs = static_cast<double>(num);
-                    const double rhs = mid * static_cast<double>(den);
-                    bool up = lhs > rhs;
-                    if (lhs == rhs) up = (__float_as_uint(succ_pos(best)) & 1u) == 0u;
-                    if (!(den > 0.0f)) up = (num / den) > best;
-                    if (up) {
-                        const float v = num / den;
-                        if (v > best) {
-                            best = v;
-                            mid = bls_midpoint_up(best);
-                        }
-                    }
-                }
+            };
+            int d = a.d_lo;
+            // Four durations per step: their LDS reads are issued before any
+            // of them is evaluated, so the read latency is paid once.
+            for (; d + 3 <= a.d_hi; d += 4) {
+                float r0, s0, r1, s1, r2, s2, r3, s3;
+                window(d, r0, s0);
+                window(d + 1, r1, s1);
+                window(d + 2, r2, s2);
+                window(d + 3, r3, s3);
+                consider(r0, s0);
+                consider(r1, s1);
+                consider(r2, s2);
+                consider(r3, s3);
+            }
+            for (; d <= a.d_hi; ++d) {
+                float r, s;
+                window(d, r, s);
+                consider(r, s);
             }
         }
     }
@@ -515,6 +542,7 @@ static inline dim3 grid_of(const GridArgs& g) { return dim3(static_cast<uint32_t
 template <bool P, bool E>
 void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s) {
     if (!E) hipLaunchKernelGGL((ce_kernel<P, false, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+    else if (grp <= 0) hipLaunchKernelGGL((ce_kernel<P, E, 0>), grid_of(g), dim3(kWave), lds, s, g, a);
     else if (grp >= 4) hipLaunchKernelGGL((ce_kernel<P, E, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
     else if (grp == 2) hipLaunchKernelGGL((ce_kernel<P, E, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
     else hipLaunchKernelGGL((ce_kernel<P, E, 1>), grid_of(g), dim3(kWave), lds, s, g, a);

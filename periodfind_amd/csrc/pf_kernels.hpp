@@ -82,6 +82,7 @@ hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w
 // Per-curve summary statistics and top-n (pf_stats.hip).
 constexpr int64_t kStatsMaxT = 8192;
 size_t stats_lds_bytes(int64_t T);
+size_t stats_lds_static_bytes();
 hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_top, int use_max,
                         float* summary, int64_t* top_idx, float* top_val, int* flags,
                         hipStream_t s);

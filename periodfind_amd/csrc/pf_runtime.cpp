@@ -129,7 +129,8 @@ struct pf_ctx {
     bool timed = false;
     float last_ms = 0.0f;
     size_t lds_max = 65536;
-    int rmw_group = 4;  // LDS read-modify-write group size of the binned kernels
+    int rmw_group = 2;  // LDS read-modify-write group size of the f32-binned kernels
+    int ce_group = 0;   // CE: 0 = LDS u32 atomics, else read-modify-write group size
     CurveSet scratch;  // curve set of the drop-in calls
     // Trial grid (cached by content).
     DevBuf periods, pdts;
@@ -366,7 +367,7 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     switch (p.algo) {
         case PF_ALGO_CE: {
             pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->rmw_group, s);
+            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->ce_group, s);
             break;
         }
         case PF_ALGO_AOV: {
@@ -482,7 +483,8 @@ int stats_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, i
                     static_cast<long long>(pf::kStatsMaxT));
     if (n_stats < 1 || n_stats > T) return fail(PF_E_ARG, "n_stats must be in [1, T]");
     if (!summary || !idx || !val || !fallback) return fail(PF_E_ARG, "NULL output");
-    if (pf::stats_lds_bytes(T) > ctx->lds_max) return fail(PF_E_LIMIT, "stats exceed LDS");
+    if (pf::stats_lds_bytes(T) + pf::stats_lds_static_bytes() > ctx->lds_max)
+        return fail(PF_E_LIMIT, "stats exceed LDS");
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(run_grid(ctx, cs, p, np, nd));
     const size_t C = static_cast<size_t>(cs.n_curves);
@@ -549,6 +551,10 @@ int pf_ctx_create(pf_ctx** out, int device) {
     if (const char* g = std::getenv("PF_RMW_GROUP")) {
         const int v = std::atoi(g);
         ctx->rmw_group = (v >= 4) ? 4 : (v == 2 ? 2 : 1);
+    }
+    if (const char* g = std::getenv("PF_CE_GROUP")) {
+        const int v = std::atoi(g);
+        ctx->ce_group = (v >= 4) ? 4 : (v == 2 ? 2 : (v == 1 ? 1 : 0));
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {

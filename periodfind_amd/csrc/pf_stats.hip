@@ -34,51 +34,98 @@ __device__ __forceinline__ float key_value(uint32_t k) {
     return __uint_as_float(u);
 }
 
-// numpy's pairwise summation of x[0..n) (serial, one thread).
-__device__ float pairwise_sum(const float* x, int n) {
-    // Iterative post-order walk of the split tree with an explicit stack.
+// numpy's pairwise summation of x[0..n), n <= 8192 (one numpy block).  The
+// split tree (halves rounded down to a multiple of 8, leaves of <= 128
+// elements) is enumerated once; leaf sums run in parallel, one thread per
+// leaf; thread 0 then combines them in the tree's order.
+constexpr int kMaxLeaves = 72;  // the tree of n <= 8192 has at most 65 leaves
+
+struct LeafPlan {
+    int start[kMaxLeaves];
+    int len[kMaxLeaves];
+    float sum[kMaxLeaves];
+    int count;
+};
+
+// Left-to-right leaves of the split tree of [0, n) (thread 0).
+__device__ void plan_leaves(int n, LeafPlan& L) {
+    int st_s[16], st_n[16];
+    int sp = 0;
+    st_s[0] = 0;
+    st_n[0] = n;
+    sp = 1;
+    L.count = 0;
+    while (sp > 0) {
+        --sp;
+        const int s0 = st_s[sp], m = st_n[sp];
+        if (m <= 128) {
+            L.start[L.count] = s0;
+            L.len[L.count] = m;
+            ++L.count;
+            continue;
+        }
+        int n2 = m / 2;
+        n2 -= n2 % 8;
+        // push right first so the left half is visited first
+        st_s[sp] = s0 + n2;
+        st_n[sp] = m - n2;
+        ++sp;
+        st_s[sp] = s0;
+        st_n[sp] = n2;
+        ++sp;
+    }
+}
+
+// numpy's leaf: < 8 elements serial; else 8 accumulators, combined
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the remainder serially.
+__device__ float leaf_sum(const float* x, int n) {
+    if (n < 8) {
+        float res = -0.0f;
+        for (int i = 0; i < n; ++i) res += x[i];
+        return res;
+    }
+    float r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = x[j];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] += x[i + j];
+    }
+    float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += x[i];
+    return res;
+}
+
+// Post-order combination of the leaf sums (thread 0).
+__device__ float combine_leaves(int n, const LeafPlan& L) {
     struct Frame {
-        int start, n, state;
+        int n, state;
         float left;
     };
     Frame st[16];
     int sp = 0;
-    st[0] = {0, n, 0, 0.0f};
+    st[0] = {n, 0, 0.0f};
+    int next = 0;
     float ret = 0.0f;
     while (true) {
         Frame& f = st[sp];
         if (f.n <= 128) {
-            float res;
-            if (f.n < 8) {
-                res = -0.0f;
-                for (int i = 0; i < f.n; ++i) res += x[f.start + i];
-            } else {
-                float r[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) r[j] = x[f.start + j];
-                int i = 8;
-                for (; i < f.n - (f.n % 8); i += 8) {
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) r[j] += x[f.start + i + j];
-                }
-                res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-                for (; i < f.n; ++i) res += x[f.start + i];
-            }
-            ret = res;
+            ret = L.sum[next++];
             if (sp == 0) return ret;
             --sp;
-            continue;  // parent consumes `ret`
+            continue;
         }
         int n2 = f.n / 2;
         n2 -= n2 % 8;
         if (f.state == 0) {
             f.state = 1;
-            st[sp + 1] = {f.start, n2, 0, 0.0f};
+            st[sp + 1] = {n2, 0, 0.0f};
             ++sp;
         } else if (f.state == 1) {
             f.left = ret;
             f.state = 2;
-            st[sp + 1] = {f.start + n2, f.n - n2, 0, 0.0f};
+            st[sp + 1] = {f.n - n2, 0, 0.0f};
             ++sp;
         } else {
             ret = f.left + ret;
@@ -88,14 +135,33 @@ __device__ float pairwise_sum(const float* x, int n) {
     }
 }
 
-// np.add.reduce over a contiguous float32 run: pairwise within 8192 blocks.
-__device__ float numpy_sum(const float* x, int n) {
-    float s = 0.0f;
-    for (int b = 0; b < n; b += 8192) {
-        const int m = (n - b) < 8192 ? (n - b) : 8192;
-        s += pairwise_sum(x + b, m);
+// np.add.reduce of x[0..n) (n <= 8192) by the whole block; result in *out.
+__device__ void block_numpy_sum(const float* x, int n, LeafPlan& L, float* out) {
+    if (threadIdx.x == 0) plan_leaves(n, L);
+    __syncthreads();
+    for (int i = threadIdx.x; i < L.count; i += kThreads) L.sum[i] = leaf_sum(x + L.start[i], L.len[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = 0.0f + combine_leaves(n, L);
+    __syncthreads();
+}
+
+// Exclusive prefix of a per-thread flag over the block in thread order;
+// returns this thread's prefix and writes the block total to *total.
+__device__ int block_scan(bool flag, int* wave_tot, int* total) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    const int in_wave = __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) wave_tot[wv] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wv) base += wave_tot[w];
+        tot += wave_tot[w];
     }
-    return s;
+    __syncthreads();
+    *total = tot;
+    return base + in_wave;
 }
 
 __device__ void bitonic_sort(uint32_t* k, int n2) {
@@ -141,8 +207,10 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
     float* a = reinterpret_cast<float*>(sm);  // original order [n]
     uint32_t* k = sm + n;                     // keys [n2]
     float* d = reinterpret_cast<float*>(k);   // scratch aliasing keys
-    __shared__ float s_mean, s_med;
-    __shared__ int s_nan;
+    __shared__ float s_mean, s_med, s_sum;
+    __shared__ int s_nan, s_better;
+    __shared__ int wave_tot[kThreads / 64];
+    __shared__ LeafPlan plan;
     const int64_t c = blockIdx.x;
     const float* row = data + c * T;
     if (threadIdx.x == 0) s_nan = 0;
@@ -158,8 +226,9 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
         return;
     }
     // mean
+    block_numpy_sum(a, n, plan, &s_sum);
     if (threadIdx.x == 0) {
-        s_mean = static_cast<float>(static_cast<double>(numpy_sum(a, n)) / static_cast<double>(n));
+        s_mean = static_cast<float>(static_cast<double>(s_sum) / static_cast<double>(n));
         flags[c] = 0;
     }
     __syncthreads();
@@ -170,9 +239,10 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
         d[i] = x * x;
     }
     __syncthreads();
+    block_numpy_sum(d, n, plan, &s_sum);
     if (threadIdx.x == 0) {
         const float var =
-            static_cast<float>(static_cast<double>(numpy_sum(d, n)) / static_cast<double>(n));
+            static_cast<float>(static_cast<double>(s_sum) / static_cast<double>(n));
         summary[c * 4 + 0] = mean;
         // f64 sqrt then f32: correctly rounded (__fsqrt_rn maps to the native
         // approximate sqrt unless OCML's rounded operations are enabled).
@@ -186,9 +256,20 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
     if (threadIdx.x == 0) s_med = sorted_median(k, n);
     __syncthreads();
     const float med = s_med;
-    // top-n threshold from the sorted keys: the n_top-th best value.
+    // top-n threshold from the sorted keys: the n_top-th best value, and the
+    // number of values strictly better than it (binary search).
     uint32_t thr_key = 0;
     if (n_top > 0) thr_key = use_max ? k[n - n_top] : k[n_top - 1];
+    if (threadIdx.x == 0 && n_top > 0) {
+        int lo = 0, hi = n;  // first index with key > thr (max) / >= thr (min)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const bool go_right = use_max ? (k[mid] <= thr_key) : (k[mid] < thr_key);
+            if (go_right) lo = mid + 1;
+            else hi = mid;
+        }
+        s_better = use_max ? (n - lo) : lo;
+    }
     __syncthreads();
     // MAD
     for (int i = threadIdx.x; i < n2; i += kThreads)
@@ -199,40 +280,49 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
         summary[c * 4 + 2] = med;
         summary[c * 4 + 3] = sorted_median(k, n);
     }
-    // Top-n (thread 0): strictly better than the threshold value, then ties in
-    // index order, then best-first insertion sort (ties keep index order).
-    if (threadIdx.x == 0 && n_top > 0) {
+    // Top-n: values strictly better than the threshold, then equal values in
+    // index order, compacted in index order by block scans; then a best-first
+    // insertion sort of the n_top entries (equal values keep index order).
+    if (n_top > 0) {
         const float thr = key_value(thr_key);
+        const int nb = s_better;
         int64_t* oi = top_idx + c * n_top;
         float* ov = top_val + c * n_top;
-        int cnt = 0;
-        for (int i = 0; i < n && cnt < n_top; ++i) {
-            const float v = a[i];
-            if (use_max ? (v > thr) : (v < thr)) {
-                oi[cnt] = i;
-                ov[cnt] = v;
-                ++cnt;
+        int base_b = 0, base_t = 0;
+        for (int r = 0; r < n; r += kThreads) {
+            const int i = r + threadIdx.x;
+            const float v = i < n ? a[i] : 0.0f;
+            const bool bt = i < n && (use_max ? (v > thr) : (v < thr));
+            const bool tie = i < n && v == thr;
+            int tot_b, tot_t;
+            const int pb = block_scan(bt, wave_tot, &tot_b);
+            const int pt = block_scan(tie, wave_tot, &tot_t);
+            if (bt) {
+                oi[base_b + pb] = i;
+                ov[base_b + pb] = v;
             }
+            if (tie && nb + base_t + pt < n_top) {
+                oi[nb + base_t + pt] = i;
+                ov[nb + base_t + pt] = v;
+            }
+            base_b += tot_b;
+            base_t += tot_t;
         }
-        for (int i = 0; i < n && cnt < n_top; ++i) {
-            if (a[i] == thr) {
-                oi[cnt] = i;
-                ov[cnt] = a[i];
-                ++cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int x = 1; x < n_top; ++x) {
+                const float v = ov[x];
+                const int64_t id = oi[x];
+                int y = x - 1;
+                while (y >= 0 && (use_max ? (ov[y] < v || (ov[y] == v && oi[y] > id))
+                                          : (ov[y] > v || (ov[y] == v && oi[y] > id)))) {
+                    ov[y + 1] = ov[y];
+                    oi[y + 1] = oi[y];
+                    --y;
+                }
+                ov[y + 1] = v;
+                oi[y + 1] = id;
             }
-        }
-        for (int x = 1; x < cnt; ++x) {
-            const float v = ov[x];
-            const int64_t id = oi[x];
-            int y = x - 1;
-            while (y >= 0 && (use_max ? (ov[y] < v || (ov[y] == v && oi[y] > id))
-                                      : (ov[y] > v || (ov[y] == v && oi[y] > id)))) {
-                ov[y + 1] = ov[y];
-                oi[y + 1] = oi[y];
-                --y;
-            }
-            ov[y + 1] = v;
-            oi[y + 1] = id;
         }
     }
 }
@@ -244,6 +334,8 @@ size_t stats_lds_bytes(int64_t T) {
     while (n2 < T) n2 <<= 1;
     return static_cast<size_t>(T + n2) * 4;
 }
+
+size_t stats_lds_static_bytes() { return sizeof(LeafPlan) + 64; }
 
 hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_top, int use_max,
                         float* summary, int64_t* top_idx, float* top_val, int* flags,

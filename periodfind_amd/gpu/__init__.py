@@ -116,8 +116,17 @@ class _Algorithm:
                       times, mags_use, errs):
         """'stats' computed on the device (numpy float32 semantics); curves whose
         periodogram holds NaN are summarised by numpy from the full grid."""
-        summ, idx, val, fb = ctx.stats(self._algo, self._params(), off, t, y, e, P, D,
-                                       int(n_stats))
+        try:
+            summ, idx, val, fb = ctx.stats(self._algo, self._params(), off, t, y, e, P, D,
+                                           int(n_stats))
+        except RuntimeError as exc:
+            if "(code -5)" not in str(exc):  # PF_E_LIMIT: grid too large to stage in LDS
+                raise
+            out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
+            ctx.grid(self._algo, self._params(), off, t, y, e, P, D, out)
+            return [Statistics.statistics_from_data(out[i], params, self._use_max, n=n_stats,
+                                                    significance_type=significance_type)
+                    for i in range(len(times))]
         nd = D.size
         grid = None
         res = []
