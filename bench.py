@@ -156,6 +156,93 @@ def drop_in_rate(datasets, reps=2):
     return res
 
 
+def run_ragged(args):
+    """Ragged workload: 10,000 curves, lengths log-uniform in [64, 16384], gappy
+    cadence, 20,000 frequencies x 3 period_dts, CE(10,10) and BLS(50), peaks
+    (n_peaks=32).  Compares the ragged persistent + hipGraph path with the plain
+    per-call path (one C-ABI call per curve, as the reference launches per
+    curve) and with the single plain grid launch."""
+    from periodfind_amd import capi, synth
+
+    print("bench: building ragged workload", file=sys.stderr, flush=True)
+    t, m, e, P, D = synth.config3()
+    npts = sum(len(x) for x in t)
+    ctx = capi.Context(0)
+    out = {"metric": "ragged vs per-call speedup", "unit": "x", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic", "dtype": "f32 (fold in f64)",
+           "config": {"workload": "ragged: 10000 curves, N log-uniform [64,16384], gappy "
+                                  "cadence, 20000 freqs x 3 pdts, peaks n_peaks=32",
+                      "n_curves": len(t), "n_points": npts, "trials_per_curve": P.size * D.size},
+           "algos": {}}
+    for algo in ("ce", "bls"):
+        prm = PARAMS[algo]
+        mags = prep_mags(algo, m)
+        errs = e if algo == "bls" else None
+        cs = capi.CurveSet(ctx, t, mags, errs)
+        print(f"bench: {algo} ragged", file=sys.stderr, flush=True)
+        gi, gv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)  # capture
+        best = 1e9
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            gi, gv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
+            best = min(best, time.perf_counter() - t0)
+        graph_ms = ctx.last_kernel_ms()
+        print(f"bench: {algo} plain single launch", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        pi, pv = cs.peaks(algo, P, D, 32, 1, ragged=False, **prm)
+        plain_s = time.perf_counter() - t0
+        cs.close()
+        print(f"bench: {algo} per-call", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        same = 0
+        for c in range(len(t)):
+            ci, cv = ctx.peaks(algo, [t[c]], [mags[c]], P, D,
+                               errs=[errs[c]] if errs is not None else None, n_peaks=32,
+                               min_distance=1, **prm)
+            same += int(np.array_equal(ci[0], gi[c]) and np.array_equal(cv[0], gv[c]))
+        percall_s = time.perf_counter() - t0
+        evals = npts * P.size * D.size
+        out["algos"][algo] = {
+            "ragged_s": best, "ragged_graph_ms": graph_ms, "plain_single_launch_s": plain_s,
+            "per_call_s": percall_s, "speedup_vs_per_call": percall_s / best,
+            "speedup_vs_plain_launch": plain_s / best,
+            "pts_per_s": npts / best, "evals_per_s": evals / best,
+            "identical_to_per_call": same / len(t),
+            "identical_plain": bool(np.array_equal(pi, gi) and np.array_equal(pv, gv)),
+        }
+    ctx.close()
+    if args.cpu_seconds > 0:
+        from oracle import oracle as O
+
+        print("bench: CPU baseline (stratified sample)", file=sys.stderr, flush=True)
+        lens = np.array([len(x) for x in t])
+        order = np.argsort(lens)
+        pick = order[np.linspace(0, len(t) - 1, 40).astype(int)]
+        threads = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS")
+        if omp and omp.isdigit():
+            threads = min(threads, int(omp))
+        cpu = {}
+        for algo in ("ce", "bls"):
+            mags = prep_mags(algo, [m[i] for i in pick])
+            t0 = time.perf_counter()
+            O.calc_grid(algo, [t[i] for i in pick], mags, P, D,
+                        errs=[e[i] for i in pick] if algo == "bls" else None, threads=threads,
+                        **({k: v for k, v in PARAMS[algo].items()} if algo != "ce" else
+                           {"n_phase": 10, "n_mag": 10, "normalize": False}))
+            dt = time.perf_counter() - t0
+            sample_pts = int(lens[pick].sum())
+            cpu[algo] = {"pts_per_s": sample_pts / dt,
+                         "full_workload_s_est": npts / (sample_pts / dt)}
+        out["cpu_baseline"] = {"cores": threads, "kind": "port",
+                               "sample": "40 curves evenly spaced in length rank, full grid",
+                               **cpu}
+    sp = [v["speedup_vs_per_call"] for v in out["algos"].values()]
+    out["value"] = float(np.exp(np.mean(np.log(sp))))
+    out["steps"] = args.steps
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,11 +251,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU-baseline budget (0 disables)")
     ap.add_argument("--no-drop-in", action="store_true")
+    ap.add_argument("--workload", default="batch", choices=["batch", "ragged"],
+                    help="batch: the headline 5-statistic workload; ragged: 10k ragged curves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(curve sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ragged":
+        return run_ragged(args)
 
     from periodfind_amd import capi, synth
 

@@ -172,6 +172,17 @@ int pf_curves_peaks(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const f
                     int64_t n_periods, const float* period_dts, int64_t n_pdts,
                     int64_t n_peaks, int64_t min_distance, int64_t* idx, float* val);
 
+/* Ragged batching: the same result as pf_curves_peaks, computed by one
+ * persistent launch over every (curve, 64-trial block) work item, curves
+ * ordered longest first and pulled from a device work counter, followed by
+ * the peak kernel; the sequence is captured once as a hipGraph and replayed
+ * while inputs, buffers and parameters are unchanged.  No counterpart in the
+ * reference, which launches one kernel per curve (cuda/ce.cu:509-538). */
+int pf_curves_peaks_ragged(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
+                           const float* periods, int64_t n_periods, const float* period_dts,
+                           int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                           float* val);
+
 /* ------------------------------------------------------------------------ */
 /* Device 'stats' output.  Replaces the per-curve host loop over             */
 /* Statistics.statistics_from_data (periodfind/ce.pyx:189-202,               */

@@ -87,6 +87,8 @@ SIGNATURES = {
                                 ctypes.POINTER(_i64)]),
     "pf_curves_peaks": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
                                _i64, _i64, _i64p, _f32p]),
+    "pf_curves_peaks_ragged": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p,
+                                      _i64, _i64, _i64, _i64p, _f32p]),
     "pf_stats": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p, _f32p_or_null,
                         _f32p, _i64, _f32p, _i64, _int, _f32p, _i64p, _f32p, _i32p]),
     "pf_curves_stats": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
@@ -250,3 +252,47 @@ def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=Non
     prm.qmin = qmin
     prm.qmax = qmax
     return prm
+
+
+class CurveSet:
+    """A resident curve set (pf_curves) on a Context."""
+
+    def __init__(self, ctx, times, mags, errs=None):
+        self.ctx = ctx
+        self.h = _vp()
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        self.n_curves = len(times)
+        check(lib().pf_curves_create(ctx.h, ctypes.byref(self.h), len(times), off, t, y, e))
+
+    def close(self):
+        if self.h:
+            lib().pf_curves_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self, algo, periods, period_dts, **p):
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        out = np.zeros((self.n_curves, P.size, D.size), np.float32)
+        prm = make_params(algo, **p)
+        check(lib().pf_curves_grid(self.ctx.h, self.h, ctypes.byref(prm), P, P.size, D, D.size,
+                                   out.reshape(-1)))
+        return out
+
+    def peaks(self, algo, periods, period_dts, n_peaks=32, min_distance=1, ragged=False, **p):
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        prm = make_params(algo, **p)
+        idx = np.zeros((self.n_curves, n_peaks), np.int64)
+        val = np.zeros((self.n_curves, n_peaks), np.float32)
+        fn = lib().pf_curves_peaks_ragged if ragged else lib().pf_curves_peaks
+        check(fn(self.ctx.h, self.h, ctypes.byref(prm), P, P.size, D, D.size, n_peaks,
+                 min_distance, idx.reshape(-1), val.reshape(-1)))
+        return idx, val

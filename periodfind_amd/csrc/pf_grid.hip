@@ -42,6 +42,28 @@ __device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tbloc
     return lt;
 }
 
+// Persistent-launch work queue: item = rank * tblocks + trial block, where
+// rank orders curves longest first (so the longest items start first and
+// the tail is short); waves claim `claim` consecutive items per atomic.
+__device__ __forceinline__ int64_t next_item(const WorkQueue& q) {
+    unsigned long long v = 0;
+    if (threadIdx.x == 0) v = atomicAdd(q.counter, static_cast<unsigned long long>(q.claim));
+    v = __shfl(v, 0);
+    return static_cast<int64_t>(v) < q.total ? static_cast<int64_t>(v) : -1;
+}
+
+__device__ __forceinline__ LaneTrial queue_trial(const WorkQueue& q, const GridArgs& g,
+                                                 int64_t item) {
+    LaneTrial lt;
+    const int64_t rank = item / g.tblocks;
+    const int64_t tb = item - rank * g.tblocks;
+    lt.curve = q.order[rank];
+    const int64_t tr = tb * kWave + threadIdx.x;
+    lt.active = tr < g.T;
+    lt.trial = lt.active ? tr : g.T - 1;
+    return lt;
+}
+
 // Wrap `k + o` into [0, n) for k <= n and small o (rust `% n`, ce.rs:57-58).
 __device__ __forceinline__ uint32_t wrap_add(uint32_t k, uint32_t o, uint32_t n) {
     uint32_t v = k + o;
@@ -145,10 +167,9 @@ struct F2 {
 // (mag bin % n_mag) * 256 of the point's magnitude column.
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, bool EXT1, int G>
-__global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
+__device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt) {
     extern __shared__ uint32_t hist[];
     const int lane = threadIdx.x;
-    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nrows = EXT1 ? a.n_phase + 1 : a.n_phase;
     const int ncell = nrows * a.n_mag;
     for (int b = 0; b < ncell; ++b) hist[b * kWave + lane] = 0u;
@@ -226,6 +247,21 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = ce;
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, bool EXT1, int G>
+__global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
+    ce_block<HAS_PDT, EXT1, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+template <bool HAS_PDT, bool EXT1, int G>
+__global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, WorkQueue q) {
+    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
+        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
+        for (; it < stop; ++it) ce_block<HAS_PDT, EXT1, G>(g, a, queue_trial(q, g, it));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // AOV — rust/src/aov.rs:12-89.
 // LDS: per bin three planes (count, sum y, sum y^2), [bin][3][64] f32; the
@@ -233,10 +269,9 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
 // epilogue, aov.rs:68).  Per-curve mean is the serial f32 mean (aov.rs:27).
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, bool EXT1, int G>
-__global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
+__device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, const LaneTrial lt) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x;
-    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
     float* col = lds + lane;
     for (int b = 0; b < 3 * nb; ++b) col[b * kWave] = 0.0f;
@@ -304,6 +339,21 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = F;
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, bool EXT1, int G>
+__global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
+    aov_block<HAS_PDT, EXT1, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+template <bool HAS_PDT, bool EXT1, int G>
+__global__ __launch_bounds__(kWave) void aov_persistent(GridArgs g, AovArgs a, WorkQueue q) {
+    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
+        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
+        for (; it < stop; ++it) aov_block<HAS_PDT, EXT1, G>(g, a, queue_trial(q, g, it));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Lomb-Scargle — rust/src/ls.rs:15-79.
 // Registers only: C = sum y cos, S = sum y sin, CC = sum cos^2, CS = sum cos sin
@@ -312,8 +362,7 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
 // fl(2pi_f32 * phase) as the reference computes it.
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, bool FAST>
-__global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
-    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
+__device__ __forceinline__ void ls_block(const GridArgs& g, const LsArgs& a, const LaneTrial lt) {
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
     const int64_t n = g.off[lt.curve + 1] - p0;
@@ -358,16 +407,30 @@ __global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = P;
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, bool FAST>
+__global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
+    ls_block<HAS_PDT, FAST>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+template <bool HAS_PDT, bool FAST>
+__global__ __launch_bounds__(kWave) void ls_persistent(GridArgs g, LsArgs a, WorkQueue q) {
+    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
+        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
+        for (; it < stop; ++it) ls_block<HAS_PDT, FAST>(g, a, queue_trial(q, g, it));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FPW — rust/src/fpw.rs:13-67.
 // LDS: [bin][2][64] f32 (sum w, sum w*y); w = 1/(e*e), wy = w*y prepared per
 // curve set (fpw.rs:26-27, uncentred).
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, int G>
-__global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
+__device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, const LaneTrial lt) {
     extern __shared__ float fl[];
     const int lane = threadIdx.x;
-    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
     float* col = fl + lane;
     for (int b = 0; b < 2 * nb; ++b) col[b * kWave] = 0.0f;
@@ -407,6 +470,21 @@ __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = s;
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
+    fpw_block<HAS_PDT, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave) void fpw_persistent(GridArgs g, FpwArgs a, WorkQueue q) {
+    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
+        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
+        for (; it < stop; ++it) fpw_block<HAS_PDT, G>(g, a, queue_trial(q, g, it));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // BLS — rust/src/bls.rs:20-134.
 // LDS: [slot 0..nb][2][64] f32 (W, Y).  Binning adds bin k into slot k+1; the
@@ -431,10 +509,9 @@ __device__ __forceinline__ double bls_midpoint_up(float b) {
 }
 
 template <bool HAS_PDT, int G>
-__global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
+__device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, const LaneTrial lt) {
     extern __shared__ float fl[];
     const int lane = threadIdx.x;
-    const LaneTrial lt = lane_trial(g.n_blocks, g.tblocks, g.T);
     const int nb = a.n_bins;
     float* col = fl + lane;  // slot s: W at plane 0, Y at plane 1
     for (int b = 0; b < 2 * (nb + 1); ++b) col[b * kWave] = 0.0f;
@@ -534,74 +611,125 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = best;
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
+    bls_block<HAS_PDT, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave) void bls_persistent(GridArgs g, BlsArgs a, WorkQueue q) {
+    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
+        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
+        for (; it < stop; ++it) bls_block<HAS_PDT, G>(g, a, queue_trial(q, g, it));
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Host-side launchers.  `grp` selects the lds_rmw group size (1, 2 or 4).
+// Host-side launchers.  `grp` selects the lds_rmw group size (1, 2 or 4; CE
+// also 0 = LDS atomics).  With q == nullptr the grid kernel runs one
+// workgroup per (curve, trial block); otherwise the persistent kernel runs
+// as many single-wave workgroups as fit on the device at once.
 // ---------------------------------------------------------------------------
 static inline dim3 grid_of(const GridArgs& g) { return dim3(static_cast<uint32_t>(g.n_blocks)); }
 
-template <bool P, bool E>
-void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s) {
-    if (!E) hipLaunchKernelGGL((ce_kernel<P, false, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp <= 0) hipLaunchKernelGGL((ce_kernel<P, E, 0>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp >= 4) hipLaunchKernelGGL((ce_kernel<P, E, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp == 2) hipLaunchKernelGGL((ce_kernel<P, E, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((ce_kernel<P, E, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
+// Resident single-wave workgroups of `kernel` with `lds` bytes, device-wide.
+template <class K>
+static uint32_t persistent_grid(K kernel, size_t lds) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
 }
 
-template <bool P, bool E>
-void aov_g(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s) {
-    if (!E) hipLaunchKernelGGL((aov_kernel<P, false, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp >= 4) hipLaunchKernelGGL((aov_kernel<P, E, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp == 2) hipLaunchKernelGGL((aov_kernel<P, E, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((aov_kernel<P, E, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
-}
-
-template <bool P>
-void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s) {
-    if (grp >= 4) hipLaunchKernelGGL((fpw_kernel<P, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp == 2) hipLaunchKernelGGL((fpw_kernel<P, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((fpw_kernel<P, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
-}
-
-template <bool P>
-void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s) {
-    if (grp >= 4) hipLaunchKernelGGL((bls_kernel<P, 4>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else if (grp == 2) hipLaunchKernelGGL((bls_kernel<P, 2>), grid_of(g), dim3(kWave), lds, s, g, a);
-    else hipLaunchKernelGGL((bls_kernel<P, 1>), grid_of(g), dim3(kWave), lds, s, g, a);
-}
-
-hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s) {
-    const bool ext1 = a.phase_ext == 1 && a.mag_ext == 1;
-    if (g.has_pdt) ext1 ? ce_g<true, true>(g, a, lds, grp, s) : ce_g<true, false>(g, a, lds, grp, s);
-    else ext1 ? ce_g<false, true>(g, a, lds, grp, s) : ce_g<false, false>(g, a, lds, grp, s);
-    return hipGetLastError();
-}
-
-hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s) {
-    const bool ext1 = a.ext == 1;
-    if (g.has_pdt) ext1 ? aov_g<true, true>(g, a, lds, grp, s) : aov_g<true, false>(g, a, lds, grp, s);
-    else ext1 ? aov_g<false, true>(g, a, lds, grp, s) : aov_g<false, false>(g, a, lds, grp, s);
-    return hipGetLastError();
-}
-
-hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s) {
-    if (g.has_pdt) {
-        if (a.fast) hipLaunchKernelGGL((ls_kernel<true, true>), grid_of(g), dim3(kWave), 0, s, g, a);
-        else hipLaunchKernelGGL((ls_kernel<true, false>), grid_of(g), dim3(kWave), 0, s, g, a);
+// Grid or persistent launch of one kernel pair.
+template <class KG, class KP, class Args>
+void launch2(KG kgrid, KP kpers, const GridArgs& g, const Args& a, size_t lds, const WorkQueue* q,
+             hipStream_t s) {
+    if (q) {
+        hipLaunchKernelGGL(kpers, dim3(persistent_grid(kpers, lds)), dim3(kWave), lds, s, g, a,
+                           *q);
     } else {
-        if (a.fast) hipLaunchKernelGGL((ls_kernel<false, true>), grid_of(g), dim3(kWave), 0, s, g, a);
-        else hipLaunchKernelGGL((ls_kernel<false, false>), grid_of(g), dim3(kWave), 0, s, g, a);
+        hipLaunchKernelGGL(kgrid, grid_of(g), dim3(kWave), lds, s, g, a);
+    }
+}
+
+template <bool P, bool E>
+void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
+          hipStream_t s) {
+    if (!E) launch2(ce_kernel<P, false, 1>, ce_persistent<P, false, 1>, g, a, lds, q, s);
+    else if (grp <= 0) launch2(ce_kernel<P, E, 0>, ce_persistent<P, E, 0>, g, a, lds, q, s);
+    else if (grp >= 4) launch2(ce_kernel<P, E, 4>, ce_persistent<P, E, 4>, g, a, lds, q, s);
+    else if (grp == 2) launch2(ce_kernel<P, E, 2>, ce_persistent<P, E, 2>, g, a, lds, q, s);
+    else launch2(ce_kernel<P, E, 1>, ce_persistent<P, E, 1>, g, a, lds, q, s);
+}
+
+template <bool P, bool E>
+void aov_g(const GridArgs& g, const AovArgs& a, size_t lds, int grp, const WorkQueue* q,
+           hipStream_t s) {
+    if (!E) launch2(aov_kernel<P, false, 1>, aov_persistent<P, false, 1>, g, a, lds, q, s);
+    else if (grp >= 4) launch2(aov_kernel<P, E, 4>, aov_persistent<P, E, 4>, g, a, lds, q, s);
+    else if (grp == 2) launch2(aov_kernel<P, E, 2>, aov_persistent<P, E, 2>, g, a, lds, q, s);
+    else launch2(aov_kernel<P, E, 1>, aov_persistent<P, E, 1>, g, a, lds, q, s);
+}
+
+template <bool P>
+void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, const WorkQueue* q,
+           hipStream_t s) {
+    if (grp >= 4) launch2(fpw_kernel<P, 4>, fpw_persistent<P, 4>, g, a, lds, q, s);
+    else if (grp == 2) launch2(fpw_kernel<P, 2>, fpw_persistent<P, 2>, g, a, lds, q, s);
+    else launch2(fpw_kernel<P, 1>, fpw_persistent<P, 1>, g, a, lds, q, s);
+}
+
+template <bool P>
+void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, const WorkQueue* q,
+           hipStream_t s) {
+    if (grp >= 4) launch2(bls_kernel<P, 4>, bls_persistent<P, 4>, g, a, lds, q, s);
+    else if (grp == 2) launch2(bls_kernel<P, 2>, bls_persistent<P, 2>, g, a, lds, q, s);
+    else launch2(bls_kernel<P, 1>, bls_persistent<P, 1>, g, a, lds, q, s);
+}
+
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s) {
+    const bool ext1 = a.phase_ext == 1 && a.mag_ext == 1;
+    if (g.has_pdt) ext1 ? ce_g<true, true>(g, a, lds, grp, q, s) : ce_g<true, false>(g, a, lds, grp, q, s);
+    else ext1 ? ce_g<false, true>(g, a, lds, grp, q, s) : ce_g<false, false>(g, a, lds, grp, q, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s) {
+    const bool ext1 = a.ext == 1;
+    if (g.has_pdt) ext1 ? aov_g<true, true>(g, a, lds, grp, q, s) : aov_g<true, false>(g, a, lds, grp, q, s);
+    else ext1 ? aov_g<false, true>(g, a, lds, grp, q, s) : aov_g<false, false>(g, a, lds, grp, q, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hipStream_t s) {
+    if (g.has_pdt) {
+        if (a.fast) launch2(ls_kernel<true, true>, ls_persistent<true, true>, g, a, 0, q, s);
+        else launch2(ls_kernel<true, false>, ls_persistent<true, false>, g, a, 0, q, s);
+    } else {
+        if (a.fast) launch2(ls_kernel<false, true>, ls_persistent<false, true>, g, a, 0, q, s);
+        else launch2(ls_kernel<false, false>, ls_persistent<false, false>, g, a, 0, q, s);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s) {
-    g.has_pdt ? fpw_g<true>(g, a, lds, grp, s) : fpw_g<false>(g, a, lds, grp, s);
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s) {
+    g.has_pdt ? fpw_g<true>(g, a, lds, grp, q, s) : fpw_g<false>(g, a, lds, grp, q, s);
     return hipGetLastError();
 }
 
-hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s) {
-    g.has_pdt ? bls_g<true>(g, a, lds, grp, s) : bls_g<false>(g, a, lds, grp, s);
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s) {
+    g.has_pdt ? bls_g<true>(g, a, lds, grp, q, s) : bls_g<false>(g, a, lds, grp, q, s);
     return hipGetLastError();
 }
 

@@ -23,6 +23,14 @@ struct GridArgs {
     int has_pdt;            // any nonzero period_dt
 };
 
+// Persistent-launch work queue (see pf_grid.hip).
+struct WorkQueue {
+    unsigned long long* counter;  // next unclaimed item (zeroed before launch)
+    const int64_t* order;         // rank -> curve index, longest curve first
+    int64_t total;                // n_curves * tblocks
+    int64_t claim;                // items claimed per atomic
+};
+
 struct CeArgs {
     const uint32_t* mrow;   // [npts] ((mag bin) % n_mag) * 256: byte offset of the column
     int n_phase, n_mag, phase_ext, mag_ext;
@@ -53,11 +61,19 @@ struct BlsArgs {
 };
 
 // `grp`: points per LDS read-modify-write group (1, 2 or 4), a tuning knob.
-hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, hipStream_t s);
-hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp, hipStream_t s);
-hipError_t launch_ls(const GridArgs& g, const LsArgs& a, hipStream_t s);
-hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, hipStream_t s);
-hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, hipStream_t s);
+// q == nullptr: one workgroup per (curve, trial block); else a persistent
+// launch pulling items from *q.
+hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s);
+hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s);
+hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hipStream_t s);
+hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s);
+hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp,
+                      const WorkQueue* q, hipStream_t s);
+// Zero a u64 counter with a kernel (graph-capture safe: no memset node).
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }

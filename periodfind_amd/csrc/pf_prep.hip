@@ -109,7 +109,14 @@ __global__ void bls_center_kernel(const int64_t* __restrict__ off, const float* 
     }
 }
 
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
 }  // namespace
+
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, p);
+    return hipGetLastError();
+}
 
 hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t npts,
                               hipStream_t s) {

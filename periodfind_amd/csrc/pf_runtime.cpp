@@ -9,6 +9,7 @@
 // (cuda/errchk.cuh:14-28).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -101,15 +102,29 @@ struct CurveSet {
     // Last periodogram.
     DevBuf out;
     int64_t out_T = 0;
+    // Ragged (persistent) launches: curves longest first, and the work counter.
+    std::vector<int64_t> off_h;  // host copy of the offsets
+    DevBuf order, counter;
+    bool order_ok = false;
+    // Captured launch sequence of the ragged peaks path, and what it was built for.
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_key;
 
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_key.clear();
+    }
     void invalidate() {
-        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = false;
+        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = order_ok = false;
         ce_nmag = -1;
         out_T = 0;
+        drop_graph();
     }
     void release() {
+        drop_graph();
         DevBuf* bufs[] = {&off, &t, &y, &e, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                          &fpw_wy, &bls_yw, &bls_mean, &out};
+                          &fpw_wy, &bls_yw, &bls_mean, &out, &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
                 (void)hipFree(b->p);
@@ -257,6 +272,7 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
     cs.n_curves = n_curves;
     cs.npts = npts;
     cs.max_len = mx;
+    cs.off_h.assign(off, off + n_curves + 1);
     cs.has_err = e != nullptr;
     const size_t nb = static_cast<size_t>(npts) * sizeof(float);
     PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
@@ -334,16 +350,13 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
 }
 
 // Launch the grid kernel of params.algo over cs into cs.out ([n_curves][T]).
-// The kernel is bracketed by the context's timing events.
-int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd) {
-    PF_TRY(check_params(p, ctx->lds_max));
+// With q, the persistent kernel pulls work from *q (derived arrays must
+// already be prepared).  `timed` brackets the launch with the context's
+// events (not allowed while the stream is being captured).
+int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd,
+                const pf::WorkQueue* q, bool timed) {
     const int64_t T = np * nd;
-    ctx->timed = false;
-    if (cs.n_curves == 0 || T == 0) return PF_OK;
     const bool has_pdt = ctx->has_pdt;
-    PF_TRY(prepare(ctx, cs, p, has_pdt));
-    PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
-    cs.out_T = T;
     hipStream_t s = ctx->stream;
 
     pf::GridArgs g;
@@ -362,43 +375,56 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
         return fail(PF_E_LIMIT, "grid of %llu workgroups exceeds one launch",
                     static_cast<unsigned long long>(g.n_blocks));
 
-    PF_HIP(hipEventRecord(ctx->ev0, s));
+    if (timed) PF_HIP(hipEventRecord(ctx->ev0, s));
     hipError_t le = hipSuccess;
     switch (p.algo) {
         case PF_ALGO_CE: {
             pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->ce_group, s);
+            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->ce_group, q, s);
             break;
         }
         case PF_ALGO_AOV: {
             pf::AovArgs a{as<float>(cs.y), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
                           p.n_phase, p.phase_ext};
-            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), ctx->rmw_group, s);
+            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), ctx->rmw_group, q, s);
             break;
         }
         case PF_ALGO_LS: {
             pf::LsArgs a{as<float>(cs.y), std::getenv("PF_LS_PRECISE") ? 0 : 1};
-            le = pf::launch_ls(g, a, s);
+            le = pf::launch_ls(g, a, q, s);
             break;
         }
         case PF_ALGO_FPW: {
             pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
-            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), ctx->rmw_group, s);
+            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), ctx->rmw_group, q, s);
             break;
         }
         case PF_ALGO_BLS: {
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
             pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi};
-            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), ctx->rmw_group, s);
+            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), ctx->rmw_group, q, s);
             break;
         }
     }
     if (le != hipSuccess)
         return fail(PF_E_HIP, "grid kernel launch failed: %s", hipGetErrorString(le));
-    PF_HIP(hipEventRecord(ctx->ev1, s));
-    ctx->timed = true;
+    if (timed) {
+        PF_HIP(hipEventRecord(ctx->ev1, s));
+        ctx->timed = true;
+    }
     return PF_OK;
+}
+
+int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd) {
+    PF_TRY(check_params(p, ctx->lds_max));
+    const int64_t T = np * nd;
+    ctx->timed = false;
+    if (cs.n_curves == 0 || T == 0) return PF_OK;
+    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
+    cs.out_T = T;
+    return launch_grid(ctx, cs, p, np, nd, nullptr, true);
 }
 
 int finish(pf_ctx* ctx) {
@@ -503,6 +529,112 @@ int stats_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, i
     PF_HIP(hipMemcpyAsync(val, ctx->pval.p, C * n_stats * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->stream));
     PF_HIP(hipMemcpyAsync(fallback, ctx->flags.p, C * sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return finish(ctx);
+}
+
+// Rank -> curve index, longest curve first (stable), uploaded once per set.
+int ensure_order(pf_ctx* ctx, CurveSet& cs) {
+    if (cs.order_ok) return PF_OK;
+    std::vector<int64_t> ord(cs.n_curves);
+    for (int64_t c = 0; c < cs.n_curves; ++c) ord[c] = c;
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+        return (cs.off_h[a + 1] - cs.off_h[a]) > (cs.off_h[b + 1] - cs.off_h[b]);
+    });
+    PF_TRY(ensure(cs.order, cs.n_curves * sizeof(int64_t)));
+    PF_TRY(ensure(cs.counter, sizeof(unsigned long long)));
+    PF_HIP(hipMemcpyAsync(cs.order.p, ord.data(), cs.n_curves * sizeof(int64_t),
+                          hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    cs.order_ok = true;
+    return PF_OK;
+}
+
+// Ragged peaks: one persistent launch over every (curve, trial block) item,
+// longest curves first, then the peak kernel.  The sequence (counter reset,
+// grid, peaks) is captured once as a hipGraph and replayed while the inputs,
+// buffers and parameters it was captured with are unchanged.
+int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods,
+                    int64_t np, const float* pdts, int64_t nd, int64_t n_peaks,
+                    int64_t min_distance, int64_t* idx, float* val) {
+    PF_TRY(check_params(p, ctx->lds_max));
+    PF_TRY(check_grid(periods, np, pdts, nd));
+    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
+    if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
+    if (cs.n_curves > 0 && n_peaks > 0 && (!idx || !val)) return fail(PF_E_ARG, "idx/val is NULL");
+    const size_t pk = static_cast<size_t>(cs.n_curves) * n_peaks;
+    if (pk == 0) return PF_OK;
+    const int64_t T = np * nd;
+    if (T == 0) {
+        for (size_t i = 0; i < pk; ++i) {
+            idx[i] = -1;
+            val[i] = 0.0f;
+        }
+        return PF_OK;
+    }
+    PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
+    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
+    cs.out_T = T;
+    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    PF_TRY(ensure_order(ctx, cs));
+    const int64_t tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
+    pf::WorkQueue q;
+    q.counter = as<unsigned long long>(cs.counter);
+    q.order = as<int64_t>(cs.order);
+    q.total = cs.n_curves * tblocks;
+    q.claim = 4;
+    const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
+    // Everything the captured kernels read as arguments.
+    std::vector<uint64_t> key = {
+        static_cast<uint64_t>(p.algo), static_cast<uint64_t>(p.n_phase),
+        static_cast<uint64_t>(p.n_mag), static_cast<uint64_t>(p.phase_ext),
+        static_cast<uint64_t>(p.mag_ext), static_cast<uint64_t>(p.n_bins),
+        static_cast<uint64_t>(__builtin_bit_cast(uint32_t, p.qmin)),
+        static_cast<uint64_t>(__builtin_bit_cast(uint32_t, p.qmax)),
+        static_cast<uint64_t>(np), static_cast<uint64_t>(nd), static_cast<uint64_t>(n_peaks),
+        static_cast<uint64_t>(min_distance), static_cast<uint64_t>(ctx->has_pdt),
+        static_cast<uint64_t>(ctx->rmw_group), static_cast<uint64_t>(ctx->ce_group),
+        reinterpret_cast<uint64_t>(ctx->periods.p), reinterpret_cast<uint64_t>(ctx->pdts.p),
+        reinterpret_cast<uint64_t>(ctx->pidx.p), reinterpret_cast<uint64_t>(ctx->pval.p),
+        reinterpret_cast<uint64_t>(cs.out.p), reinterpret_cast<uint64_t>(cs.t64.p),
+        reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
+        reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
+        reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p)};
+    if (!cs.graph || cs.graph_key != key) {
+        cs.drop_graph();
+        hipStream_t s = ctx->stream;
+        PF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = PF_OK;
+        if (pf::launch_zero_u64(q.counter, s) != hipSuccess) rc = fail(PF_E_HIP, "zero launch");
+        if (rc == PF_OK) rc = launch_grid(ctx, cs, p, np, nd, &q, false);
+        if (rc == PF_OK &&
+            pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
+                             as<int64_t>(ctx->pidx), as<float>(ctx->pval), s) != hipSuccess)
+            rc = fail(PF_E_HIP, "peaks launch failed during capture");
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc != PF_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(PF_E_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&cs.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            cs.graph = nullptr;
+            return fail(PF_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+        }
+        cs.graph_key = key;
+    }
+    PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    PF_HIP(hipGraphLaunch(cs.graph, ctx->stream));
+    PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->stream));
     return finish(ctx);
 }
@@ -748,6 +880,17 @@ int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const f
     PF_HIP(hipSetDevice(ctx->device));
     return stats_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts, n_pdts,
                     n_stats, summary, idx, val, fallback);
+}
+
+int pf_curves_peaks_ragged(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
+                           const float* periods, int64_t n_periods, const float* period_dts,
+                           int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                           float* val) {
+    if (!ctx || !cv || !params) return fail(PF_E_ARG, "NULL argument");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    return ragged_peaks_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts,
+                           n_pdts, n_peaks, min_distance, idx, val);
 }
 
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms) {

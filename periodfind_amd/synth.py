@@ -60,18 +60,19 @@ def gappy_cadence(rng, n_points, baseline=1500.0, gap_days=120.0, year=365.0):
     """Clusters of 1-4 points within +-0.1 d, centres uniform in the observable
     windows (a 120-day seasonal gap each 365-day year), until n_points."""
     season = year - gap_days
+    n_years = int(np.ceil(baseline / year))
     out = []
     total = 0
     while total < n_points:
-        k = int(rng.integers(1, 5))
-        yr = rng.integers(0, int(np.ceil(baseline / year)))
-        c = yr * year + rng.uniform(0.0, season)
-        if c > baseline:
-            continue
-        pts = c + rng.uniform(-0.1, 0.1, k)
+        m = max(8, int((n_points - total) / 2.5 * 1.2))
+        k = rng.integers(1, 5, m)
+        c = rng.integers(0, n_years, m) * year + rng.uniform(0.0, season, m)
+        keep = c <= baseline
+        k, c = k[keep], c[keep]
+        pts = np.repeat(c, k) + rng.uniform(-0.1, 0.1, int(k.sum()))
         out.append(pts)
-        total += k
-    t = np.sort(np.concatenate(out))[:n_points]
+        total += pts.size
+    t = np.sort(np.concatenate(out)[:n_points])
     return np.clip(t, 0.0, None)
 
 

@@ -240,3 +240,36 @@ def test_device_stats_match_numpy(ctx, algo, n_stats, shape):
         np.testing.assert_array_equal(val[c], np.array([s.value for s in ref], np.float32))
         flat = g.reshape(-1)
         np.testing.assert_array_equal(flat[idx[c]], val[c])
+
+
+@pytest.mark.parametrize("algo", ["ce", "bls", "fpw", "aov", "ls"])
+def test_ragged_persistent_graph_peaks(ctx, algo):
+    """Persistent longest-first launch + hipGraph replay == plain grid path == oracle."""
+    from periodfind_amd import capi
+
+    rng = np.random.default_rng(21)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(3000), 40))).astype(int)
+    lens[5] = 1 if algo == "ce" else 0  # tiny / empty curve in the middle
+    t, m, e = _curves(list(lens), seed=22)
+    m = _prep(algo, m)
+    P = (1.0 / np.linspace(0.05, 5.0, 700)).astype(np.float32)
+    D = np.array([0.0, 1e-5, -1e-5], np.float32)
+    prm = PARAMS[algo] if algo != "aov" else dict(n_phase=15)
+    cs = capi.CurveSet(ctx, t, m, errs=e if algo in ("fpw", "bls") else None)
+    ri, rv = cs.peaks(algo, P, D, n_peaks=32, min_distance=2, ragged=False, **prm)
+    for rep in range(3):  # first call captures the graph, later calls replay it
+        gi, gv = cs.peaks(algo, P, D, n_peaks=32, min_distance=2, ragged=True, **prm)
+        np.testing.assert_array_equal(gi, ri)
+        np.testing.assert_array_equal(gv, rv)
+    # a different grid re-captures
+    P2 = P[::2].copy()
+    gi2, gv2 = cs.peaks(algo, P2, D, n_peaks=16, min_distance=1, ragged=True, **prm)
+    ri2, rv2 = cs.peaks(algo, P2, D, n_peaks=16, min_distance=1, ragged=False, **prm)
+    np.testing.assert_array_equal(gi2, ri2)
+    np.testing.assert_array_equal(gv2, rv2)
+    if algo in ("aov", "fpw", "bls"):
+        grid = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+        oi, ov = O.find_top_peaks_batched(grid, 32, 2, use_max=True)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gv, ov)
+    cs.close()
