@@ -243,6 +243,111 @@ def run_ragged(args):
     print(json.dumps(out), flush=True)
 
 
+def run_lsrefine(args):
+    """Differentiable-LS workload: 1000 sinusoids (SNR U[3,30]) x 2048 points,
+    10,000-frequency grid linspace(0.1, 2); f32 LS grid + device top-8 peaks +
+    FP64 Newton refinement (pf_curves_ls_peaks_refined) on resident curves.
+    Reports the refined-frequency error against the long-double oracle's
+    refinement of the same peaks, the pipeline time, and the autograd
+    forward+backward rate of ls_power at the 8 refined frequencies."""
+    from periodfind_amd import capi, synth
+
+    print("bench: building lsrefine workload", file=sys.stderr, flush=True)
+    t, m, F, true_p = synth.config4()
+    y = prep_mags("ls", m)
+    ctx = capi.Context(0)
+    cs = capi.CurveSet(ctx, t, y)
+    K = 8
+    for _ in range(max(1, args.warmup)):
+        cs.ls_peaks_refined(F, n_peaks=K)
+    walls, grid_ms = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        idx, val, fr, pw, st = cs.ls_peaks_refined(F, n_peaks=K)
+        walls.append(time.perf_counter() - t0)
+        grid_ms.append(ctx.last_kernel_ms())
+    wall = float(np.median(walls))
+    C, N = len(t), len(t[0])
+    ok = st == 0
+    best = np.argmax(np.where(ok, pw, -1.0), axis=1)
+    fbest = fr[np.arange(C), best]
+    recovered = float(np.mean(np.abs(1.0 / fbest - true_p) / true_p < 1e-3))
+    out = {"metric": "LS refine err", "unit": "max relative period error vs FP64 oracle",
+           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "data": "synthetic", "dtype": "f32 grid, f64 refinement",
+           "config": {"workload": "differentiable LS: 1000 sinusoids (SNR U[3,30]) x 2048 "
+                                  "points, 10000 freqs linspace(0.1,2), top-8 peaks refined",
+                      "n_curves": C, "n_points": N, "n_freqs": int(F.size), "n_peaks": K},
+           "pipeline_ms": wall * 1e3, "grid_kernel_ms": float(np.median(grid_ms)),
+           "curves_per_s": C / wall,
+           "status_counts": {str(k): int((st == k).sum()) for k in range(4)},
+           "best_peak_recovers_true_period": recovered}
+    cs.close()
+
+    print("bench: oracle refinement of the same peaks", file=sys.stderr, flush=True)
+    from oracle import oracle as O
+
+    n_or = min(C, args.oracle_curves)
+    t0 = time.perf_counter()
+    rf, rp, rst = O.lsd_peaks_refined(t[:n_or], y[:n_or], F, idx[:n_or])
+    or_s = time.perf_counter() - t0
+    okc = (st[:n_or] == 0) & (rst == 0)
+    rel = np.abs(fr[:n_or][okc] - rf[okc]) / rf[okc]
+    grid_rel = np.abs(F[idx[:n_or][okc]] - rf[okc]) / rf[okc]
+    out["value"] = float(rel.max()) if rel.size else None
+    out["refine_err"] = {"curves_checked": n_or, "peaks_checked": int(okc.sum()),
+                         "max_rel": float(rel.max()) if rel.size else None,
+                         "median_rel": float(np.median(rel)) if rel.size else None,
+                         "status_mismatch": int((st[:n_or] != rst).sum()),
+                         "grid_only_max_rel": float(grid_rel.max()) if rel.size else None}
+
+    print("bench: autograd forward+backward", file=sys.stderr, flush=True)
+    import torch
+    from periodfind_amd import autograd as AG
+
+    dev = torch.device("cuda:0")
+    tt = torch.tensor(np.stack(t), device=dev)
+    yt = torch.tensor(np.stack(y), device=dev, requires_grad=True)
+    wt = torch.tensor(2 * np.pi * np.where(fr > 0, fr, F[0]), device=dev, requires_grad=True)
+    for _ in range(2):
+        AG.ls_power(tt, yt, wt).sum().backward()
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        AG.ls_power(tt, yt, wt).sum().backward()
+    torch.cuda.synchronize()
+    fb = (time.perf_counter() - t0) / reps
+    out["autograd_fwd_bwd_ms"] = fb * 1e3
+    out["autograd_point_freq_evals_per_s"] = C * N * K * 2 / fb
+    ctx.close()
+
+    if args.cpu_seconds > 0:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        threads = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS")
+        if omp and omp.isdigit():
+            threads = min(threads, int(omp))
+        Pf = (1.0 / F).astype(np.float32)
+
+        def cpu_leg(k):
+            t0 = time.perf_counter()
+            g = O.ls_grid(t[:k], y[:k], Pf, np.zeros(1, np.float32), threads=threads)
+            gi, _ = O.find_top_peaks_batched(g, K, 1, use_max=True)
+            O.lsd_peaks_refined(t[:k], y[:k], F, gi)
+            return time.perf_counter() - t0
+
+        per_curve = cpu_leg(2) / 2  # calibration, then a sample of ~cpu_seconds
+        k = int(min(C, max(2, args.cpu_seconds / max(per_curve, 1e-6))))
+        dt = cpu_leg(k)
+        out["cpu_baseline"] = {"value": k / dt, "unit": "curves/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{k} curves: f32 LS grid (OpenMP), peaks, long-double "
+                                         f"refinement (1 thread) ({dt:.1f} s)"}
+        out["gpu_over_cpu"] = round(out["curves_per_s"] / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,8 +356,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU-baseline budget (0 disables)")
     ap.add_argument("--no-drop-in", action="store_true")
-    ap.add_argument("--workload", default="batch", choices=["batch", "ragged"],
-                    help="batch: the headline 5-statistic workload; ragged: 10k ragged curves")
+    ap.add_argument("--workload", default="batch", choices=["batch", "ragged", "lsrefine"],
+                    help="batch: the headline 5-statistic workload; ragged: 10k ragged curves; "
+                         "lsrefine: differentiable LS peak refinement")
+    ap.add_argument("--oracle-curves", type=int, default=100,
+                    help="lsrefine: curves whose refinement is checked against the oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -261,6 +369,8 @@ def main():
 
     if args.workload == "ragged":
         return run_ragged(args)
+    if args.workload == "lsrefine":
+        return run_lsrefine(args)
 
     from periodfind_amd import capi, synth
 

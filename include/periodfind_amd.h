@@ -202,6 +202,65 @@ int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const f
                     int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
                     float* summary, int64_t* idx, float* val, int* fallback);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiable Lomb-Scargle (FP64).  No counterpart in the reference; the */
+/* power is the reference's (rust/src/ls.rs:15-79: unweighted, no floating   */
+/* mean, tau-rotated, 0 when a denominator is 0) as a function of the        */
+/* angular frequency w = 2 pi / period, evaluated in FP64 at arbitrary w:    */
+/*   P(w) = 1/2 b^T M^-1 b,  b = sum y_i u_i,  M = sum u_i u_i^T,             */
+/*   u_i = (cos w t_i, sin w t_i)  (identical to the tau form).              */
+/*                                                                          */
+/* Device-pointer entry points (pf_ls_jet, pf_ls_backward, pf_ls_refine):    */
+/* every array argument is device memory on ctx's device (e.g. a torch       */
+/* tensor's data_ptr()); offsets is a device int64[n_curves+1] CSR index;    */
+/* per-frequency arrays are [n_curves][K].  Work is queued on `stream` (a    */
+/* hipStream_t; NULL = the default stream, as in HIP) and NOT waited for;    */
+/* the caller's current device is restored on return.                        */
+/* ------------------------------------------------------------------------ */
+
+/* p = P(w); dp = dP/dw, d2p = d2P/dw2 (may be NULL); coef = [n_curves][K][2]
+ * x = M^-1 b in the curve's shifted time frame (tau = t - (t_first+t_last)/2),
+ * the input of pf_ls_backward (may be NULL). */
+int pf_ls_jet(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+              const double* t, const double* y, const double* omega, int64_t K, double* p,
+              double* dp, double* d2p, double* coef);
+
+/* Backward of p: grad_y[i] = sum_k grad_p[c][k] dP_k/dy_i for every point i
+ * of curve c (dP/dy_i = x . u_i), and grad_omega = grad_p * dp (skipped when
+ * grad_omega is NULL).  coef and dp come from pf_ls_jet with the same t and
+ * omega.  max_len bounds the longest curve (sizes the launch). */
+int pf_ls_backward(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+                   int64_t max_len, const double* t, const double* omega, int64_t K,
+                   const double* grad_p, const double* coef, const double* dp, double* grad_y,
+                   double* grad_omega);
+
+/* Newton refinement of maxima: for each entry with status == 0 on entry,
+ * safeguarded Newton on dP/dw inside [lo, hi] starting at omega (a valid
+ * bracket has dP/dw(lo) > 0 > dP/dw(hi)); stops when a step is below
+ * rtol * |w|.  Out: omega, p = P(omega), status 0 converged, 1 no bracketed
+ * maximum (omega, p at the start), 2 not converged in max_iter.  Entries
+ * with a nonzero status on entry are left unchanged. */
+int pf_ls_refine(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+                 const double* t, const double* y, int64_t K, const double* lo, const double* hi,
+                 double* omega, double* p, int* status, double rtol, int max_iter);
+
+/* Host-array pipeline: f32 LS grid over freqs (periods 1/f, period_dts [0],
+ * rust/src/ls.rs semantics), device top-n_peaks (peaks.rs), then pf_ls_refine
+ * of every peak k inside [f(k-1), f(k+1)] on the f32 data widened to f64.
+ * freqs (cycles per time unit, f64) must be monotone; y are prepared (the
+ * reference LS centres them, periodfind/cpu/__init__.py:318).  Out per
+ * [curve][peak]: grid idx/val, refined frequency, its FP64 power, status
+ * (0/1/2 as pf_ls_refine, 3 no peak in this slot). */
+int pf_ls_peaks_refined(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                        const float* y, const double* freqs, int64_t n_freqs, int64_t n_peaks,
+                        int64_t min_distance, double rtol, int max_iter, int64_t* idx,
+                        float* val, double* freq_out, double* power_out, int* status);
+
+int pf_curves_ls_peaks_refined(pf_ctx* ctx, pf_curves* cv, const double* freqs, int64_t n_freqs,
+                               int64_t n_peaks, int64_t min_distance, double rtol, int max_iter,
+                               int64_t* idx, float* val, double* freq_out, double* power_out,
+                               int* status);
+
 /* Kernel time of the last grid kernel launched on ctx (HIP events), ms. */
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms);
 

@@ -22,8 +22,18 @@ _lib = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
+
+
+class _F64OrNull:
+    @classmethod
+    def from_param(cls, obj):
+        return None if obj is None else _f64p.from_param(obj)
+
+
+_f64p_or_null = _F64OrNull
 _flt = ctypes.c_float
 
 
@@ -66,6 +76,15 @@ def lib():
         L.or_find_top_peaks_batched.argtypes = [_f32p, _i64, _i64, _i64, _i64, _int, _i64p,
                                                 _f32p]
         L.or_max_threads.restype = _int
+        L.or_lsd_tau_power.restype = ctypes.c_double
+        L.or_lsd_tau_power.argtypes = [_f64p, _f64p, _i64, ctypes.c_double]
+        L.or_lsd_jet.restype = None
+        L.or_lsd_jet.argtypes = [_f64p, _f64p, _i64, ctypes.c_double, _f64p, _f64p_or_null]
+        L.or_lsd_refine.restype = _int
+        L.or_lsd_refine.argtypes = [_f64p, _f64p, _i64, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, _int,
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -204,3 +223,55 @@ def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **pa
     if algo == "bls":
         return bls_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
     raise ValueError(algo)
+
+
+# --- Differentiable Lomb-Scargle (long double; pf_oracle_lsd.cpp) ---------
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def lsd_tau_power(t, y, omega):
+    """rust/src/ls.rs:30-74 at angular frequency omega, theta = omega t, long double."""
+    t, y = _f64(t), _f64(y)
+    return lib().or_lsd_tau_power(t, y, t.size, float(omega))
+
+
+def lsd_jet(t, y, omega, with_dpdy=False):
+    """(P, dP/domega, d2P/domega2[, dP/dy]) from the quadratic form, long double."""
+    t, y = _f64(t), _f64(y)
+    out = np.zeros(3, np.float64)
+    g = np.zeros(t.size, np.float64) if with_dpdy else None
+    lib().or_lsd_jet(t, y, t.size, float(omega), out, g)
+    return (out[0], out[1], out[2], g) if with_dpdy else (out[0], out[1], out[2])
+
+
+def lsd_refine(t, y, lo, hi, w0, rtol=1e-15, max_iter=200):
+    """Safeguarded Newton on dP/domega in [lo, hi] -> (omega, P, status)."""
+    t, y = _f64(t), _f64(y)
+    w = ctypes.c_double(0.0)
+    p = ctypes.c_double(0.0)
+    st = lib().or_lsd_refine(t, y, t.size, float(lo), float(hi), float(w0), float(rtol),
+                             int(max_iter), ctypes.byref(w), ctypes.byref(p))
+    return w.value, p.value, st
+
+
+def lsd_peaks_refined(times, mags, freqs, idx, rtol=1e-15, max_iter=200):
+    """Refine grid peaks idx[c][k] (indices into freqs) as the device pipeline
+    does: bracket [f(k-1), f(k+1)], start f(k), omega = 2 pi f.
+    Returns (freq, power, status) arrays shaped like idx (status 3: no peak)."""
+    F = _f64(freqs)
+    C, K = idx.shape
+    fr = np.zeros((C, K))
+    pw = np.zeros((C, K))
+    st = np.full((C, K), 3, np.int32)
+    for c in range(C):
+        t, y = _f64(times[c]), _f64(mags[c])
+        for k in range(K):
+            i = int(idx[c, k])
+            if i <= 0 or i >= F.size - 1:
+                continue
+            a, b = 2 * np.pi * F[i - 1], 2 * np.pi * F[i + 1]
+            w, p, s = lsd_refine(t, y, min(a, b), max(a, b), 2 * np.pi * F[i], rtol, max_iter)
+            fr[c, k], pw[c, k], st[c, k] = w / (2 * np.pi), p, s
+    return fr, pw, st

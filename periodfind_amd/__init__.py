@@ -27,6 +27,9 @@ _default_device = None  # None: auto-detect
 
 def _gpu_available():
     try:
+        from periodfind_amd import _hipruntime
+
+        _hipruntime.preload()
         from periodfind_amd import _pfhip
     except ImportError:
         return False

@@ -62,6 +62,11 @@ cdef extern from "periodfind_amd.h":
     int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
                           int64_t n_peaks, int64_t min_distance, int use_max, int64_t* idx,
                           float* val) nogil
+    int pf_ls_peaks_refined(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets,
+                            const float* t, const float* y, const double* freqs,
+                            int64_t n_freqs, int64_t n_peaks, int64_t min_distance, double rtol,
+                            int max_iter, int64_t* idx, float* val, double* freq_out,
+                            double* power_out, int* status) nogil
     int pf_stats(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
                  const int64_t* offsets, const float* t, const float* y, const float* err,
                  const float* periods, int64_t n_periods, const float* period_dts,
@@ -227,6 +232,38 @@ cdef class Context:
         if rc != 0:
             _raise(rc)
         return idx, val
+
+    def ls_peaks_refined(self, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y,
+                         cnp.ndarray freqs, int64_t n_peaks, int64_t min_distance, double rtol,
+                         int max_iter):
+        """LS grid over freqs, device peaks, FP64 Newton refinement ->
+        (idx, val, freq, power, status), each [C, n_peaks]."""
+        cdef int64_t C = off.shape[0] - 1
+        if freqs.dtype != np.float64 or not freqs.flags.c_contiguous:
+            raise TypeError("freqs must be a C-contiguous float64 array")
+        idx = np.full((C, n_peaks), -1, dtype=np.int64)
+        val = np.zeros((C, n_peaks), dtype=np.float32)
+        fr = np.zeros((C, n_peaks), dtype=np.float64)
+        pw = np.zeros((C, n_peaks), dtype=np.float64)
+        st = np.zeros((C, n_peaks), dtype=np.int32)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef double* frp = <double*> cnp.PyArray_DATA(fr)
+        cdef double* pwp = <double*> cnp.PyArray_DATA(pw)
+        cdef int* stp = <int*> cnp.PyArray_DATA(st)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const double* fp = <const double*> cnp.PyArray_DATA(freqs)
+        cdef int64_t NF = freqs.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_ls_peaks_refined(self.ctx, C, o, tp, yp, fp, NF, n_peaks, min_distance,
+                                         rtol, max_iter, ip, vp, frp, pwp, stp)
+        if rc != 0:
+            _raise(rc)
+        return idx, val, fr, pw, st
 
     def find_top_peaks(self, cnp.ndarray data, int64_t n_peaks, int64_t min_distance,
                        bint use_max):

@@ -27,7 +27,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", f"-I{INCLUDE}",
 ]
 
-SOURCES = ["pf_grid.hip", "pf_prep.hip", "pf_peaks.hip", "pf_stats.hip", "pf_runtime.cpp"]
+SOURCES = ["pf_grid.hip", "pf_prep.hip", "pf_peaks.hip", "pf_stats.hip", "pf_lsdiff.hip",
+           "pf_runtime.cpp"]
 
 
 def _compile(src):

@@ -27,8 +27,10 @@ PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
+_dbl = ctypes.c_double
 _int = ctypes.c_int
 _flt = ctypes.c_float
 
@@ -93,6 +95,16 @@ SIGNATURES = {
                         _f32p, _i64, _f32p, _i64, _int, _f32p, _i64p, _f32p, _i32p]),
     "pf_curves_stats": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
                                _int, _f32p, _i64p, _f32p, _i32p]),
+    # Differentiable LS: device pointers (passed as integers) and a stream handle.
+    "pf_ls_jet": (_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "pf_ls_backward": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                              _vp]),
+    "pf_ls_refine": (_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _dbl,
+                            _int]),
+    "pf_ls_peaks_refined": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f64p, _i64, _i64, _i64, _dbl,
+                                   _int, _i64p, _f32p, _f64p, _f64p, _i32p]),
+    "pf_curves_ls_peaks_refined": (_int, [_vp, _vp, _f64p, _i64, _i64, _i64, _dbl, _int, _i64p,
+                                          _f32p, _f64p, _f64p, _i32p]),
     "pf_last_kernel_ms": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
 }
 
@@ -112,6 +124,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing; build it with `python -m periodfind_amd.build`")
+        from periodfind_amd import _hipruntime
+
+        _hipruntime.preload()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -225,6 +240,18 @@ class Context:
                              n_stats, summ.reshape(-1), idx.reshape(-1), val.reshape(-1), fb))
         return summ, idx, val, fb
 
+    def ls_peaks_refined(self, times, mags, freqs, n_peaks=8, min_distance=1, rtol=1e-13,
+                         max_iter=60):
+        """f32 LS grid over `freqs`, device peaks, FP64 Newton refinement.
+        Returns (idx, val, freq, power, status), each [n_curves][n_peaks]."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        F = np.ascontiguousarray(freqs, dtype=np.float64)
+        out = _refine_outputs(len(times), n_peaks)
+        check(lib().pf_ls_peaks_refined(self.h, len(times), off, t, y, F, F.size, n_peaks,
+                                        min_distance, rtol, max_iter, *(a.reshape(-1) for a in out)))
+        return out
+
     def find_top_peaks(self, data, n_peaks=32, min_distance=1, use_max=True):
         data = np.ascontiguousarray(data, dtype=np.float32)
         C = data.shape[0]
@@ -234,6 +261,11 @@ class Context:
         check(lib().pf_find_top_peaks(self.h, data.reshape(-1), C, T, n_peaks, min_distance,
                                       int(bool(use_max)), idx.reshape(-1), val.reshape(-1)))
         return idx, val
+
+
+def _refine_outputs(C, n):
+    return (np.zeros((C, n), np.int64), np.zeros((C, n), np.float32), np.zeros((C, n), np.float64),
+            np.zeros((C, n), np.float64), np.zeros((C, n), np.int32))
 
 
 def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
@@ -296,3 +328,12 @@ class CurveSet:
         check(fn(self.ctx.h, self.h, ctypes.byref(prm), P, P.size, D, D.size, n_peaks,
                  min_distance, idx.reshape(-1), val.reshape(-1)))
         return idx, val
+
+    def ls_peaks_refined(self, freqs, n_peaks=8, min_distance=1, rtol=1e-13, max_iter=60):
+        """pf_curves_ls_peaks_refined (magnitudes must have been prepared for LS)."""
+        F = np.ascontiguousarray(freqs, dtype=np.float64)
+        out = _refine_outputs(self.n_curves, n_peaks)
+        check(lib().pf_curves_ls_peaks_refined(self.ctx.h, self.h, F, F.size, n_peaks,
+                                               min_distance, rtol, max_iter,
+                                               *(a.reshape(-1) for a in out)))
+        return out

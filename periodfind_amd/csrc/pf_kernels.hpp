@@ -108,4 +108,21 @@ hipError_t launch_peaks(const float* data, int64_t n_curves, int64_t T, int64_t 
                         int64_t min_distance, int use_max, int64_t* idx, float* val,
                         hipStream_t s);
 
+// Differentiable Lomb-Scargle in FP64 (pf_lsdiff.hip).  y is float when
+// y_f32, else double; offsets, t, y, omega [n_curves][K] are device arrays.
+hipError_t launch_ls_jet(const int64_t* off, const double* t, const void* y, bool y_f32,
+                         const double* omega, int64_t n_curves, int64_t K, double* p, double* dp,
+                         double* d2p, double* coef, hipStream_t s);
+hipError_t launch_ls_refine(const int64_t* off, const double* t, const void* y, bool y_f32,
+                            int64_t n_curves, int64_t K, const double* lo, const double* hi,
+                            double* omega, double* p, int* status, double rtol, int max_iter,
+                            hipStream_t s);
+hipError_t launch_ls_backward(const int64_t* off, const double* t, const double* omega,
+                              int64_t n_curves, int64_t max_len, int64_t K, const double* grad_p,
+                              const double* coef, const double* dp, double* grad_y,
+                              double* grad_omega, hipStream_t s);
+hipError_t launch_ls_bracket(const int64_t* idx, const double* freq, int64_t nf, int64_t n,
+                             double* lo, double* hi, double* w, double* p, int* status,
+                             hipStream_t s);
+
 }  // namespace pf

@@ -153,6 +153,8 @@ struct pf_ctx {
     bool has_pdt = false;
     // Peak / stats outputs.
     DevBuf pidx, pval, summ, flags;
+    // LS refinement: frequency grid (f64) and per-peak bracket / result arrays.
+    DevBuf ls_freq, ls_lo, ls_hi, ls_w, ls_p, ls_status;
 };
 
 struct pf_curves {
@@ -652,6 +654,122 @@ int grid_host(pf_ctx* ctx, const Params& p, int64_t n_curves, const int64_t* off
     return grid_on(ctx, ctx->scratch, p, periods, np, pdts, nd, out);
 }
 
+// --- Differentiable Lomb-Scargle -------------------------------------------
+
+// A device-pointer argument must be device memory of ctx's device.
+int check_dev(pf_ctx* ctx, const void* ptr, const char* name) {
+    if (!ptr) return fail(PF_E_ARG, "%s is NULL", name);
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PF_E_ARG, "%s is not HIP device memory", name);
+    }
+    if (a.type != hipMemoryTypeDevice && !a.isManaged)
+        return fail(PF_E_ARG, "%s is not HIP device memory", name);
+    if (a.device != ctx->device)
+        return fail(PF_E_ARG, "%s lives on device %d, the context on device %d", name, a.device,
+                    ctx->device);
+    return PF_OK;
+}
+
+int check_dev_opt(pf_ctx* ctx, const void* ptr, const char* name) {
+    return ptr ? check_dev(ctx, ptr, name) : PF_OK;
+}
+
+// Makes ctx's device current for the scope of a device-pointer call and
+// restores the caller's device afterwards (PyTorch tracks its own).
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Grid launches are 1-D: n_curves * blocks-per-curve must fit one dimension.
+int check_blocks(int64_t n_curves, int64_t per_curve) {
+    if (per_curve > 0 && n_curves > (int64_t(INT32_MAX) / per_curve))
+        return fail(PF_E_LIMIT, "too many curves x blocks for one launch");
+    return PF_OK;
+}
+
+int64_t freq_blocks(int64_t K) {
+    int64_t kb = 1;
+    while (kb < K && kb < 64) kb <<= 1;
+    return (K + kb - 1) / kb;
+}
+
+// LS grid over freqs (as f32 periods 1/f), device peaks, then Newton
+// refinement of every peak inside its grid-neighbour bracket.
+int ls_refine_on(pf_ctx* ctx, CurveSet& cs, const double* freqs, int64_t nf, int64_t n_peaks,
+                 int64_t min_distance, double rtol, int max_iter, int64_t* idx, float* val,
+                 double* f_out, double* p_out, int* status) {
+    if (!freqs && nf > 0) return fail(PF_E_ARG, "freqs is NULL");
+    if (nf < 0) return fail(PF_E_ARG, "n_freqs < 0");
+    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
+    if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
+    if (!(rtol > 0.0) || max_iter < 1) return fail(PF_E_ARG, "rtol must be > 0, max_iter >= 1");
+    const int64_t C = cs.n_curves;
+    const size_t pk = static_cast<size_t>(C) * n_peaks;
+    if (pk == 0) return PF_OK;
+    if (!idx || !val || !f_out || !p_out || !status) return fail(PF_E_ARG, "an output is NULL");
+    std::vector<float> periods(static_cast<size_t>(nf));
+    for (int64_t k = 0; k < nf; ++k) {
+        if (!(freqs[k] > 0.0) || !std::isfinite(freqs[k]))
+            return fail(PF_E_ARG, "freqs must be finite and > 0 (index %lld)",
+                        static_cast<long long>(k));
+        periods[k] = static_cast<float>(1.0 / freqs[k]);
+    }
+    if (nf < 3) {
+        for (size_t i = 0; i < pk; ++i) {
+            idx[i] = -1;
+            val[i] = 0.0f;
+            f_out[i] = 0.0;
+            p_out[i] = 0.0;
+            status[i] = 3;
+        }
+        return PF_OK;
+    }
+    PF_TRY(check_blocks(C, freq_blocks(n_peaks)));
+    const float zero = 0.0f;
+    Params p{PF_ALGO_LS, 0, 0, 0, 0, 0, 0.0f, 0.0f};
+    PF_TRY(upload_grid(ctx, periods.data(), nf, &zero, 1));
+    PF_TRY(run_grid(ctx, cs, p, nf, 1));
+    hipStream_t s = ctx->stream;
+    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    PF_HIP(pf::launch_peaks(as<float>(cs.out), C, nf, n_peaks, min_distance, 1,
+                            as<int64_t>(ctx->pidx), as<float>(ctx->pval), s));
+    PF_TRY(ensure(ctx->ls_freq, nf * sizeof(double)));
+    PF_TRY(ensure(ctx->ls_lo, pk * sizeof(double)));
+    PF_TRY(ensure(ctx->ls_hi, pk * sizeof(double)));
+    PF_TRY(ensure(ctx->ls_w, pk * sizeof(double)));
+    PF_TRY(ensure(ctx->ls_p, pk * sizeof(double)));
+    PF_TRY(ensure(ctx->ls_status, pk * sizeof(int)));
+    PF_HIP(hipMemcpyAsync(ctx->ls_freq.p, freqs, nf * sizeof(double), hipMemcpyHostToDevice, s));
+    PF_HIP(pf::launch_ls_bracket(as<int64_t>(ctx->pidx), as<double>(ctx->ls_freq), nf,
+                                 static_cast<int64_t>(pk), as<double>(ctx->ls_lo),
+                                 as<double>(ctx->ls_hi), as<double>(ctx->ls_w),
+                                 as<double>(ctx->ls_p), as<int>(ctx->ls_status), s));
+    PF_HIP(pf::launch_ls_refine(as<int64_t>(cs.off), as<double>(cs.t64), cs.y.p, true, C, n_peaks,
+                                as<double>(ctx->ls_lo), as<double>(ctx->ls_hi),
+                                as<double>(ctx->ls_w), as<double>(ctx->ls_p),
+                                as<int>(ctx->ls_status), rtol, max_iter, s));
+    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(f_out, ctx->ls_w.p, pk * sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(p_out, ctx->ls_p.p, pk * sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(status, ctx->ls_status.p, pk * sizeof(int), hipMemcpyDeviceToHost, s));
+    PF_TRY(finish(ctx));
+    constexpr double kTwoPi = 6.283185307179586476925286766559;
+    for (size_t i = 0; i < pk; ++i) f_out[i] /= kTwoPi;
+    return PF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -703,7 +821,8 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->pidx, &ctx->pval, &ctx->summ,
-                      &ctx->flags};
+                      &ctx->flags, &ctx->ls_freq, &ctx->ls_lo, &ctx->ls_hi, &ctx->ls_w,
+                      &ctx->ls_p, &ctx->ls_status};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -891,6 +1010,99 @@ int pf_curves_peaks_ragged(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
     PF_HIP(hipSetDevice(ctx->device));
     return ragged_peaks_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts,
                            n_pdts, n_peaks, min_distance, idx, val);
+}
+
+int pf_ls_jet(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+              const double* t, const double* y, const double* omega, int64_t K, double* p,
+              double* dp, double* d2p, double* coef) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    if (n_curves < 0 || K < 0) return fail(PF_E_ARG, "negative size");
+    if (n_curves == 0 || K == 0) return PF_OK;
+    PF_TRY(check_blocks(n_curves, freq_blocks(K)));
+    DeviceGuard dg(ctx->device);
+    PF_TRY(check_dev(ctx, offsets, "offsets"));
+    PF_TRY(check_dev(ctx, t, "t"));
+    PF_TRY(check_dev(ctx, y, "y"));
+    PF_TRY(check_dev(ctx, omega, "omega"));
+    PF_TRY(check_dev(ctx, p, "p"));
+    PF_TRY(check_dev_opt(ctx, dp, "dp"));
+    PF_TRY(check_dev_opt(ctx, d2p, "d2p"));
+    PF_TRY(check_dev_opt(ctx, coef, "coef"));
+    PF_HIP(pf::launch_ls_jet(offsets, t, y, false, omega, n_curves, K, p, dp, d2p, coef,
+                             static_cast<hipStream_t>(stream)));
+    return PF_OK;
+}
+
+int pf_ls_backward(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+                   int64_t max_len, const double* t, const double* omega, int64_t K,
+                   const double* grad_p, const double* coef, const double* dp, double* grad_y,
+                   double* grad_omega) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    if (n_curves < 0 || K < 0 || max_len < 0) return fail(PF_E_ARG, "negative size");
+    if (n_curves == 0) return PF_OK;
+    const int64_t pblocks = max_len > 0 ? (max_len + 255) / 256 : 1;
+    PF_TRY(check_blocks(n_curves, pblocks));
+    DeviceGuard dg(ctx->device);
+    PF_TRY(check_dev(ctx, offsets, "offsets"));
+    PF_TRY(check_dev(ctx, t, "t"));
+    PF_TRY(check_dev(ctx, grad_y, "grad_y"));
+    if (K > 0) {
+        PF_TRY(check_dev(ctx, omega, "omega"));
+        PF_TRY(check_dev(ctx, grad_p, "grad_p"));
+        PF_TRY(check_dev(ctx, coef, "coef"));
+    }
+    if (grad_omega && K > 0) {
+        PF_TRY(check_dev(ctx, grad_omega, "grad_omega"));
+        PF_TRY(check_dev(ctx, dp, "dp"));
+    }
+    PF_HIP(pf::launch_ls_backward(offsets, t, omega, n_curves, max_len, K, grad_p, coef, dp,
+                                  grad_y, K > 0 ? grad_omega : nullptr,
+                                  static_cast<hipStream_t>(stream)));
+    return PF_OK;
+}
+
+int pf_ls_refine(pf_ctx* ctx, void* stream, int64_t n_curves, const int64_t* offsets,
+                 const double* t, const double* y, int64_t K, const double* lo, const double* hi,
+                 double* omega, double* p, int* status, double rtol, int max_iter) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    if (n_curves < 0 || K < 0) return fail(PF_E_ARG, "negative size");
+    if (!(rtol > 0.0) || max_iter < 1) return fail(PF_E_ARG, "rtol must be > 0, max_iter >= 1");
+    if (n_curves == 0 || K == 0) return PF_OK;
+    PF_TRY(check_blocks(n_curves, freq_blocks(K)));
+    DeviceGuard dg(ctx->device);
+    PF_TRY(check_dev(ctx, offsets, "offsets"));
+    PF_TRY(check_dev(ctx, t, "t"));
+    PF_TRY(check_dev(ctx, y, "y"));
+    PF_TRY(check_dev(ctx, lo, "lo"));
+    PF_TRY(check_dev(ctx, hi, "hi"));
+    PF_TRY(check_dev(ctx, omega, "omega"));
+    PF_TRY(check_dev(ctx, p, "p"));
+    PF_TRY(check_dev(ctx, status, "status"));
+    PF_HIP(pf::launch_ls_refine(offsets, t, y, false, n_curves, K, lo, hi, omega, p, status, rtol,
+                                max_iter, static_cast<hipStream_t>(stream)));
+    return PF_OK;
+}
+
+int pf_ls_peaks_refined(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                        const float* y, const double* freqs, int64_t n_freqs, int64_t n_peaks,
+                        int64_t min_distance, double rtol, int max_iter, int64_t* idx,
+                        float* val, double* freq_out, double* power_out, int* status) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, nullptr));
+    return ls_refine_on(ctx, ctx->scratch, freqs, n_freqs, n_peaks, min_distance, rtol, max_iter,
+                        idx, val, freq_out, power_out, status);
+}
+
+int pf_curves_ls_peaks_refined(pf_ctx* ctx, pf_curves* cv, const double* freqs, int64_t n_freqs,
+                               int64_t n_peaks, int64_t min_distance, double rtol, int max_iter,
+                               int64_t* idx, float* val, double* freq_out, double* power_out,
+                               int* status) {
+    if (!ctx || !cv) return fail(PF_E_ARG, "ctx or curve set is NULL");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    return ls_refine_on(ctx, cv->cs, freqs, n_freqs, n_peaks, min_distance, rtol, max_iter, idx,
+                        val, freq_out, power_out, status);
 }
 
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms) {

@@ -15,7 +15,11 @@ import threading
 
 import numpy as np
 
-from periodfind_amd import _pfhip
+from periodfind_amd import _hipruntime
+
+_hipruntime.preload()
+
+from periodfind_amd import _pfhip  # noqa: E402  (after the runtime preload)
 from periodfind_amd._utils import ensure_float32, prepare_magnitudes, validate_errs, validate_inputs
 from periodfind_amd.results import Periodogram, Statistics
 
@@ -175,6 +179,35 @@ class ConditionalEntropy(_Algorithm):
         return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
+    def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
+                     normalize=False, center=True, rtol=1e-13, max_iter=60):
+        """Top `n_peaks` LS maxima per curve over the frequency grid `freqs`
+        (cycles per time unit, monotone), each refined to sub-grid precision.
+
+        The grid is the float32 periodogram at periods 1/f (period_dts [0]);
+        each peak k is then refined in FP64 by safeguarded Newton on dP/dw
+        inside [f(k-1), f(k+1)].  Returns a list (per curve) of dicts with
+        arrays 'grid_index', 'grid_power' (float32), 'frequency', 'period',
+        'power' (FP64 power at the refined frequency) and 'status' (0 refined,
+        1 no bracketed maximum: left at the grid frequency, 2 not converged,
+        3 no peak in this slot)."""
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        off, t = pack(times)
+        _, y = pack(mags_use)
+        F = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
+        idx, val, fr, pw, st = context(self.device).ls_peaks_refined(
+            off, t, y, F, int(n_peaks), int(min_distance), float(rtol), int(max_iter))
+        out = []
+        for c in range(idx.shape[0]):
+            with np.errstate(divide="ignore"):
+                period = np.where(fr[c] > 0, 1.0 / np.where(fr[c] > 0, fr[c], 1.0), 0.0)
+            out.append({"grid_index": idx[c], "grid_power": val[c], "frequency": fr[c],
+                        "period": period, "power": pw[c], "status": st[c]})
+        return out
+
 
 class AOV(_Algorithm):
     """Analysis of Variance F-statistic (periodfind/aov.pyx; rust/src/aov.rs).  Higher is better."""
@@ -199,6 +232,35 @@ class AOV(_Algorithm):
         return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
+    def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
+                     normalize=False, center=True, rtol=1e-13, max_iter=60):
+        """Top `n_peaks` LS maxima per curve over the frequency grid `freqs`
+        (cycles per time unit, monotone), each refined to sub-grid precision.
+
+        The grid is the float32 periodogram at periods 1/f (period_dts [0]);
+        each peak k is then refined in FP64 by safeguarded Newton on dP/dw
+        inside [f(k-1), f(k+1)].  Returns a list (per curve) of dicts with
+        arrays 'grid_index', 'grid_power' (float32), 'frequency', 'period',
+        'power' (FP64 power at the refined frequency) and 'status' (0 refined,
+        1 no bracketed maximum: left at the grid frequency, 2 not converged,
+        3 no peak in this slot)."""
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        off, t = pack(times)
+        _, y = pack(mags_use)
+        F = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
+        idx, val, fr, pw, st = context(self.device).ls_peaks_refined(
+            off, t, y, F, int(n_peaks), int(min_distance), float(rtol), int(max_iter))
+        out = []
+        for c in range(idx.shape[0]):
+            with np.errstate(divide="ignore"):
+                period = np.where(fr[c] > 0, 1.0 / np.where(fr[c] > 0, fr[c], 1.0), 0.0)
+            out.append({"grid_index": idx[c], "grid_power": val[c], "frequency": fr[c],
+                        "period": period, "power": pw[c], "status": st[c]})
+        return out
+
 
 class LombScargle(_Algorithm):
     """Lomb-Scargle power (periodfind/ls.pyx; rust/src/ls.rs).  Higher is better;
@@ -218,6 +280,35 @@ class LombScargle(_Algorithm):
         mags_use = prepare_magnitudes(mags, center, normalize)
         return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
+
+    def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
+                     normalize=False, center=True, rtol=1e-13, max_iter=60):
+        """Top `n_peaks` LS maxima per curve over the frequency grid `freqs`
+        (cycles per time unit, monotone), each refined to sub-grid precision.
+
+        The grid is the float32 periodogram at periods 1/f (period_dts [0]);
+        each peak k is then refined in FP64 by safeguarded Newton on dP/dw
+        inside [f(k-1), f(k+1)].  Returns a list (per curve) of dicts with
+        arrays 'grid_index', 'grid_power' (float32), 'frequency', 'period',
+        'power' (FP64 power at the refined frequency) and 'status' (0 refined,
+        1 no bracketed maximum: left at the grid frequency, 2 not converged,
+        3 no peak in this slot)."""
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mags_use = prepare_magnitudes(mags, center, normalize)
+        off, t = pack(times)
+        _, y = pack(mags_use)
+        F = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
+        idx, val, fr, pw, st = context(self.device).ls_peaks_refined(
+            off, t, y, F, int(n_peaks), int(min_distance), float(rtol), int(max_iter))
+        out = []
+        for c in range(idx.shape[0]):
+            with np.errstate(divide="ignore"):
+                period = np.where(fr[c] > 0, 1.0 / np.where(fr[c] > 0, fr[c], 1.0), 0.0)
+            out.append({"grid_index": idx[c], "grid_power": val[c], "frequency": fr[c],
+                        "period": period, "power": pw[c], "status": st[c]})
+        return out
 
 
 class FPW(_Algorithm):

@@ -189,3 +189,14 @@ class TestGpuClassValidation:
         with pytest.raises(ValueError, match="different lengths"):
             gpu.FPW().calc(t, m, np.ones(3, np.float32), np.zeros(1, np.float32),
                            errs=[np.ones(5, np.float32)])
+
+
+def test_autograd_refuses_cpu_tensors():
+    """The differentiable LS has no CPU path: CPU tensors raise before any
+    library call."""
+    torch = pytest.importorskip("torch")
+    from periodfind_amd import autograd as AG
+
+    t = torch.linspace(0, 10, 20, dtype=torch.float64)
+    with pytest.raises(ValueError, match="GPU tensors"):
+        AG.ls_power(t, torch.sin(t), torch.tensor([1.0, 2.0], dtype=torch.float64))

@@ -61,3 +61,23 @@ def test_null_context_is_an_error_not_a_crash():
     D = np.zeros(1, np.float32)
     rc = capi.lib().pf_ls_grid(None, 1, off, t, t, P, 3, D, 1, out)
     assert rc == capi.PF_E_ARG
+
+
+def test_one_hip_runtime_with_torch():
+    """Loading the library and then torch maps a single HIP runtime: two
+    runtimes in one process leave the second without GPUs."""
+    import subprocess
+    import sys
+
+    pytest.importorskip("torch")
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from periodfind_amd import capi; capi.lib()\n"
+        "import periodfind_amd.gpu\n"
+        "import torch\n"
+        "maps = open('/proc/self/maps').read().splitlines()\n"
+        "libs = {l.split()[-1] for l in maps if 'libamdhip64' in l}\n"
+        "print(len(libs))\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "1", r.stdout
