@@ -273,3 +273,27 @@ def test_ragged_persistent_graph_peaks(ctx, algo):
         np.testing.assert_array_equal(gi, oi)
         np.testing.assert_array_equal(gv, ov)
     cs.close()
+
+
+@pytest.mark.parametrize("algo", ["ce", "bls"])
+def test_heavy_shape_subsample(ctx, algo):
+    """The heavy workload's shape: 65,536 points over a 3000-day baseline,
+    log-spaced periods in [0.05, 100].  Two curves on every 40th period of the
+    100,000-period grid (2,500 trials) against the oracle, full grid and peaks."""
+    t, m, e, P, D = synth.config5(n_curves=2, n_points=65536, n_periods=100000)
+    P = np.ascontiguousarray(P[::40])
+    m = _prep(algo, m)
+    ref = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+    got = _gpu(ctx, algo, t, m, e, P, D, **PARAMS[algo])
+    _check(algo, got, ref)
+    gi, gv = ctx.peaks(algo, t, m, P, D, errs=e if algo == "bls" else None, n_peaks=32,
+                       min_distance=1, **PARAMS[algo])
+    # Device peak selection is exact on the device grid; for BLS the grid is
+    # bit-identical too (CE values carry the device ln's rounding).
+    oi, ov = O.find_top_peaks_batched(got, 32, 1, use_max=algo != "ce")
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gv, ov)
+    if algo == "bls":
+        oi, ov = O.find_top_peaks_batched(ref, 32, 1, use_max=True)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gv, ov)
