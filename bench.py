@@ -187,6 +187,14 @@ def run_ragged(args):
             gi, gv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
             best = min(best, time.perf_counter() - t0)
         graph_ms = ctx.last_kernel_ms()
+        tuned = [e for e in ctx.get_tuning() if e[0] == algo]
+        saved = ctx.get_tuning()
+        ctx.set_tuning([])  # built-in defaults, for comparison
+        cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
+        t0 = time.perf_counter()
+        cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
+        default_s = time.perf_counter() - t0
+        ctx.set_tuning(saved)
         print(f"bench: {algo} plain single launch", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         pi, pv = cs.peaks(algo, P, D, 32, 1, ragged=False, **prm)
@@ -204,6 +212,8 @@ def run_ragged(args):
         evals = npts * P.size * D.size
         out["algos"][algo] = {
             "ragged_s": best, "ragged_graph_ms": graph_ms, "plain_single_launch_s": plain_s,
+            "ragged_untuned_s": default_s, "tuning": tuned,
+            "tuning_source": ctx.tuning_source(),
             "per_call_s": percall_s, "speedup_vs_per_call": percall_s / best,
             "speedup_vs_plain_launch": plain_s / best,
             "pts_per_s": npts / best, "evals_per_s": evals / best,

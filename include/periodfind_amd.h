@@ -61,9 +61,46 @@ const char* pf_last_error(void);
 
 /* Create / destroy a context bound to one device.  The context owns a stream,
  * device buffers that grow on demand and are reused across calls, and the
- * per-length-bucket tuning table. */
+ * per-length-bucket tuning table (see pf_tune_entry).  On creation the table
+ * is read from the file named by $PF_TUNE_TABLE ("none": no table), else from
+ * tuning_<arch>.txt next to this library when present (e.g. tuning_gfx950.txt);
+ * an unreadable or invalid table makes creation fail. */
 int pf_ctx_create(pf_ctx** ctx, int device);
 void pf_ctx_destroy(pf_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Launch tuning.  No counterpart in the reference (fixed launch shapes,     */
+/* cuda/ce.cu:509-538).  For one statistic and curves of at least min_len    */
+/* points (the entry with the largest min_len <= length applies):            */
+/*   group: CE 0 = LDS u32 atomics, 1/2/4 = read-modify-write groups of that  */
+/*          many points; AOV/FPW/BLS 1/2/4 (read-modify-write); LS: unused.  */
+/*   claim: most work items a persistent (ragged) launch takes per atomic    */
+/*          claim, 1..64; claims are guided below it (remaining work / twice */
+/*          the number of waves), so they shrink as the queue drains.        */
+/* Results are identical for every choice; only speed differs.  The plain    */
+/* launch uses the entry of the work-weighted mean length; the ragged launch */
+/* runs one persistent kernel per length bucket.  At most 31 entries per     */
+/* statistic.  Text form (pf_ctx_load_tuning): "<algo> <min_len> <group>     */
+/* <claim>" per line, algo in ce/aov/ls/fpw/bls, '#' comments.               */
+/* ------------------------------------------------------------------------ */
+typedef struct pf_tune_entry {
+    int algo;        /* PF_ALGO_* */
+    int group;
+    int claim;
+    int reserved;    /* 0 */
+    int64_t min_len;
+} pf_tune_entry;
+
+/* Replace the table (n = 0: built-in defaults, group 2 / CE atomics, claim 16). */
+int pf_ctx_set_tuning(pf_ctx* ctx, const pf_tune_entry* entries, int64_t n);
+/* Copy up to cap entries; *n = number of entries in the table. */
+int pf_ctx_get_tuning(pf_ctx* ctx, pf_tune_entry* entries, int64_t cap, int64_t* n);
+/* Replace the table from a text file. */
+int pf_ctx_load_tuning(pf_ctx* ctx, const char* path);
+/* Where the table came from: a path, "set", or "" (defaults). */
+const char* pf_ctx_tuning_source(pf_ctx* ctx);
+/* Architecture of the context's device, e.g. "gfx950". */
+const char* pf_ctx_arch(pf_ctx* ctx);
 
 /* ------------------------------------------------------------------------ */
 /* Plain drop-in grid evaluation (host in, host out).                        */

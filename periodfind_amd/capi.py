@@ -35,6 +35,16 @@ _int = ctypes.c_int
 _flt = ctypes.c_float
 
 
+class PfTuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("algo", ctypes.c_int),
+        ("group", ctypes.c_int),
+        ("claim", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("min_len", ctypes.c_int64),
+    ]
+
+
 class PfParams(ctypes.Structure):
     _fields_ = [
         ("algo", ctypes.c_int),
@@ -105,8 +115,16 @@ SIGNATURES = {
                                    _int, _i64p, _f32p, _f64p, _f64p, _i32p]),
     "pf_curves_ls_peaks_refined": (_int, [_vp, _vp, _f64p, _i64, _i64, _i64, _dbl, _int, _i64p,
                                           _f32p, _f64p, _f64p, _i32p]),
+    "pf_ctx_set_tuning": (_int, [_vp, ctypes.POINTER(PfTuneEntry), _i64]),
+    "pf_ctx_get_tuning": (_int, [_vp, ctypes.POINTER(PfTuneEntry), _i64,
+                                 ctypes.POINTER(_i64)]),
+    "pf_ctx_load_tuning": (_int, [_vp, ctypes.c_char_p]),
+    "pf_ctx_tuning_source": (ctypes.c_char_p, [_vp]),
+    "pf_ctx_arch": (ctypes.c_char_p, [_vp]),
     "pf_last_kernel_ms": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
 }
+
+ALGO_NAMES = {v: k for k, v in PF_ALGO.items()}
 
 _lib = None
 
@@ -173,6 +191,30 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_tuning(self, entries):
+        """entries: iterable of (algo name, min_len, group, claim); [] = defaults."""
+        entries = list(entries)
+        arr = (PfTuneEntry * max(1, len(entries)))()
+        for i, (algo, min_len, group, claim) in enumerate(entries):
+            arr[i] = PfTuneEntry(PF_ALGO[algo], int(group), int(claim), 0, int(min_len))
+        check(lib().pf_ctx_set_tuning(self.h, arr, len(entries)))
+
+    def get_tuning(self):
+        n = _i64(0)
+        check(lib().pf_ctx_get_tuning(self.h, None, 0, ctypes.byref(n)))
+        arr = (PfTuneEntry * max(1, n.value))()
+        check(lib().pf_ctx_get_tuning(self.h, arr, n.value, ctypes.byref(n)))
+        return [(ALGO_NAMES[e.algo], e.min_len, e.group, e.claim) for e in arr[:n.value]]
+
+    def load_tuning(self, path):
+        check(lib().pf_ctx_load_tuning(self.h, os.fsencode(path)))
+
+    def tuning_source(self):
+        return lib().pf_ctx_tuning_source(self.h).decode()
+
+    def arch(self):
+        return lib().pf_ctx_arch(self.h).decode()
 
     def last_kernel_ms(self):
         ms = ctypes.c_float(0.0)

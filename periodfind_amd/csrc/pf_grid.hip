@@ -44,12 +44,45 @@ __device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tbloc
 
 // Persistent-launch work queue: item = rank * tblocks + trial block, where
 // rank orders curves longest first (so the longest items start first and
-// the tail is short); waves claim `claim` consecutive items per atomic.
-__device__ __forceinline__ int64_t next_item(const WorkQueue& q) {
+// the tail is short).  Claims are guided: a wave takes remaining / (2 * waves)
+// items (at least 1, at most the claim bound of the bucket it is in),
+// estimated from the counter value it saw last, so claims are large while the
+// queue is full and single items at the end, with few atomics on the shared
+// counter.  `seen` is wave-uniform.
+struct Claim {
+    int64_t begin, end;
+};
+
+// Binning variant and claim bound of a rank's bucket (buckets are contiguous
+// rank ranges; wave-uniform).  Constant indices only, so the queue stays in
+// kernel-argument registers.
+__device__ __forceinline__ int rank_group(const WorkQueue& q, int64_t rank) {
+    int v = q.bucket_group[0];
+#pragma unroll
+    for (int i = 1; i < kQueueBuckets; ++i)
+        if (i < q.n_buckets && rank >= q.bucket_end[i - 1]) v = q.bucket_group[i];
+    return v;
+}
+
+__device__ __forceinline__ int rank_claim(const WorkQueue& q, int64_t rank) {
+    int v = q.bucket_claim[0];
+#pragma unroll
+    for (int i = 1; i < kQueueBuckets; ++i)
+        if (i < q.n_buckets && rank >= q.bucket_end[i - 1]) v = q.bucket_claim[i];
+    return v;
+}
+
+__device__ __forceinline__ Claim next_claim(const WorkQueue& q, const GridArgs& g,
+                                            int64_t& seen) {
+    const int64_t bound = rank_claim(q, seen / g.tblocks);
+    int64_t c = (q.total - seen) / (2 * static_cast<int64_t>(gridDim.x));
+    c = c < 1 ? 1 : (c > bound ? bound : c);
     unsigned long long v = 0;
-    if (threadIdx.x == 0) v = atomicAdd(q.counter, static_cast<unsigned long long>(q.claim));
+    if (threadIdx.x == 0) v = atomicAdd(q.counter, static_cast<unsigned long long>(c));
     v = __shfl(v, 0);
-    return static_cast<int64_t>(v) < q.total ? static_cast<int64_t>(v) : -1;
+    const int64_t b = static_cast<int64_t>(v);
+    seen = b + c;
+    return Claim{b, b + c < q.total ? b + c : q.total};
 }
 
 __device__ __forceinline__ LaneTrial queue_trial(const WorkQueue& q, const GridArgs& g,
@@ -254,12 +287,23 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
 }
 
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
-template <bool HAS_PDT, bool EXT1, int G>
+template <bool HAS_PDT, bool EXT1>
 __global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, WorkQueue q) {
-    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
-        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
-        for (; it < stop; ++it) ce_block<HAS_PDT, EXT1, G>(g, a, queue_trial(q, g, it));
-    }
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            if (!EXT1) {
+                ce_block<HAS_PDT, false, 1>(g, a, lt);
+                continue;
+            }
+            switch (rank_group(q, it / g.tblocks)) {
+                case 0: ce_block<HAS_PDT, EXT1, 0>(g, a, lt); break;
+                case 2: ce_block<HAS_PDT, EXT1, 2>(g, a, lt); break;
+                case 4: ce_block<HAS_PDT, EXT1, 4>(g, a, lt); break;
+                default: ce_block<HAS_PDT, EXT1, 1>(g, a, lt); break;
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -346,12 +390,22 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
 }
 
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
-template <bool HAS_PDT, bool EXT1, int G>
+template <bool HAS_PDT, bool EXT1>
 __global__ __launch_bounds__(kWave) void aov_persistent(GridArgs g, AovArgs a, WorkQueue q) {
-    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
-        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
-        for (; it < stop; ++it) aov_block<HAS_PDT, EXT1, G>(g, a, queue_trial(q, g, it));
-    }
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            if (!EXT1) {
+                aov_block<HAS_PDT, false, 1>(g, a, lt);
+                continue;
+            }
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: aov_block<HAS_PDT, EXT1, 2>(g, a, lt); break;
+                case 4: aov_block<HAS_PDT, EXT1, 4>(g, a, lt); break;
+                default: aov_block<HAS_PDT, EXT1, 1>(g, a, lt); break;
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -416,10 +470,10 @@ __global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
 template <bool HAS_PDT, bool FAST>
 __global__ __launch_bounds__(kWave) void ls_persistent(GridArgs g, LsArgs a, WorkQueue q) {
-    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
-        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
-        for (; it < stop; ++it) ls_block<HAS_PDT, FAST>(g, a, queue_trial(q, g, it));
-    }
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it)
+            ls_block<HAS_PDT, FAST>(g, a, queue_trial(q, g, it));
 }
 
 // ---------------------------------------------------------------------------
@@ -477,12 +531,18 @@ __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
 }
 
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
-template <bool HAS_PDT, int G>
+template <bool HAS_PDT>
 __global__ __launch_bounds__(kWave) void fpw_persistent(GridArgs g, FpwArgs a, WorkQueue q) {
-    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
-        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
-        for (; it < stop; ++it) fpw_block<HAS_PDT, G>(g, a, queue_trial(q, g, it));
-    }
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: fpw_block<HAS_PDT, 2>(g, a, lt); break;
+                case 4: fpw_block<HAS_PDT, 4>(g, a, lt); break;
+                default: fpw_block<HAS_PDT, 1>(g, a, lt); break;
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -618,12 +678,18 @@ __global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
 }
 
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
-template <bool HAS_PDT, int G>
+template <bool HAS_PDT>
 __global__ __launch_bounds__(kWave) void bls_persistent(GridArgs g, BlsArgs a, WorkQueue q) {
-    for (int64_t it = next_item(q); it >= 0; it = next_item(q)) {
-        const int64_t stop = it + q.claim < q.total ? it + q.claim : q.total;
-        for (; it < stop; ++it) bls_block<HAS_PDT, G>(g, a, queue_trial(q, g, it));
-    }
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: bls_block<HAS_PDT, 2>(g, a, lt); break;
+                case 4: bls_block<HAS_PDT, 4>(g, a, lt); break;
+                default: bls_block<HAS_PDT, 1>(g, a, lt); break;
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -647,51 +713,56 @@ static uint32_t persistent_grid(K kernel, size_t lds) {
     return static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
 }
 
-// Grid or persistent launch of one kernel pair.
-template <class KG, class KP, class Args>
-void launch2(KG kgrid, KP kpers, const GridArgs& g, const Args& a, size_t lds, const WorkQueue* q,
-             hipStream_t s) {
-    if (q) {
-        hipLaunchKernelGGL(kpers, dim3(persistent_grid(kpers, lds)), dim3(kWave), lds, s, g, a,
-                           *q);
-    } else {
-        hipLaunchKernelGGL(kgrid, grid_of(g), dim3(kWave), lds, s, g, a);
-    }
+template <class K, class Args>
+void launch_grid_kernel(K kernel, const GridArgs& g, const Args& a, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(kernel, grid_of(g), dim3(kWave), lds, s, g, a);
 }
 
+template <class K, class Args>
+void launch_persistent(K kernel, const GridArgs& g, const Args& a, size_t lds,
+                       const WorkQueue& q, hipStream_t s) {
+    hipLaunchKernelGGL(kernel, dim3(persistent_grid(kernel, lds)), dim3(kWave), lds, s, g, a, q);
+}
+
+// Plain launches take the binning variant `grp`; persistent launches carry
+// one per length bucket in the queue.
 template <bool P, bool E>
 void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
           hipStream_t s) {
-    if (!E) launch2(ce_kernel<P, false, 1>, ce_persistent<P, false, 1>, g, a, lds, q, s);
-    else if (grp <= 0) launch2(ce_kernel<P, E, 0>, ce_persistent<P, E, 0>, g, a, lds, q, s);
-    else if (grp >= 4) launch2(ce_kernel<P, E, 4>, ce_persistent<P, E, 4>, g, a, lds, q, s);
-    else if (grp == 2) launch2(ce_kernel<P, E, 2>, ce_persistent<P, E, 2>, g, a, lds, q, s);
-    else launch2(ce_kernel<P, E, 1>, ce_persistent<P, E, 1>, g, a, lds, q, s);
+    if (q) launch_persistent(ce_persistent<P, E>, g, a, lds, *q, s);
+    else if (!E) launch_grid_kernel(ce_kernel<P, false, 1>, g, a, lds, s);
+    else if (grp <= 0) launch_grid_kernel(ce_kernel<P, E, 0>, g, a, lds, s);
+    else if (grp >= 4) launch_grid_kernel(ce_kernel<P, E, 4>, g, a, lds, s);
+    else if (grp == 2) launch_grid_kernel(ce_kernel<P, E, 2>, g, a, lds, s);
+    else launch_grid_kernel(ce_kernel<P, E, 1>, g, a, lds, s);
 }
 
 template <bool P, bool E>
 void aov_g(const GridArgs& g, const AovArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
-    if (!E) launch2(aov_kernel<P, false, 1>, aov_persistent<P, false, 1>, g, a, lds, q, s);
-    else if (grp >= 4) launch2(aov_kernel<P, E, 4>, aov_persistent<P, E, 4>, g, a, lds, q, s);
-    else if (grp == 2) launch2(aov_kernel<P, E, 2>, aov_persistent<P, E, 2>, g, a, lds, q, s);
-    else launch2(aov_kernel<P, E, 1>, aov_persistent<P, E, 1>, g, a, lds, q, s);
+    if (q) launch_persistent(aov_persistent<P, E>, g, a, lds, *q, s);
+    else if (!E) launch_grid_kernel(aov_kernel<P, false, 1>, g, a, lds, s);
+    else if (grp >= 4) launch_grid_kernel(aov_kernel<P, E, 4>, g, a, lds, s);
+    else if (grp == 2) launch_grid_kernel(aov_kernel<P, E, 2>, g, a, lds, s);
+    else launch_grid_kernel(aov_kernel<P, E, 1>, g, a, lds, s);
 }
 
 template <bool P>
 void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
-    if (grp >= 4) launch2(fpw_kernel<P, 4>, fpw_persistent<P, 4>, g, a, lds, q, s);
-    else if (grp == 2) launch2(fpw_kernel<P, 2>, fpw_persistent<P, 2>, g, a, lds, q, s);
-    else launch2(fpw_kernel<P, 1>, fpw_persistent<P, 1>, g, a, lds, q, s);
+    if (q) launch_persistent(fpw_persistent<P>, g, a, lds, *q, s);
+    else if (grp >= 4) launch_grid_kernel(fpw_kernel<P, 4>, g, a, lds, s);
+    else if (grp == 2) launch_grid_kernel(fpw_kernel<P, 2>, g, a, lds, s);
+    else launch_grid_kernel(fpw_kernel<P, 1>, g, a, lds, s);
 }
 
 template <bool P>
 void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
-    if (grp >= 4) launch2(bls_kernel<P, 4>, bls_persistent<P, 4>, g, a, lds, q, s);
-    else if (grp == 2) launch2(bls_kernel<P, 2>, bls_persistent<P, 2>, g, a, lds, q, s);
-    else launch2(bls_kernel<P, 1>, bls_persistent<P, 1>, g, a, lds, q, s);
+    if (q) launch_persistent(bls_persistent<P>, g, a, lds, *q, s);
+    else if (grp >= 4) launch_grid_kernel(bls_kernel<P, 4>, g, a, lds, s);
+    else if (grp == 2) launch_grid_kernel(bls_kernel<P, 2>, g, a, lds, s);
+    else launch_grid_kernel(bls_kernel<P, 1>, g, a, lds, s);
 }
 
 hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
@@ -712,11 +783,15 @@ hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
 
 hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hipStream_t s) {
     if (g.has_pdt) {
-        if (a.fast) launch2(ls_kernel<true, true>, ls_persistent<true, true>, g, a, 0, q, s);
-        else launch2(ls_kernel<true, false>, ls_persistent<true, false>, g, a, 0, q, s);
+        if (q) a.fast ? launch_persistent(ls_persistent<true, true>, g, a, 0, *q, s)
+                      : launch_persistent(ls_persistent<true, false>, g, a, 0, *q, s);
+        else a.fast ? launch_grid_kernel(ls_kernel<true, true>, g, a, 0, s)
+                    : launch_grid_kernel(ls_kernel<true, false>, g, a, 0, s);
     } else {
-        if (a.fast) launch2(ls_kernel<false, true>, ls_persistent<false, true>, g, a, 0, q, s);
-        else launch2(ls_kernel<false, false>, ls_persistent<false, false>, g, a, 0, q, s);
+        if (q) a.fast ? launch_persistent(ls_persistent<false, true>, g, a, 0, *q, s)
+                      : launch_persistent(ls_persistent<false, false>, g, a, 0, *q, s);
+        else a.fast ? launch_grid_kernel(ls_kernel<false, true>, g, a, 0, s)
+                    : launch_grid_kernel(ls_kernel<false, false>, g, a, 0, s);
     }
     return hipGetLastError();
 }

@@ -24,11 +24,19 @@ struct GridArgs {
 };
 
 // Persistent-launch work queue (see pf_grid.hip).
+// Persistent (ragged) launch: items are (rank, trial block) pairs in rank
+// order; ranks are grouped into up to kQueueBuckets length buckets, each with
+// its own kernel variant (binning group) and claim bound.
+constexpr int kQueueBuckets = 8;
+
 struct WorkQueue {
     unsigned long long* counter;  // next unclaimed item (zeroed before launch)
     const int64_t* order;         // rank -> curve index, longest curve first
-    int64_t total;                // n_curves * tblocks
-    int64_t claim;                // items claimed per atomic
+    int64_t total;                // n_ranks * tblocks
+    int n_buckets;                // 1..kQueueBuckets
+    int bucket_group[kQueueBuckets];    // binning variant (see launch_*: grp)
+    int bucket_claim[kQueueBuckets];    // most items per atomic claim (guided below)
+    int64_t bucket_end[kQueueBuckets];  // exclusive rank end of each bucket
 };
 
 struct CeArgs {
@@ -73,7 +81,7 @@ hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp,
 hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp,
                       const WorkQueue* q, hipStream_t s);
 // Zero a u64 counter with a kernel (graph-capture safe: no memset node).
-hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);  // n <= 64
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }

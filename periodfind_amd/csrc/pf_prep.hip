@@ -109,12 +109,15 @@ __global__ void bls_center_kernel(const int64_t* __restrict__ off, const float* 
     }
 }
 
-__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
 
 }  // namespace
 
-hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, p);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+    if (n <= 0 || n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 // kernel over every (curve, 64-trial block), and copies the result back.
 // Errors are returned as codes with a per-thread message instead of exit()
 // (cuda/errchk.cuh:14-28).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -104,6 +107,8 @@ struct CurveSet {
     int64_t out_T = 0;
     // Ragged (persistent) launches: curves longest first, and the work counter.
     std::vector<int64_t> off_h;  // host copy of the offsets
+    int64_t rep_len = 0;         // work-weighted mean length sum(len^2)/sum(len)
+    std::vector<int64_t> order_h;  // curves longest first (host copy of `order`)
     DevBuf order, counter;
     bool order_ok = false;
     // Captured launch sequence of the ragged peaks path, and what it was built for.
@@ -146,6 +151,10 @@ struct pf_ctx {
     size_t lds_max = 65536;
     int rmw_group = 2;  // LDS read-modify-write group size of the f32-binned kernels
     int ce_group = 0;   // CE: 0 = LDS u32 atomics, else read-modify-write group size
+    // Per-(statistic, length bucket) launch choices; empty = the defaults above.
+    std::vector<pf_tune_entry> tune;
+    std::string tune_source;
+    std::string arch;  // e.g. "gfx950"
     CurveSet scratch;  // curve set of the drop-in calls
     // Trial grid (cached by content).
     DevBuf periods, pdts;
@@ -169,6 +178,117 @@ struct Params {
     int n_phase, n_mag, phase_ext, mag_ext, n_bins;
     float qmin, qmax;
 };
+
+// --- Launch tuning -----------------------------------------------------------
+// Results do not depend on these choices (every variant accumulates in the
+// same order); only speed does.
+
+struct Choice {
+    int group;  // CE: 0 = LDS u32 atomics, 1/2/4 = read-modify-write group; others 1/2/4
+    int claim;  // work items per atomic claim of a persistent launch
+};
+
+Choice tune_choice(const pf_ctx* ctx, int algo, int64_t len) {
+    Choice c{algo == PF_ALGO_CE ? ctx->ce_group : ctx->rmw_group, 16};
+    int64_t best = -1;
+    for (const pf_tune_entry& e : ctx->tune)
+        if (e.algo == algo && e.min_len <= len && e.min_len > best) {
+            best = e.min_len;
+            c = Choice{e.group, e.claim};
+        }
+    return c;
+}
+
+// Ascending bucket lower bounds of `algo` (always starting at 0).
+std::vector<int64_t> tune_edges(const pf_ctx* ctx, int algo) {
+    std::vector<int64_t> ed{0};
+    for (const pf_tune_entry& e : ctx->tune)
+        if (e.algo == algo && e.min_len > 0) ed.push_back(e.min_len);
+    std::sort(ed.begin(), ed.end());
+    ed.erase(std::unique(ed.begin(), ed.end()), ed.end());
+    return ed;
+}
+
+constexpr int kMaxTuneBuckets = 32;
+const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls"};
+
+int validate_tune(const pf_tune_entry* t, int64_t n) {
+    if (n < 0 || (n > 0 && !t)) return fail(PF_E_ARG, "tuning table pointer/size invalid");
+    int per_algo[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        const pf_tune_entry& e = t[i];
+        if (e.algo < PF_ALGO_CE || e.algo > PF_ALGO_BLS)
+            return fail(PF_E_ARG, "tuning entry %lld: unknown algorithm %d",
+                        static_cast<long long>(i), e.algo);
+        if (e.min_len < 0)
+            return fail(PF_E_ARG, "tuning entry %lld: min_len < 0", static_cast<long long>(i));
+        if (e.claim < 1 || e.claim > 64)
+            return fail(PF_E_ARG, "tuning entry %lld: claim must be in [1, 64]",
+                        static_cast<long long>(i));
+        const bool g124 = e.group == 1 || e.group == 2 || e.group == 4;
+        if (!(g124 || (e.algo == PF_ALGO_CE && e.group == 0)))
+            return fail(PF_E_ARG, "tuning entry %lld: group %d not supported for %s",
+                        static_cast<long long>(i), e.group, kAlgoNames[e.algo]);
+        // Plus the implicit bucket from length 0: at most kMaxTuneBuckets in all.
+        if (++per_algo[e.algo] > kMaxTuneBuckets - 1)
+            return fail(PF_E_LIMIT, "more than %d tuning entries for %s", kMaxTuneBuckets - 1,
+                        kAlgoNames[e.algo]);
+    }
+    return PF_OK;
+}
+
+// Text table: one entry per line, "<algo> <min_len> <group> <claim>",
+// algo in {ce, aov, ls, fpw, bls}; '#' starts a comment.
+int parse_tune_file(const char* path, std::vector<pf_tune_entry>* out) {
+    std::ifstream f(path);
+    if (!f) return fail(PF_E_ARG, "cannot open tuning table %s", path);
+    std::string line;
+    int lineno = 0;
+    out->clear();
+    while (std::getline(f, line)) {
+        ++lineno;
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream ss(line);
+        std::string name;
+        if (!(ss >> name)) continue;
+        pf_tune_entry e{};
+        e.algo = -1;
+        for (int a = 0; a < 5; ++a)
+            if (name == kAlgoNames[a]) e.algo = a;
+        long long ml = 0;
+        if (e.algo < 0 || !(ss >> ml >> e.group >> e.claim))
+            return fail(PF_E_ARG, "%s:%d: expected '<algo> <min_len> <group> <claim>'", path,
+                        lineno);
+        e.min_len = ml;
+        out->push_back(e);
+    }
+    return validate_tune(out->data(), static_cast<int64_t>(out->size()));
+}
+
+// Base architecture name of a device ("gfx950" of "gfx950:sramecc+:xnack-").
+std::string device_arch(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "";
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    return arch;
+}
+
+// The table shipped next to this library for an architecture
+// (tuning_<arch>.txt), or "" when there is none.
+std::string default_tune_path(const std::string& arch) {
+    if (arch.empty()) return "";
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void*>(&default_tune_path), &info) || !info.dli_fname) return "";
+    std::string dir = info.dli_fname;
+    const size_t slash = dir.rfind('/');
+    dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+    std::string path = dir + "/tuning_" + arch + ".txt";
+    std::ifstream f(path);
+    return f ? path : "";
+}
 
 int check_params(const Params& p, size_t lds_max) {
     switch (p.algo) {
@@ -275,6 +395,15 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
     cs.npts = npts;
     cs.max_len = mx;
     cs.off_h.assign(off, off + n_curves + 1);
+    {
+        double s1 = 0.0, s2 = 0.0;
+        for (int64_t c = 0; c < n_curves; ++c) {
+            const double len = static_cast<double>(off[c + 1] - off[c]);
+            s1 += len;
+            s2 += len * len;
+        }
+        cs.rep_len = s1 > 0.0 ? static_cast<int64_t>(s2 / s1) : 0;
+    }
     cs.has_err = e != nullptr;
     const size_t nb = static_cast<size_t>(npts) * sizeof(float);
     PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
@@ -356,7 +485,7 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
 // already be prepared).  `timed` brackets the launch with the context's
 // events (not allowed while the stream is being captured).
 int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd,
-                const pf::WorkQueue* q, bool timed) {
+                const pf::WorkQueue* q, bool timed, int group) {
     const int64_t T = np * nd;
     const bool has_pdt = ctx->has_pdt;
     hipStream_t s = ctx->stream;
@@ -382,13 +511,13 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
     switch (p.algo) {
         case PF_ALGO_CE: {
             pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), ctx->ce_group, q, s);
+            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), group, q, s);
             break;
         }
         case PF_ALGO_AOV: {
             pf::AovArgs a{as<float>(cs.y), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
                           p.n_phase, p.phase_ext};
-            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), ctx->rmw_group, q, s);
+            le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), group, q, s);
             break;
         }
         case PF_ALGO_LS: {
@@ -398,14 +527,14 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
         }
         case PF_ALGO_FPW: {
             pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
-            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), ctx->rmw_group, q, s);
+            le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), group, q, s);
             break;
         }
         case PF_ALGO_BLS: {
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
             pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi};
-            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), ctx->rmw_group, q, s);
+            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), group, q, s);
             break;
         }
     }
@@ -426,7 +555,8 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
-    return launch_grid(ctx, cs, p, np, nd, nullptr, true);
+    return launch_grid(ctx, cs, p, np, nd, nullptr, true,
+                       tune_choice(ctx, p.algo, cs.rep_len).group);
 }
 
 int finish(pf_ctx* ctx) {
@@ -544,10 +674,11 @@ int ensure_order(pf_ctx* ctx, CurveSet& cs) {
         return (cs.off_h[a + 1] - cs.off_h[a]) > (cs.off_h[b + 1] - cs.off_h[b]);
     });
     PF_TRY(ensure(cs.order, cs.n_curves * sizeof(int64_t)));
-    PF_TRY(ensure(cs.counter, sizeof(unsigned long long)));
+    PF_TRY(ensure(cs.counter, kMaxTuneBuckets * sizeof(unsigned long long)));
     PF_HIP(hipMemcpyAsync(cs.order.p, ord.data(), cs.n_curves * sizeof(int64_t),
                           hipMemcpyHostToDevice, ctx->stream));
     PF_HIP(hipStreamSynchronize(ctx->stream));
+    cs.order_h.swap(ord);
     cs.order_ok = true;
     return PF_OK;
 }
@@ -582,11 +713,38 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
     PF_TRY(ensure_order(ctx, cs));
     const int64_t tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
-    pf::WorkQueue q;
-    q.counter = as<unsigned long long>(cs.counter);
-    q.order = as<int64_t>(cs.order);
-    q.total = cs.n_curves * tblocks;
-    q.claim = 4;
+    // Length buckets of the tuning table over the longest-first order: each
+    // is a contiguous rank range with its own binning variant and claim
+    // bound, all served by one persistent launch (one drain tail).
+    struct Bucket {
+        int64_t r0, r1;
+        Choice ch;
+    };
+    std::vector<Bucket> buckets;
+    {
+        const std::vector<int64_t> edges = tune_edges(ctx, p.algo);
+        int64_t r = 0;
+        for (size_t b = edges.size(); b-- > 0;) {
+            const int64_t r0 = r;
+            while (r < cs.n_curves &&
+                   cs.off_h[cs.order_h[r] + 1] - cs.off_h[cs.order_h[r]] >= edges[b])
+                ++r;
+            if (r > r0) buckets.push_back(Bucket{r0, r, tune_choice(ctx, p.algo, edges[b])});
+        }
+        // Adjacent buckets with the same choices are one bucket; past the
+        // queue's capacity, the shortest buckets join the last one kept.
+        std::vector<Bucket> merged;
+        for (const Bucket& b : buckets) {
+            if (!merged.empty() && merged.back().ch.group == b.ch.group &&
+                merged.back().ch.claim == b.ch.claim)
+                merged.back().r1 = b.r1;
+            else if (static_cast<int>(merged.size()) == pf::kQueueBuckets)
+                merged.back().r1 = b.r1;
+            else
+                merged.push_back(b);
+        }
+        buckets.swap(merged);
+    }
     const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
     // Everything the captured kernels read as arguments.
     std::vector<uint64_t> key = {
@@ -597,20 +755,40 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         static_cast<uint64_t>(__builtin_bit_cast(uint32_t, p.qmax)),
         static_cast<uint64_t>(np), static_cast<uint64_t>(nd), static_cast<uint64_t>(n_peaks),
         static_cast<uint64_t>(min_distance), static_cast<uint64_t>(ctx->has_pdt),
-        static_cast<uint64_t>(ctx->rmw_group), static_cast<uint64_t>(ctx->ce_group),
         reinterpret_cast<uint64_t>(ctx->periods.p), reinterpret_cast<uint64_t>(ctx->pdts.p),
         reinterpret_cast<uint64_t>(ctx->pidx.p), reinterpret_cast<uint64_t>(ctx->pval.p),
         reinterpret_cast<uint64_t>(cs.out.p), reinterpret_cast<uint64_t>(cs.t64.p),
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
-        reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p)};
+        reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
+        reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p)};
+    for (const Bucket& b : buckets) {
+        key.push_back(static_cast<uint64_t>(b.r0));
+        key.push_back(static_cast<uint64_t>(b.r1));
+        key.push_back(static_cast<uint64_t>(b.ch.group));
+        key.push_back(static_cast<uint64_t>(b.ch.claim));
+    }
     if (!cs.graph || cs.graph_key != key) {
         cs.drop_graph();
         hipStream_t s = ctx->stream;
+        unsigned long long* counters = as<unsigned long long>(cs.counter);
         PF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int rc = PF_OK;
-        if (pf::launch_zero_u64(q.counter, s) != hipSuccess) rc = fail(PF_E_HIP, "zero launch");
-        if (rc == PF_OK) rc = launch_grid(ctx, cs, p, np, nd, &q, false);
+        if (pf::launch_zero_u64(counters, 1, s) != hipSuccess)
+            rc = fail(PF_E_HIP, "counter reset launch failed during capture");
+        if (rc == PF_OK) {
+            pf::WorkQueue q{};
+            q.counter = counters;
+            q.order = as<int64_t>(cs.order);
+            q.total = cs.n_curves * tblocks;
+            q.n_buckets = static_cast<int>(buckets.size());
+            for (size_t b = 0; b < buckets.size(); ++b) {
+                q.bucket_group[b] = buckets[b].ch.group;
+                q.bucket_claim[b] = buckets[b].ch.claim;
+                q.bucket_end[b] = buckets[b].r1;
+            }
+            rc = launch_grid(ctx, cs, p, np, nd, &q, false, buckets[0].ch.group);
+        }
         if (rc == PF_OK &&
             pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
                              as<int64_t>(ctx->pidx), as<float>(ctx->pval), s) != hipSuccess)
@@ -805,6 +983,21 @@ int pf_ctx_create(pf_ctx** out, int device) {
     if (const char* g = std::getenv("PF_CE_GROUP")) {
         const int v = std::atoi(g);
         ctx->ce_group = (v >= 4) ? 4 : (v == 2 ? 2 : (v == 1 ? 1 : 0));
+    }
+    ctx->arch = device_arch(device);
+    {
+        const char* env = std::getenv("PF_TUNE_TABLE");
+        std::string path = env ? std::string(env) : default_tune_path(ctx->arch);
+        if (!path.empty() && path != "none") {
+            std::vector<pf_tune_entry> t;
+            const int rc = parse_tune_file(path.c_str(), &t);
+            if (rc != PF_OK) {
+                delete ctx;
+                return rc;
+            }
+            ctx->tune.swap(t);
+            ctx->tune_source = path;
+        }
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
@@ -1104,6 +1297,38 @@ int pf_curves_ls_peaks_refined(pf_ctx* ctx, pf_curves* cv, const double* freqs, 
     return ls_refine_on(ctx, cv->cs, freqs, n_freqs, n_peaks, min_distance, rtol, max_iter, idx,
                         val, freq_out, power_out, status);
 }
+
+int pf_ctx_set_tuning(pf_ctx* ctx, const pf_tune_entry* entries, int64_t n) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    PF_TRY(validate_tune(entries, n));
+    ctx->tune.assign(entries, entries + n);
+    ctx->tune_source = n ? "set" : "";
+    return PF_OK;
+}
+
+int pf_ctx_get_tuning(pf_ctx* ctx, pf_tune_entry* entries, int64_t cap, int64_t* n) {
+    if (!ctx || !n) return fail(PF_E_ARG, "ctx or n is NULL");
+    const int64_t have = static_cast<int64_t>(ctx->tune.size());
+    *n = have;
+    if (cap > 0 && !entries) return fail(PF_E_ARG, "entries is NULL");
+    for (int64_t i = 0; i < std::min(cap, have); ++i) entries[i] = ctx->tune[i];
+    return PF_OK;
+}
+
+int pf_ctx_load_tuning(pf_ctx* ctx, const char* path) {
+    if (!ctx || !path) return fail(PF_E_ARG, "ctx or path is NULL");
+    std::vector<pf_tune_entry> t;
+    PF_TRY(parse_tune_file(path, &t));
+    ctx->tune.swap(t);
+    ctx->tune_source = path;
+    return PF_OK;
+}
+
+const char* pf_ctx_tuning_source(pf_ctx* ctx) {
+    return ctx ? ctx->tune_source.c_str() : "";
+}
+
+const char* pf_ctx_arch(pf_ctx* ctx) { return ctx ? ctx->arch.c_str() : ""; }
 
 int pf_last_kernel_ms(pf_ctx* ctx, float* ms) {
     if (!ctx || !ms) return fail(PF_E_ARG, "NULL argument");

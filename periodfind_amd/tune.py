@@ -1,0 +1,171 @@
+"""Autotuner for the grid launches: per statistic and curve-length bucket,
+the binning strategy (CE: LDS u32 atomics or lane-private read-modify-write
+groups; AOV/FPW/BLS: read-modify-write group size) and the persistent
+launch's claim size.
+
+Every candidate is timed with the ragged persistent + hipGraph path
+(pf_curves_peaks_ragged) on synthetic curves of the bucket's lengths; all
+candidates must return identical peaks (the choices change speed, never
+results).  The fastest per bucket is written as a tuning table that
+pf_ctx_create reads: tuning_<arch>.txt next to libpfhip.so.
+
+    python -m periodfind_amd.tune [--out PATH] [--algos ce,aov,ls,fpw,bls]
+                                  [--trials 6000] [--reps 3] [--report PATH]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Length buckets [lo, hi) and the candidates of each statistic.
+BUCKETS = [(0, 256), (256, 1024), (1024, 4096), (4096, 16384), (16384, 65536)]
+GROUPS = {"ce": [0, 1, 2, 4], "aov": [1, 2, 4], "ls": [2], "fpw": [1, 2, 4], "bls": [1, 2, 4]}
+CLAIMS = [4, 16, 64]  # upper bounds of the guided claim size
+PARAMS = {"ce": dict(n_phase=10, n_mag=10), "aov": dict(n_phase=15), "ls": {},
+          "fpw": dict(n_bins=10), "bls": dict(n_bins=50, qmin=0.01, qmax=0.5)}
+
+
+def bucket_curves(lo, hi, n_trials, target_evals, seed):
+    """Curves with lengths log-uniform in [max(lo, 16), hi), mixed signals,
+    sized so that sum(len) * n_trials ~ target_evals."""
+    from periodfind_amd import synth
+
+    rng = np.random.default_rng(seed)
+    a = max(lo, 16)
+    mean_len = (hi - a) / max(np.log(hi / a), 1e-9)
+    n = int(np.clip(target_evals / (mean_len * n_trials), 64, 20000))
+    lens = np.exp(rng.uniform(np.log(a), np.log(hi), n)).astype(np.int64)
+    times, mags, errs = [], [], []
+    for i, L in enumerate(lens):
+        t = np.sort(rng.uniform(0.0, 1500.0, int(L))).astype(np.float32)
+        times.append(t)
+        mags.append(synth._signal(rng, t.astype(np.float64), i % 3))
+        errs.append(np.full(int(L), 0.05, np.float32))
+    return times, mags, errs
+
+
+def prepared(algo, mags):
+    from periodfind_amd._utils import prepare_magnitudes
+
+    if algo == "ce":
+        return prepare_magnitudes(mags, False, True)
+    if algo == "ls":
+        return prepare_magnitudes(mags, True, False)
+    return mags
+
+
+def tune(algos, n_trials=6000, reps=3, target_evals=1.5e10, device=0, log=print):
+    from periodfind_amd import capi
+
+    ctx = capi.Context(device)
+    periods = np.linspace(0.5, 20.0, n_trials).astype(np.float32)
+    pdts = np.zeros(1, np.float32)
+    report = {"n_trials": n_trials, "reps": reps, "buckets": {}}
+    table = []
+    for bi, (lo, hi) in enumerate(BUCKETS):
+        t, m, e = bucket_curves(lo, hi, n_trials, target_evals, seed=20261203 + 100 + bi)
+        key = f"{lo}-{hi}"
+        report["buckets"][key] = {"n_curves": len(t), "n_points": int(sum(len(x) for x in t))}
+        for algo in algos:
+            cs = capi.CurveSet(ctx, t, prepared(algo, m), e if algo in ("fpw", "bls") else None)
+            ref = None
+            timings = {}
+            for g in GROUPS[algo]:
+                for c in CLAIMS:
+                    ctx.set_tuning([(algo, 0, g, c)])
+                    out = cs.peaks(algo, periods, pdts, 32, 1, ragged=True, **PARAMS[algo])
+                    if ref is None:
+                        ref = out
+                    elif not (np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])):
+                        raise RuntimeError(f"{algo} bucket {key}: group {g} claim {c} changed "
+                                           "the results")
+                    w = []
+                    for _ in range(reps):
+                        t0 = time.perf_counter()
+                        cs.peaks(algo, periods, pdts, 32, 1, ragged=True, **PARAMS[algo])
+                        w.append(time.perf_counter() - t0)
+                    timings[(g, c)] = float(np.median(w))
+            cs.close()
+            (g, c), best = min(timings.items(), key=lambda kv: kv[1])
+            worst = max(timings.values())
+            table.append((algo, lo, g, c))
+            report["buckets"][key][algo] = {
+                "best": {"group": g, "claim": c, "s": best},
+                "worst_s": worst,
+                "all": {f"g{gg}_c{cc}": v for (gg, cc), v in timings.items()}}
+            log(f"tune: {algo:4s} len {key:>11s}: group {g} claim {c} "
+                f"{best * 1e3:8.2f} ms (worst {worst * 1e3:8.2f} ms)")
+    ctx.set_tuning([])
+    ctx.close()
+    return merge(table), report
+
+
+def merge(table):
+    """Drop entries that repeat the previous bucket's choice of the same algo."""
+    out = []
+    last = {}
+    for algo, lo, g, c in table:
+        if last.get(algo) == (g, c):
+            continue
+        last[algo] = (g, c)
+        out.append((algo, lo, g, c))
+    return out
+
+
+def device_arch(device=0):
+    """Architecture name of `device` as the runtime reports it (e.g. gfx950)."""
+    from periodfind_amd import capi
+
+    ctx = capi.Context(device)
+    try:
+        return ctx.arch()
+    finally:
+        ctx.close()
+
+
+def write_table(path, entries, header):
+    with open(path, "w") as f:
+        f.write("# periodfind_amd launch tuning table: <algo> <min_len> <group> <claim>\n")
+        for line in header:
+            f.write(f"# {line}\n")
+        for algo, lo, g, c in entries:
+            f.write(f"{algo} {lo} {g} {c}\n")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=None, help="table path (default: tuning_<arch>.txt in the "
+                                                "package, read by pf_ctx_create)")
+    ap.add_argument("--report", default=None, help="also write all timings as JSON")
+    ap.add_argument("--algos", default="ce,aov,ls,fpw,bls")
+    ap.add_argument("--trials", type=int, default=6000)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--target-evals", type=float, default=1.5e10)
+    args = ap.parse_args(argv)
+    algos = [a for a in args.algos.split(",") if a]
+    for a in algos:
+        if a not in GROUPS:
+            ap.error(f"unknown algo {a}")
+    # Tune from the built-in defaults, not from a table already in place.
+    os.environ["PF_TUNE_TABLE"] = "none"
+    out = args.out or os.path.join(HERE, f"tuning_{device_arch()}.txt")
+    entries, report = tune(algos, args.trials, args.reps, args.target_evals,
+                           log=lambda s: print(s, flush=True))
+    write_table(out, entries, [f"measured by periodfind_amd.tune: {args.trials} trials, "
+                               f"{args.reps} reps, buckets {BUCKETS}"])
+    print(f"tune: wrote {out}", flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

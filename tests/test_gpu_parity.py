@@ -297,3 +297,53 @@ def test_heavy_shape_subsample(ctx, algo):
         oi, ov = O.find_top_peaks_batched(ref, 32, 1, use_max=True)
         np.testing.assert_array_equal(gi, oi)
         np.testing.assert_array_equal(gv, ov)
+
+
+def test_tuning_table_buckets_keep_results(ctx, tmp_path):
+    """Per-length-bucket launch choices (several persistent launches in one
+    graph) give the same peaks as the plain launch; the table API round-trips
+    and rejects invalid entries."""
+    from periodfind_amd import capi
+
+    saved = ctx.get_tuning()
+    rng = np.random.default_rng(31)
+    lens = np.round(np.exp(rng.uniform(np.log(20), np.log(4000), 60))).astype(int)
+    t, m, e = _curves(list(lens), seed=32)
+    P = np.linspace(0.3, 12.0, 900).astype(np.float32)
+    D = np.array([0.0, 2e-5], np.float32)
+    table = [("ce", 0, 0, 1), ("ce", 100, 2, 16), ("ce", 700, 4, 4),
+             ("bls", 0, 1, 1), ("bls", 200, 4, 16), ("aov", 0, 2, 4), ("aov", 300, 1, 2),
+             ("fpw", 0, 4, 1), ("fpw", 1000, 1, 8), ("ls", 0, 2, 2), ("ls", 50, 2, 16)]
+    try:
+        for algo in ("ce", "aov", "ls", "fpw", "bls"):
+            mm = _prep(algo, m)
+            cs = capi.CurveSet(ctx, t, mm, errs=e if algo in ("fpw", "bls") else None)
+            prm = PARAMS[algo] if algo != "aov" else dict(n_phase=15)
+            ctx.set_tuning([])
+            ri, rv = cs.peaks(algo, P, D, n_peaks=16, min_distance=1, ragged=False, **prm)
+            ctx.set_tuning(table)
+            for _ in range(2):
+                gi, gv = cs.peaks(algo, P, D, n_peaks=16, min_distance=1, ragged=True, **prm)
+                np.testing.assert_array_equal(gi, ri)
+                np.testing.assert_array_equal(gv, rv)
+            pi, pv = cs.peaks(algo, P, D, n_peaks=16, min_distance=1, ragged=False, **prm)
+            np.testing.assert_array_equal(pi, ri)
+            np.testing.assert_array_equal(pv, rv)
+            cs.close()
+        assert sorted(ctx.get_tuning()) == sorted(table)
+        assert ctx.tuning_source() == "set"
+        f = tmp_path / "table.txt"
+        f.write_text("# comment\nce 0 0 4\nbls 0 2 4  # trailing\nbls 5000 4 16\n")
+        ctx.load_tuning(str(f))
+        assert ctx.get_tuning() == [("ce", 0, 0, 4), ("bls", 0, 2, 4), ("bls", 5000, 4, 16)]
+        assert ctx.tuning_source() == str(f)
+        for bad in ([("aov", 0, 0, 4)], [("ce", 0, 3, 4)], [("ce", 0, 1, 0)],
+                    [("ce", -1, 1, 1)]):
+            with pytest.raises(capi.PfError):
+                ctx.set_tuning(bad)
+        f.write_text("ce 0 zero 4\n")
+        with pytest.raises(capi.PfError):
+            ctx.load_tuning(str(f))
+        assert ctx.arch().startswith("gfx")
+    finally:
+        ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
