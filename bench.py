@@ -49,6 +49,18 @@ PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
 P100_8192 = {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (e.g. "bls@8192") from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_batch.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        return int(round(k["traffic_bytes"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def prep_mags(algo, mags):
     from periodfind_amd._utils import prepare_magnitudes
 
@@ -445,6 +457,19 @@ def run_heavy(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` (torch), the reference kernel for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +482,8 @@ def main():
                     help="batch: the headline 5-statistic workload; ragged: 10k ragged curves; "
                          "lsrefine: differentiable LS peak refinement; heavy: 1000 x 65536 "
                          "points, 1e5 periods")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (profiler counter calibration)")
     ap.add_argument("--oracle-curves", type=int, default=100,
                     help="lsrefine: curves whose refinement is checked against the oracle")
     args = ap.parse_args()
@@ -465,6 +492,8 @@ def main():
               "(curve sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.workload == "ragged":
         return run_ragged(args)
     if args.workload == "lsrefine":
@@ -532,7 +561,7 @@ def main():
                    "n_curves": N_CURVES, "sizes": list(SIZES), "n_periods": 1000},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(ach, 3),
                      "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": None},
+                     "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom)},
         "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
         "kernels": detail,
     }
