@@ -136,7 +136,16 @@ def cpu_baseline(datasets, budget_s):
                         threads=threads, **kw)
             pts += k * n
     dt = time.perf_counter() - t0
+    # Per statistic at 8192 points (the same k curves, timed separately).
+    per_algo = {}
+    t8, m8, e8, P8, D8 = datasets[8192]
+    for algo in ALGOS:
+        ta = time.perf_counter()
+        O.calc_grid(algo, t8[:k], m8[:k], P8, D8, errs=e8[:k] if algo in ("fpw", "bls") else None,
+                    threads=threads, **dict(PARAMS[algo]))
+        per_algo[algo] = k * 8192 / (time.perf_counter() - ta)
     return {
+        "pts_per_s_8192": {a: round(v) for a, v in per_algo.items()},
         "value": pts / dt,
         "unit": "pts/sec @1000 periods",
         "cores": threads,
@@ -519,10 +528,13 @@ def main():
 
     print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
     kms = {k: [] for k in sets}
+    walls = {k: [] for k in sets}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for k, r in sets.items():
+            tc = time.perf_counter()
             kms[k].append(r.step())
+            walls[k].append(time.perf_counter() - tc)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     pts_per_step = sum(N_CURVES * n for (_, n) in sets)
@@ -536,6 +548,7 @@ def main():
         fpe = FLOPS_INNER[algo] + FLOPS_TRIAL[algo] / n
         achieved = evals * fpe / kt / 1e12
         detail[f"{algo}@{n}"] = {"kernel_ms": round(kt * 1e3, 4),
+                                 "call_ms": round(float(np.median(walls[(algo, n)])) * 1e3, 4),
                                  "evals_per_s": evals / kt,
                                  "tflops": round(achieved, 3),
                                  "frac": round(achieved / PEAK_VALU_TFLOPS, 4)}
@@ -579,6 +592,12 @@ def main():
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(datasets, args.cpu_seconds)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+        # GPU rate per statistic at 8192 points: curves x points / wall time of
+        # one resident grid + stats call.
+        gpu8 = {a: N_CURVES * 8192 / (result["kernels"][f"{a}@8192"]["call_ms"] * 1e-3)
+                for a in ALGOS}
+        result["gpu_over_cpu_8192"] = {
+            a: round(gpu8[a] / result["cpu_baseline"]["pts_per_s_8192"][a], 1) for a in ALGOS}
     print(json.dumps(result), flush=True)
 
 
