@@ -69,30 +69,62 @@ __global__ void mul_kernel(const float* __restrict__ a, const float* __restrict_
 }
 
 // rust/src/aov.rs:27: sum::<f32>() / length, serial.
-__global__ void curve_mean_kernel(const int64_t* __restrict__ off, const float* __restrict__ y,
-                                  float* __restrict__ avg, int64_t n_curves) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
-    if (c >= n_curves) return;
+// One wave per curve.  The wave loads 64 consecutive values (coalesced,
+// next chunk prefetched) and every lane adds them in lane order through
+// readlane, so the f32 sum is the serial one, value by value.
+__device__ __forceinline__ float lane_value(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// s += v[0] + ... v[cnt-1] in order (cnt <= 64; all lanes hold the same s).
+__device__ __forceinline__ float serial_add(float s, float v, int cnt) {
+    if (cnt == 64) {
+#pragma unroll
+        for (int j = 0; j < 64; ++j) s += lane_value(v, j);
+    } else {
+        for (int j = 0; j < cnt; ++j) s += __shfl(v, j);
+    }
+    return s;
+}
+
+// rust/src/aov.rs:27: avg = (serial f32 sum of y) / N.
+__global__ __launch_bounds__(64) void curve_mean_kernel(const int64_t* __restrict__ off,
+                                                        const float* __restrict__ y,
+                                                        float* __restrict__ avg) {
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
     const int64_t a = off[c], b = off[c + 1];
     float s = 0.0f;
-    for (int64_t i = a; i < b; ++i) s += y[i];
-    avg[c] = s / static_cast<float>(b - a);
+    float v = a + lane < b ? y[a + lane] : 0.0f;
+    for (int64_t i = a; i < b; i += 64) {
+        const float cur = v;
+        v = i + 64 + lane < b ? y[i + 64 + lane] : 0.0f;
+        s = serial_add(s, cur, static_cast<int>(b - i < 64 ? b - i : 64));
+    }
+    if (lane == 0) avg[c] = s / static_cast<float>(b - a);
 }
 
 // rust/src/bls.rs:35-47: total_w and sum(w*y) serial; mean; yw = w*(y - mean).
-__global__ void bls_mean_kernel(const int64_t* __restrict__ off, const float* __restrict__ y,
-                                const float* __restrict__ w, float* __restrict__ mean,
-                                int64_t n_curves) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
-    if (c >= n_curves) return;
+__global__ __launch_bounds__(64) void bls_mean_kernel(const int64_t* __restrict__ off,
+                                                      const float* __restrict__ y,
+                                                      const float* __restrict__ w,
+                                                      float* __restrict__ mean) {
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
     const int64_t a = off[c], b = off[c + 1];
     float tw = 0.0f, sy = 0.0f;
-    for (int64_t i = a; i < b; ++i) {
-        tw += w[i];
-        const float p = w[i] * y[i];
-        sy += p;
+    float vw = a + lane < b ? w[a + lane] : 0.0f;
+    float vy = a + lane < b ? y[a + lane] : 0.0f;
+    for (int64_t i = a; i < b; i += 64) {
+        const float cw = vw, cp = vw * vy;
+        const bool more = i + 64 + lane < b;
+        vw = more ? w[i + 64 + lane] : 0.0f;
+        vy = more ? y[i + 64 + lane] : 0.0f;
+        const int cnt = static_cast<int>(b - i < 64 ? b - i : 64);
+        tw = serial_add(tw, cw, cnt);
+        sy = serial_add(sy, cp, cnt);
     }
-    mean[c] = (tw > 0.0f) ? sy / tw : 0.0f;
+    if (lane == 0) mean[c] = (tw > 0.0f) ? sy / tw : 0.0f;
 }
 
 // `bx` blocks per curve: yw[i] = w[i] * (y[i] - mean[c]).
@@ -157,8 +189,8 @@ hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, 
 hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int64_t n_curves,
                              hipStream_t s) {
     if (n_curves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(curve_mean_kernel, dim3(static_cast<uint32_t>((n_curves + 63) / 64)),
-                       dim3(64), 0, s, off, y, avg, n_curves);
+    hipLaunchKernelGGL(curve_mean_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, y, avg);
     return hipGetLastError();
 }
 
@@ -166,8 +198,8 @@ hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int
 hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,
                               float* yw, int64_t n_curves, int64_t max_len, hipStream_t s) {
     if (n_curves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(bls_mean_kernel, dim3(static_cast<uint32_t>((n_curves + 63) / 64)),
-                       dim3(64), 0, s, off, y, w, mean, n_curves);
+    hipLaunchKernelGGL(bls_mean_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, y, w, mean);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int64_t bx = (max_len + kBlock - 1) / kBlock;

@@ -160,10 +160,13 @@ struct pf_ctx {
     DevBuf periods, pdts;
     std::vector<float> periods_h, pdts_h;
     bool has_pdt = false;
-    // Peak / stats outputs.
-    DevBuf pidx, pval, summ, flags;
-    // LS refinement: frequency grid (f64) and per-peak bracket / result arrays.
-    DevBuf ls_freq, ls_lo, ls_hi, ls_w, ls_p, ls_status;
+    // Outputs of one call, back to back (see OutPlan), and their pinned host
+    // staging buffer.
+    DevBuf outbuf;
+    void* host_stage = nullptr;
+    size_t host_cap = 0;
+    // LS refinement: frequency grid (f64) and per-peak brackets.
+    DevBuf ls_freq, ls_lo, ls_hi;
 };
 
 struct pf_curves {
@@ -569,6 +572,60 @@ int finish(pf_ctx* ctx) {
     return PF_OK;
 }
 
+// The outputs of one call live back to back (16-byte aligned) in
+// ctx->outbuf; one device-to-host copy brings them into pinned staging, and
+// they are then copied to the caller's arrays.  One transfer per call instead
+// of one per output array.
+struct OutRegion {
+    void* host;
+    size_t bytes;
+    size_t off;
+};
+
+struct OutPlan {
+    std::vector<OutRegion> r;
+    size_t total = 0;
+    size_t add(void* host, size_t bytes) {
+        const size_t off = (total + 15) & ~size_t(15);
+        r.push_back(OutRegion{host, bytes, off});
+        total = off + bytes;
+        return off;
+    }
+};
+
+template <typename T>
+T* out_dev(pf_ctx* ctx, size_t off) {
+    return reinterpret_cast<T*>(static_cast<char*>(ctx->outbuf.p) + off);
+}
+
+int ensure_host(pf_ctx* ctx, size_t bytes) {
+    if (ctx->host_cap >= bytes) return PF_OK;
+    if (ctx->host_stage) {
+        (void)hipHostFree(ctx->host_stage);
+        ctx->host_stage = nullptr;
+        ctx->host_cap = 0;
+    }
+    const size_t want = bytes + bytes / 4 + 4096;
+    if (hipHostMalloc(&ctx->host_stage, want, hipHostMallocDefault) != hipSuccess) {
+        ctx->host_stage = nullptr;
+        return fail(PF_E_NOMEM, "hipHostMalloc(%zu) failed", want);
+    }
+    ctx->host_cap = want;
+    return PF_OK;
+}
+
+// Queue the copy of every region, wait for the stream, scatter to the caller.
+int out_finish(pf_ctx* ctx, const OutPlan& plan) {
+    PF_TRY(ensure_host(ctx, plan.total));
+    if (plan.total)
+        PF_HIP(hipMemcpyAsync(ctx->host_stage, ctx->outbuf.p, plan.total, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    PF_TRY(finish(ctx));
+    for (const OutRegion& o : plan.r)
+        if (o.bytes) std::memcpy(o.host, static_cast<char*>(ctx->host_stage) + o.off, o.bytes);
+    return PF_OK;
+}
+
 Params to_params(const pf_params* p) {
     Params q;
     q.algo = p->algo;
@@ -616,16 +673,15 @@ int peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, i
     }
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(run_grid(ctx, cs, p, np, nd));
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    OutPlan plan;
+    const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
+    const size_t o_val = plan.add(val, pk * sizeof(float));
+    PF_TRY(ensure(ctx->outbuf, plan.total));
     const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
     PF_HIP(pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
-                            as<int64_t>(ctx->pidx), as<float>(ctx->pval), ctx->stream));
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    return finish(ctx);
+                            out_dev<int64_t>(ctx, o_idx), out_dev<float>(ctx, o_val),
+                            ctx->stream));
+    return out_finish(ctx, plan);
 }
 
 // 'stats' over a set already on the device.
@@ -646,23 +702,17 @@ int stats_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, i
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(run_grid(ctx, cs, p, np, nd));
     const size_t C = static_cast<size_t>(cs.n_curves);
-    PF_TRY(ensure(ctx->summ, C * 4 * sizeof(float)));
-    PF_TRY(ensure(ctx->pidx, C * n_stats * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, C * n_stats * sizeof(float)));
-    PF_TRY(ensure(ctx->flags, C * sizeof(int)));
+    OutPlan plan;
+    const size_t o_idx = plan.add(idx, C * n_stats * sizeof(int64_t));
+    const size_t o_val = plan.add(val, C * n_stats * sizeof(float));
+    const size_t o_sum = plan.add(summary, C * 4 * sizeof(float));
+    const size_t o_flg = plan.add(fallback, C * sizeof(int));
+    PF_TRY(ensure(ctx->outbuf, plan.total));
     const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
     PF_HIP(pf::launch_stats(as<float>(cs.out), cs.n_curves, T, n_stats, use_max,
-                            as<float>(ctx->summ), as<int64_t>(ctx->pidx), as<float>(ctx->pval),
-                            as<int>(ctx->flags), ctx->stream));
-    PF_HIP(hipMemcpyAsync(summary, ctx->summ.p, C * 4 * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, C * n_stats * sizeof(int64_t),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, C * n_stats * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(fallback, ctx->flags.p, C * sizeof(int), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    return finish(ctx);
+                            out_dev<float>(ctx, o_sum), out_dev<int64_t>(ctx, o_idx),
+                            out_dev<float>(ctx, o_val), out_dev<int>(ctx, o_flg), ctx->stream));
+    return out_finish(ctx, plan);
 }
 
 // Rank -> curve index, longest curve first (stable), uploaded once per set.
@@ -709,8 +759,10 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    OutPlan plan;
+    const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
+    const size_t o_val = plan.add(val, pk * sizeof(float));
+    PF_TRY(ensure(ctx->outbuf, plan.total));
     PF_TRY(ensure_order(ctx, cs));
     const int64_t tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
     // Length buckets of the tuning table over the longest-first order: each
@@ -756,7 +808,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         static_cast<uint64_t>(np), static_cast<uint64_t>(nd), static_cast<uint64_t>(n_peaks),
         static_cast<uint64_t>(min_distance), static_cast<uint64_t>(ctx->has_pdt),
         reinterpret_cast<uint64_t>(ctx->periods.p), reinterpret_cast<uint64_t>(ctx->pdts.p),
-        reinterpret_cast<uint64_t>(ctx->pidx.p), reinterpret_cast<uint64_t>(ctx->pval.p),
+        reinterpret_cast<uint64_t>(ctx->outbuf.p), static_cast<uint64_t>(o_val),
         reinterpret_cast<uint64_t>(cs.out.p), reinterpret_cast<uint64_t>(cs.t64.p),
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
@@ -791,7 +843,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         }
         if (rc == PF_OK &&
             pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
-                             as<int64_t>(ctx->pidx), as<float>(ctx->pval), s) != hipSuccess)
+                             out_dev<int64_t>(ctx, o_idx), out_dev<float>(ctx, o_val),
+                             s) != hipSuccess)
             rc = fail(PF_E_HIP, "peaks launch failed during capture");
         hipGraph_t graph = nullptr;
         hipError_t ec = hipStreamEndCapture(s, &graph);
@@ -812,11 +865,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     PF_HIP(hipGraphLaunch(cs.graph, ctx->stream));
     PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    return finish(ctx);
+    return out_finish(ctx, plan);
 }
 
 // Host in -> host out (the drop-in calls): upload into the context's set.
@@ -918,31 +967,30 @@ int ls_refine_on(pf_ctx* ctx, CurveSet& cs, const double* freqs, int64_t nf, int
     PF_TRY(upload_grid(ctx, periods.data(), nf, &zero, 1));
     PF_TRY(run_grid(ctx, cs, p, nf, 1));
     hipStream_t s = ctx->stream;
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
-    PF_HIP(pf::launch_peaks(as<float>(cs.out), C, nf, n_peaks, min_distance, 1,
-                            as<int64_t>(ctx->pidx), as<float>(ctx->pval), s));
+    OutPlan plan;
+    const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
+    const size_t o_val = plan.add(val, pk * sizeof(float));
+    const size_t o_w = plan.add(f_out, pk * sizeof(double));
+    const size_t o_p = plan.add(p_out, pk * sizeof(double));
+    const size_t o_st = plan.add(status, pk * sizeof(int));
+    PF_TRY(ensure(ctx->outbuf, plan.total));
+    int64_t* d_idx = out_dev<int64_t>(ctx, o_idx);
+    double* d_w = out_dev<double>(ctx, o_w);
+    double* d_p = out_dev<double>(ctx, o_p);
+    int* d_st = out_dev<int>(ctx, o_st);
+    PF_HIP(pf::launch_peaks(as<float>(cs.out), C, nf, n_peaks, min_distance, 1, d_idx,
+                            out_dev<float>(ctx, o_val), s));
     PF_TRY(ensure(ctx->ls_freq, nf * sizeof(double)));
     PF_TRY(ensure(ctx->ls_lo, pk * sizeof(double)));
     PF_TRY(ensure(ctx->ls_hi, pk * sizeof(double)));
-    PF_TRY(ensure(ctx->ls_w, pk * sizeof(double)));
-    PF_TRY(ensure(ctx->ls_p, pk * sizeof(double)));
-    PF_TRY(ensure(ctx->ls_status, pk * sizeof(int)));
     PF_HIP(hipMemcpyAsync(ctx->ls_freq.p, freqs, nf * sizeof(double), hipMemcpyHostToDevice, s));
-    PF_HIP(pf::launch_ls_bracket(as<int64_t>(ctx->pidx), as<double>(ctx->ls_freq), nf,
-                                 static_cast<int64_t>(pk), as<double>(ctx->ls_lo),
-                                 as<double>(ctx->ls_hi), as<double>(ctx->ls_w),
-                                 as<double>(ctx->ls_p), as<int>(ctx->ls_status), s));
+    PF_HIP(pf::launch_ls_bracket(d_idx, as<double>(ctx->ls_freq), nf, static_cast<int64_t>(pk),
+                                 as<double>(ctx->ls_lo), as<double>(ctx->ls_hi), d_w, d_p, d_st,
+                                 s));
     PF_HIP(pf::launch_ls_refine(as<int64_t>(cs.off), as<double>(cs.t64), cs.y.p, true, C, n_peaks,
-                                as<double>(ctx->ls_lo), as<double>(ctx->ls_hi),
-                                as<double>(ctx->ls_w), as<double>(ctx->ls_p),
-                                as<int>(ctx->ls_status), rtol, max_iter, s));
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipMemcpyAsync(f_out, ctx->ls_w.p, pk * sizeof(double), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipMemcpyAsync(p_out, ctx->ls_p.p, pk * sizeof(double), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipMemcpyAsync(status, ctx->ls_status.p, pk * sizeof(int), hipMemcpyDeviceToHost, s));
-    PF_TRY(finish(ctx));
+                                as<double>(ctx->ls_lo), as<double>(ctx->ls_hi), d_w, d_p, d_st,
+                                rtol, max_iter, s));
+    PF_TRY(out_finish(ctx, plan));
     constexpr double kTwoPi = 6.283185307179586476925286766559;
     for (size_t i = 0; i < pk; ++i) f_out[i] /= kTwoPi;
     return PF_OK;
@@ -1013,11 +1061,11 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
-    DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->pidx, &ctx->pval, &ctx->summ,
-                      &ctx->flags, &ctx->ls_freq, &ctx->ls_lo, &ctx->ls_hi, &ctx->ls_w,
-                      &ctx->ls_p, &ctx->ls_status};
+    DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->outbuf, &ctx->ls_freq, &ctx->ls_lo,
+                      &ctx->ls_hi};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1104,18 +1152,16 @@ int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t 
     cs.invalidate();
     const size_t db = static_cast<size_t>(n_curves) * T * sizeof(float);
     PF_TRY(ensure(cs.out, db));
-    PF_TRY(ensure(ctx->pidx, pk * sizeof(int64_t)));
-    PF_TRY(ensure(ctx->pval, pk * sizeof(float)));
+    OutPlan plan;
+    const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
+    const size_t o_val = plan.add(val, pk * sizeof(float));
+    PF_TRY(ensure(ctx->outbuf, plan.total));
     PF_HIP(hipMemcpyAsync(cs.out.p, data, db, hipMemcpyHostToDevice, ctx->stream));
     PF_HIP(pf::launch_peaks(as<float>(cs.out), n_curves, T, n_peaks, min_distance,
-                            use_max ? 1 : 0, as<int64_t>(ctx->pidx), as<float>(ctx->pval),
-                            ctx->stream));
-    PF_HIP(hipMemcpyAsync(idx, ctx->pidx.p, pk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    PF_HIP(hipMemcpyAsync(val, ctx->pval.p, pk * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->stream));
+                            use_max ? 1 : 0, out_dev<int64_t>(ctx, o_idx),
+                            out_dev<float>(ctx, o_val), ctx->stream));
     ctx->timed = false;
-    return finish(ctx);
+    return out_finish(ctx, plan);
 }
 
 int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves, const int64_t* offsets,

@@ -10,7 +10,8 @@
 //     division to float64 before the float32 cast).
 //   * np.std: x = a - mean, x*x, the same summation, f32(f64 / n), sqrt.
 //   * np.median: middle element, or fl(fl(a + b) / 2) of the two middle ones;
-//     MAD = median(|a - median|).
+//     MAD = median(|a - median|).  Order statistics by radix selection on
+//     order-preserving uint32 keys (no sort).
 //   * top-n: the n best values; among equal values the lower flat index wins
 //     (numpy's choice among exact ties is unspecified).
 // One workgroup per curve, T <= kStatsMaxT values staged in LDS; a curve
@@ -47,21 +48,23 @@ struct LeafPlan {
     int count;
 };
 
-// Left-to-right leaves of the split tree of [0, n) (thread 0).
+// Left-to-right leaves of the split tree of [0, n) (thread 0).  The explicit
+// stack lives in LDS: a dynamically indexed private array would be placed in
+// scratch memory, one global-memory round trip per push and pop.
 __device__ void plan_leaves(int n, LeafPlan& L) {
-    int st_s[16], st_n[16];
+    __shared__ int st_s[16], st_n[16];
     int sp = 0;
     st_s[0] = 0;
     st_n[0] = n;
     sp = 1;
-    L.count = 0;
+    int count = 0;
     while (sp > 0) {
         --sp;
         const int s0 = st_s[sp], m = st_n[sp];
         if (m <= 128) {
-            L.start[L.count] = s0;
-            L.len[L.count] = m;
-            ++L.count;
+            L.start[count] = s0;
+            L.len[count] = m;
+            ++count;
             continue;
         }
         int n2 = m / 2;
@@ -74,6 +77,7 @@ __device__ void plan_leaves(int n, LeafPlan& L) {
         st_n[sp] = n2;
         ++sp;
     }
+    L.count = count;
 }
 
 // numpy's leaf: < 8 elements serial; else 8 accumulators, combined
@@ -97,38 +101,39 @@ __device__ float leaf_sum(const float* x, int n) {
     return res;
 }
 
-// Post-order combination of the leaf sums (thread 0).
+// Post-order combination of the leaf sums (thread 0; stack in LDS, as above).
 __device__ float combine_leaves(int n, const LeafPlan& L) {
-    struct Frame {
-        int n, state;
-        float left;
-    };
-    Frame st[16];
+    __shared__ int st_n[16], st_state[16];
+    __shared__ float st_left[16];
     int sp = 0;
-    st[0] = {n, 0, 0.0f};
+    st_n[0] = n;
+    st_state[0] = 0;
     int next = 0;
     float ret = 0.0f;
     while (true) {
-        Frame& f = st[sp];
-        if (f.n <= 128) {
+        const int fn = st_n[sp];
+        if (fn <= 128) {
             ret = L.sum[next++];
             if (sp == 0) return ret;
             --sp;
             continue;
         }
-        int n2 = f.n / 2;
+        int n2 = fn / 2;
         n2 -= n2 % 8;
-        if (f.state == 0) {
-            f.state = 1;
-            st[sp + 1] = {n2, 0, 0.0f};
+        const int state = st_state[sp];
+        if (state == 0) {
+            st_state[sp] = 1;
+            st_n[sp + 1] = n2;
+            st_state[sp + 1] = 0;
             ++sp;
-        } else if (f.state == 1) {
-            f.left = ret;
-            f.state = 2;
-            st[sp + 1] = {f.n - n2, 0, 0.0f};
+        } else if (state == 1) {
+            st_left[sp] = ret;
+            st_state[sp] = 2;
+            st_n[sp + 1] = fn - n2;
+            st_state[sp + 1] = 0;
             ++sp;
         } else {
-            ret = f.left + ret;
+            ret = st_left[sp] + ret;
             if (sp == 0) return ret;
             --sp;
         }
@@ -136,9 +141,8 @@ __device__ float combine_leaves(int n, const LeafPlan& L) {
 }
 
 // np.add.reduce of x[0..n) (n <= 8192) by the whole block; result in *out.
+// L must hold the plan of n (plan_leaves, then a barrier).
 __device__ void block_numpy_sum(const float* x, int n, LeafPlan& L, float* out) {
-    if (threadIdx.x == 0) plan_leaves(n, L);
-    __syncthreads();
     for (int i = threadIdx.x; i < L.count; i += kThreads) L.sum[i] = leaf_sum(x + L.start[i], L.len[i]);
     __syncthreads();
     if (threadIdx.x == 0) *out = 0.0f + combine_leaves(n, L);
@@ -164,33 +168,93 @@ __device__ int block_scan(bool flag, int* wave_tot, int* total) {
     return base + in_wave;
 }
 
-__device__ void bitonic_sort(uint32_t* k, int n2) {
-    for (int size = 2; size <= n2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += kThreads) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    const uint32_t a = k[i], b = k[j];
-                    if ((a > b) == up) {
-                        k[i] = b;
-                        k[j] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+static_assert(kThreads == 256, "radix_select maps one histogram bin to each thread");
+
+// Block reductions (red: kThreads / 64 slots of LDS).
+__device__ int block_sum_int(int v, int* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int tot = 0;
+    for (int w = 0; w < kThreads / 64; ++w) tot += red[w];
+    __syncthreads();
+    return tot;
 }
 
-// Median of n sorted keys, numpy style.
-__device__ float sorted_median(const uint32_t* k, int n) {
-    if (n & 1) return key_value(k[n / 2]);
-    const float a = key_value(k[n / 2 - 1]);
-    const float b = key_value(k[n / 2]);
+__device__ uint32_t block_min_u32(uint32_t v, uint32_t* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = __shfl_xor(v, o);
+        v = u < v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t m = red[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = red[w] < m ? red[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// The k-th smallest (0-based) of keys[0..n): four 8-bit radix passes, each
+// histogramming the keys that match the digits fixed so far (per-wave LDS
+// histograms, hist: 4 x 256 ints) and fixing the digit whose bin holds rank
+// k (one bin per thread, block inclusive scan).  sel: 2 + kThreads/64 ints.
+__device__ uint32_t radix_select(const uint32_t* keys, int n, int k, int* hist, int* sel) {
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 4 * 256; i += kThreads) hist[i] = 0;
+        __syncthreads();
+        int* h = hist + wv * 256;
+        for (int i = tid; i < n; i += kThreads) {
+            const uint32_t key = keys[i];
+            if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1);
+        }
+        __syncthreads();
+        const int cnt = hist[tid] + hist[256 + tid] + hist[512 + tid] + hist[768 + tid];
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) sel[2 + wv] = incl;
+        __syncthreads();
+        for (int w = 0; w < wv; ++w) incl += sel[2 + w];
+        const int excl = incl - cnt;
+        if (k >= excl && k < incl) {
+            sel[0] = tid;
+            sel[1] = excl;
+        }
+        __syncthreads();
+        prefix |= static_cast<uint32_t>(sel[0]) << shift;
+        mask |= 255u << shift;
+        k -= sel[1];
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// numpy median of the n keys: the middle one, or fl(fl(a + b) / 2) of the
+// two middle ones (the upper is the lower again if it repeats, else the
+// smallest key above it).
+__device__ float block_median(const uint32_t* keys, int n, int* hist, int* sel, int* red) {
+    const uint32_t lo = radix_select(keys, n, (n - 1) / 2, hist, sel);
+    if (n & 1) return key_value(lo);
+    int le = 0;
+    uint32_t above = 0xffffffffu;
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const uint32_t key = keys[i];
+        le += key <= lo ? 1 : 0;
+        if (key > lo && key < above) above = key;
+    }
+    le = block_sum_int(le, red);
+    above = block_min_u32(above, reinterpret_cast<uint32_t*>(red));
+    const uint32_t hi = le >= n / 2 + 1 ? lo : above;
     float s = -0.0f;
-    s += a;
-    s += b;
+    s += key_value(lo);
+    s += key_value(hi);
     return s / 2.0f;
 }
 
@@ -202,18 +266,21 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
                                                          int* __restrict__ flags) {
     extern __shared__ uint32_t sm[];
     const int n = static_cast<int>(T);
-    int n2 = 1;
-    while (n2 < n) n2 <<= 1;
     float* a = reinterpret_cast<float*>(sm);  // original order [n]
-    uint32_t* k = sm + n;                     // keys [n2]
+    uint32_t* k = sm + n;                     // order keys [n]
     float* d = reinterpret_cast<float*>(k);   // scratch aliasing keys
-    __shared__ float s_mean, s_med, s_sum;
-    __shared__ int s_nan, s_better;
+    __shared__ float s_mean, s_sum;
+    __shared__ int s_nan;
     __shared__ int wave_tot[kThreads / 64];
+    __shared__ int hist[4 * 256];
+    __shared__ int sel[2 + kThreads / 64];
     __shared__ LeafPlan plan;
     const int64_t c = blockIdx.x;
     const float* row = data + c * T;
-    if (threadIdx.x == 0) s_nan = 0;
+    if (threadIdx.x == 0) {
+        s_nan = 0;
+        plan_leaves(n, plan);
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += kThreads) {
         const float v = row[i];
@@ -249,43 +316,34 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
         summary[c * 4 + 1] = static_cast<float>(__builtin_sqrt(static_cast<double>(var)));
     }
     __syncthreads();
-    // median
-    for (int i = threadIdx.x; i < n2; i += kThreads) k[i] = i < n ? order_key(a[i]) : 0xffffffffu;
+    // median, and the top-n threshold: the n_top-th best key and the number
+    // of keys strictly better than it.
+    for (int i = threadIdx.x; i < n; i += kThreads) k[i] = order_key(a[i]);
     __syncthreads();
-    bitonic_sort(k, n2);
-    if (threadIdx.x == 0) s_med = sorted_median(k, n);
-    __syncthreads();
-    const float med = s_med;
-    // top-n threshold from the sorted keys: the n_top-th best value, and the
-    // number of values strictly better than it (binary search).
+    const float med = block_median(k, n, hist, sel, wave_tot);
     uint32_t thr_key = 0;
-    if (n_top > 0) thr_key = use_max ? k[n - n_top] : k[n_top - 1];
-    if (threadIdx.x == 0 && n_top > 0) {
-        int lo = 0, hi = n;  // first index with key > thr (max) / >= thr (min)
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const bool go_right = use_max ? (k[mid] <= thr_key) : (k[mid] < thr_key);
-            if (go_right) lo = mid + 1;
-            else hi = mid;
-        }
-        s_better = use_max ? (n - lo) : lo;
+    int n_better = 0;
+    if (n_top > 0) {
+        thr_key = radix_select(k, n, use_max ? n - n_top : n_top - 1, hist, sel);
+        int b = 0;
+        for (int i = threadIdx.x; i < n; i += kThreads)
+            b += (use_max ? k[i] > thr_key : k[i] < thr_key) ? 1 : 0;
+        n_better = block_sum_int(b, wave_tot);
     }
-    __syncthreads();
     // MAD
-    for (int i = threadIdx.x; i < n2; i += kThreads)
-        k[i] = i < n ? order_key(__builtin_fabsf(a[i] - med)) : 0xffffffffu;
+    for (int i = threadIdx.x; i < n; i += kThreads) k[i] = order_key(__builtin_fabsf(a[i] - med));
     __syncthreads();
-    bitonic_sort(k, n2);
+    const float mad = block_median(k, n, hist, sel, wave_tot);
     if (threadIdx.x == 0) {
         summary[c * 4 + 2] = med;
-        summary[c * 4 + 3] = sorted_median(k, n);
+        summary[c * 4 + 3] = mad;
     }
     // Top-n: values strictly better than the threshold, then equal values in
     // index order, compacted in index order by block scans; then a best-first
     // insertion sort of the n_top entries (equal values keep index order).
     if (n_top > 0) {
         const float thr = key_value(thr_key);
-        const int nb = s_better;
+        const int nb = n_better;
         int64_t* oi = top_idx + c * n_top;
         float* ov = top_val + c * n_top;
         int base_b = 0, base_t = 0;
@@ -329,13 +387,10 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
 
 }  // namespace
 
-size_t stats_lds_bytes(int64_t T) {
-    int64_t n2 = 1;
-    while (n2 < T) n2 <<= 1;
-    return static_cast<size_t>(T + n2) * 4;
-}
+// Dynamic LDS: values and keys, T each.
+size_t stats_lds_bytes(int64_t T) { return static_cast<size_t>(2 * T) * 4; }
 
-size_t stats_lds_static_bytes() { return sizeof(LeafPlan) + 64; }
+size_t stats_lds_static_bytes() { return sizeof(LeafPlan) + (4 * 256 + 16) * sizeof(int) + 64; }
 
 hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_top, int use_max,
                         float* summary, int64_t* top_idx, float* top_val, int* flags,
