@@ -613,16 +613,23 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
     const float ytot = col[(2 * nb + 1) * kWave];
     float best = 0.0f;
     if (wt > 0.0f) {
+        // The reference's window test is 0 < fl(r/wt) < 1 (bls.rs:119).  For
+        // positive finite wt, fl(r/wt) < 1 iff r < wt (pred(wt)/wt <= 1 - 2^-24
+        // rounds below 1), and fl(r/wt) > 0 iff r/wt > 2^-150, i.e.
+        // r > wt * 2^-150, exact in f64 (rlo); that last test only matters for a
+        // window that would improve the best, so it sits on that path.
+        const double rlo = static_cast<double>(wt) * 0x1p-150;
         double mid = bls_midpoint_up(best);
         // One candidate window [p, p+d): Rust's window sums (bls.rs:104-117),
         // then the exact-max test.  The filter num >= M*den is a superset of
         // fl(num/den) > best (it also admits ties and den == 0); admitted
         // candidates are decided exactly by the IEEE quotient, as in Rust.
         auto consider = [&](float r, float s) {
-            if (r > 0.0f && r < wt) {  // 0 < r/wt < 1 for finite wt > 0
+            if (r > 0.0f && r < wt) {  // superset of 0 < fl(r/wt) < 1, see below
                 const float num = s * s;
                 const float den = r * (wt - r);
-                if (static_cast<double>(num) >= mid * static_cast<double>(den)) {
+                if (static_cast<double>(num) >= mid * static_cast<double>(den) &&
+                    static_cast<double>(r) > rlo) {
                     const float v = num / den;
                     if (v > best) {
                         best = v;
