@@ -59,6 +59,34 @@ __device__ __forceinline__ float fold_phase(double t, double tt, const FoldTrial
     return static_cast<float>(__builtin_amdgcn_fract(q));
 }
 
+// fold_phase for K points at once, stage by stage: each step of the fold is
+// applied to all K points before the next step starts, and scheduling
+// barriers keep that order.  One point's fold is a chain of five dependent
+// f64 operations; interleaving K independent chains lets one wave issue
+// back to back instead of waiting on each result (the compiler otherwise
+// emits the chains one after another).  Same operations, same results.
+template <bool HAS_PDT, int K>
+__device__ __forceinline__ void fold_phases(const double (&t)[K], const double (&tt)[K],
+                                            const FoldTrial& f, float (&ph)[K]) {
+    double tc[K], q0[K], r[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) tc[j] = HAS_PDT ? t[j] - f.k * tt[j] : t[j];
+#pragma unroll
+    for (int j = 0; j < K; ++j) q0[j] = tc[j] * f.inv_p;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) r[j] = __builtin_fma(-q0[j], f.p, tc[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) q0[j] = __builtin_fma(r[j], f.inv_p, q0[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) q0[j] = __builtin_amdgcn_fract(q0[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) ph[j] = static_cast<float>(q0[j]);
+}
+
 // Exact `(phase / b) as usize` with b = fl(1/n) (rust/src/ce.rs:52,
 // aov.rs:55) for a phase in [0, 1], without a division: q0 = phase*RN(1/b),
 // one FMA correction, truncate.  Checked exhaustively over every f32 phase in
@@ -80,6 +108,24 @@ __device__ __forceinline__ uint32_t div_bin(float ph, const DivBin& d) {
     const float r = __builtin_fmaf(-q0, d.b, ph);
     const float q = __builtin_fmaf(r, d.rcp_b, q0);
     return static_cast<uint32_t>(q);
+}
+
+// div_bin for K phases, stage by stage (as fold_phases).
+template <int K>
+__device__ __forceinline__ void div_bins(const float (&ph)[K], const DivBin& d,
+                                         uint32_t (&k)[K]) {
+    float q0[K], r[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) q0[j] = ph[j] * d.rcp_b;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) r[j] = __builtin_fmaf(-q0[j], d.b, ph[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) q0[j] = __builtin_fmaf(r[j], d.rcp_b, q0[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) k[j] = static_cast<uint32_t>(q0[j]);
 }
 
 // `min((phase * n as f32) as usize, n-1)` (rust/src/fpw.rs:48, bls.rs:79).
@@ -148,6 +194,56 @@ __device__ __forceinline__ void lds_rmw(T* col, const uint32_t (&off)[4], const 
     for (int a = 0; a < A; ++a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) r[a][j] = *lds_at(col, off[j], a);
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const T s0 = r[a][0] + val[a][0];
+        const T s1 = (e01 ? s0 : r[a][1]) + val[a][1];
+        const T s2 = (e12 ? s1 : (e02 ? s0 : r[a][2])) + val[a][2];
+        const T s3 = (e23 ? s2 : (e13 ? s1 : (e03 ? s0 : r[a][3]))) + val[a][3];
+        *lds_at(col, off[0], a) = s0;
+        *lds_at(col, off[1], a) = s1;
+        *lds_at(col, off[2], a) = s2;
+        *lds_at(col, off[3], a) = s3;
+    }
+}
+
+// Split form of lds_rmw for software pipelining: lds_read_group issues the
+// reads of a group, lds_finish_group (after independent work) adds the values
+// with the same in-group duplicate handling and writes back.
+template <int A, int K, typename T>
+__device__ __forceinline__ void lds_read_group(T* col, const uint32_t (&off)[K], T (&r)[A][K]) {
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int j = 0; j < K; ++j) r[a][j] = *lds_at(col, off[j], a);
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_finish_group(T* col, const uint32_t (&off)[1],
+                                                 const T (&val)[A][1], const T (&r)[A][1]) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) *lds_at(col, off[0], a) = r[a][0] + val[a][0];
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_finish_group(T* col, const uint32_t (&off)[2],
+                                                 const T (&val)[A][2], const T (&r)[A][2]) {
+    const bool e01 = off[0] == off[1];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const T s0 = r[a][0] + val[a][0];
+        const T s1 = (e01 ? s0 : r[a][1]) + val[a][1];
+        *lds_at(col, off[0], a) = s0;
+        *lds_at(col, off[1], a) = s1;
+    }
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void lds_finish_group(T* col, const uint32_t (&off)[4],
+                                                 const T (&val)[A][4], const T (&r)[A][4]) {
+    const bool e01 = off[0] == off[1];
+    const bool e02 = off[0] == off[2], e12 = off[1] == off[2];
+    const bool e03 = off[0] == off[3], e13 = off[1] == off[3], e23 = off[2] == off[3];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const T s0 = r[a][0] + val[a][0];

@@ -146,13 +146,15 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
         }
     };
     auto run = [&](const Chunk& ch) {
+        float phc[C];
+        fold_phases<HAS_PDT, C>(ch.t, ch.t2, f, phc);
 #pragma unroll
         for (int gi = 0; gi < C; gi += G) {
             float ph[G];
             Rec rc[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
-                ph[j] = fold_phase<HAS_PDT>(ch.t[gi + j], ch.t2[gi + j], f);
+                ph[j] = phc[gi + j];
                 rc[j] = ch.r[gi + j];
             }
             body(ph, rc);
@@ -188,6 +190,95 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
     }
 }
 
+// scan_points for the statistics that update lane-private LDS bins by
+// read-modify-write (AOV, FPW, BLS), software-pipelined: the reads of group
+// g are issued, then group g+1 is folded and binned (independent VALU work,
+// mostly f64), then group g is finished (added, written).  The scheduling
+// barriers keep that order, so the fold covers the LDS read latency instead
+// of each group waiting on its own reads.  `prep(ph[K], rec[K], off[K],
+// val[A][K])` maps a group's phases and records to bin offsets and addends;
+// results are identical to updating one group after another.
+template <bool HAS_PDT, int G, int A, class Load, class Prep>
+__device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
+                                                const double* __restrict__ tt, int64_t n,
+                                                const FoldTrial& f, Load load, float* col,
+                                                Prep prep) {
+    using Rec = decltype(load(int64_t(0)));
+    constexpr int C = HAS_PDT ? kChunk / 2 : kChunk;
+    constexpr int NG = C / G;
+    static_assert(C % G == 0, "chunk must hold whole groups");
+    struct Chunk {
+        double t[C];
+        double t2[C];
+        Rec r[C];
+    };
+    struct Group {
+        uint32_t off[G];
+        float val[A][G];
+    };
+    auto fetch = [&](int64_t base, Chunk& ch) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            ch.t[j] = t64[base + j];
+            ch.t2[j] = HAS_PDT ? tt[base + j] : 0.0;
+            ch.r[j] = load(base + j);
+        }
+    };
+    auto prep_group = [&](const Chunk& ch, int gi, Group& gr) {
+        float ph[G];
+        Rec rc[G];
+        double t[G], t2[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            t[j] = ch.t[gi + j];
+            t2[j] = ch.t2[gi + j];
+            rc[j] = ch.r[gi + j];
+        }
+        fold_phases<HAS_PDT, G>(t, t2, f, ph);
+        prep(ph, rc, gr.off, gr.val);
+    };
+    auto run = [&](const Chunk& ch) {
+        Group cur, nxt;
+        prep_group(ch, 0, cur);
+#pragma unroll
+        for (int gi = 0; gi < NG; ++gi) {
+            float r[A][G];
+            lds_read_group(col, cur.off, r);
+            __builtin_amdgcn_sched_barrier(0);
+            if (gi + 1 < NG) prep_group(ch, (gi + 1) * G, nxt);
+            __builtin_amdgcn_sched_barrier(0);
+            lds_finish_group(col, cur.off, cur.val, r);
+            if (gi + 1 < NG) cur = nxt;
+        }
+    };
+    // Ping-pong chunk prefetch, as in scan_points.
+    int64_t i = 0;
+    if (n >= 2 * C) {
+        const int64_t last = n - C;
+        Chunk a, b;
+        fetch(0, a);
+        for (; i + 2 * C <= n; i += 2 * C) {
+            wait_scalar_loads();
+            fetch(i + C, b);
+            __builtin_amdgcn_sched_barrier(0);
+            run(a);
+            wait_scalar_loads();
+            const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
+            fetch(na, a);
+            __builtin_amdgcn_sched_barrier(0);
+            run(b);
+        }
+    }
+    for (; i < n; ++i) {
+        float ph[1] = {fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
+        Rec rc[1] = {load(i)};
+        uint32_t off[1];
+        float val[A][1];
+        prep(ph, rc, off, val);
+        lds_rmw(col, off, val);
+    }
+}
+
 struct F2 {
     float a, b;
 };
@@ -217,19 +308,23 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
     uint32_t* __restrict__ col = hist + lane;
     const uint32_t row_bytes = nm * kPlaneBytes;
 
-    scan_points<HAS_PDT, (EXT1 && G > 0) ? G : 1>(
+    // The atomic variant takes whole chunks (the histogram is order-free), so
+    // the bins of a chunk are computed stage by stage across its points.
+    constexpr int kCeChunk = HAS_PDT ? kChunk / 2 : kChunk;
+    scan_points<HAS_PDT, EXT1 ? (G > 0 ? G : kCeChunk) : 1>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return mrow[i]; },
         [&](const auto& ph, const auto& mb) {
             constexpr int K = group_size_v<decltype(ph)>;
-            if (EXT1 && G == 0) {
+            if constexpr (EXT1 && G == 0) {
+                uint32_t kb[K];
+                div_bins(ph, db, kb);
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     uint32_t* cell = reinterpret_cast<uint32_t*>(
-                        reinterpret_cast<char*>(col) + __umul24(div_bin(ph[j], db), row_bytes) +
-                        mb[j]);
+                        reinterpret_cast<char*>(col) + __umul24(kb[j], row_bytes) + mb[j]);
                     __hip_atomic_fetch_add(cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 }
-            } else if (EXT1) {
+            } else if constexpr (EXT1) {
                 uint32_t off[K];
                 uint32_t one[1][K];
 #pragma unroll
@@ -330,13 +425,11 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
     const uint32_t nbu = static_cast<uint32_t>(nb);
     constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
 
-    scan_points<HAS_PDT, EXT1 ? G : 1>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; },
-        [&](const auto& ph, const auto& v) {
-            constexpr int K = group_size_v<decltype(ph)>;
-            if (EXT1) {
-                uint32_t off[K];
-                float val[3][K];
+    if constexpr (EXT1) {
+        scan_points_rmw<HAS_PDT, G, 3>(
+            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; }, col,
+            [&](const auto& ph, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const uint32_t k = div_bin(ph[j], db);
@@ -345,19 +438,19 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
                     val[1][j] = v[j].a;
                     val[2][j] = v[j].b;
                 }
-                lds_rmw(col, off, val);
-            } else {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const uint32_t k = div_bin(ph[j], db);
-                    for (int o = 0; o < a.ext; ++o) {
-                        const uint32_t off1[1] = {wrap_add(k, o, nbu) * kBinBytes};
-                        const float val1[3][1] = {{1.0f}, {v[j].a}, {v[j].b}};
-                        lds_rmw(col, off1, val1);
-                    }
+            });
+    } else {
+        scan_points<HAS_PDT, 1>(
+            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; },
+            [&](const auto& ph, const auto& v) {
+                const uint32_t k = div_bin(ph[0], db);
+                for (int o = 0; o < a.ext; ++o) {
+                    const uint32_t off1[1] = {wrap_add(k, o, nbu) * kBinBytes};
+                    const float val1[3][1] = {{1.0f}, {v[0].a}, {v[0].b}};
+                    lds_rmw(col, off1, val1);
                 }
-            }
-        });
+            });
+    }
 
     // aov.rs:67-84.
     const float avg = a.avg[lt.curve];
@@ -496,19 +589,16 @@ __device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, c
     const float nf = static_cast<float>(nb);
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
 
-    scan_points<HAS_PDT, G>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], wy[i]}; },
-        [&](const auto& ph, const auto& v) {
+    scan_points_rmw<HAS_PDT, G, 2>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], wy[i]}; }, col,
+        [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
-            uint32_t off[K];
-            float val[2][K];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
                 val[0][j] = v[j].a;
                 val[1][j] = v[j].b;
             }
-            lds_rmw(col, off, val);
         });
     // fpw.rs:55-60.
     float s = 0.0f;
@@ -585,19 +675,16 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
     constexpr uint32_t kSlotBytes = 2 * kPlaneBytes;
     float* col1 = col + 2 * kWave;  // slot 1
 
-    scan_points<HAS_PDT, G>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; },
-        [&](const auto& ph, const auto& v) {
+    scan_points_rmw<HAS_PDT, G, 2>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; }, col1,
+        [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
-            uint32_t off[K];
-            float val[2][K];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 off[j] = mul_bin(ph[j], nf, nm1) * kSlotBytes;
                 val[0][j] = v[j].a;
                 val[1][j] = v[j].b;
             }
-            lds_rmw(col1, off, val);
         });
     // Prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
     {
