@@ -59,6 +59,16 @@ __device__ __forceinline__ float fold_phase(double t, double tt, const FoldTrial
     return static_cast<float>(__builtin_amdgcn_fract(q));
 }
 
+// Wave-uniform, read-only point data (times and the per-point records)
+// read through the constant address space, so the compiler emits scalar
+// loads into SGPRs: one load per wave, no VGPRs, scalar-cache latency.  The
+// arrays are written by earlier kernels only (the scalar cache is
+// invalidated at each kernel start).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)(p);
+}
+
 // fold_phase for K points at once, stage by stage: each step of the fold is
 // applied to all K points before the next step starts, and scheduling
 // barriers keep that order.  One point's fold is a chain of five dependent

@@ -140,8 +140,8 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
     auto fetch = [&](int64_t base, Chunk& ch) {
 #pragma unroll
         for (int j = 0; j < C; ++j) {
-            ch.t[j] = t64[base + j];
-            ch.t2[j] = HAS_PDT ? tt[base + j] : 0.0;
+            ch.t[j] = uniform_load(t64 + base + j);
+            ch.t2[j] = HAS_PDT ? uniform_load(tt + base + j) : 0.0;
             ch.r[j] = load(base + j);
         }
     };
@@ -197,7 +197,10 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
 // barriers keep that order, so the fold covers the LDS read latency instead
 // of each group waiting on its own reads.  `prep(ph[K], rec[K], off[K],
 // val[A][K])` maps a group's phases and records to bin offsets and addends;
-// results are identical to updating one group after another.
+// results are identical to updating one group after another.  The point
+// stream is read with vector loads here: they count on vmcnt, so the waits
+// on the LDS reads (lgkmcnt) do not also wait for the next chunk's prefetch,
+// as they would for scalar loads (which return out of order on lgkmcnt).
 template <bool HAS_PDT, int G, int A, class Load, class Prep>
 __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
                                                 const double* __restrict__ tt, int64_t n,
@@ -293,10 +296,13 @@ struct F2 {
 template <bool HAS_PDT, bool EXT1, int G>
 __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt) {
     extern __shared__ uint32_t hist[];
+    // G < 0: packed atomics, cells c and c^1 share a word (16-bit counts).
+    constexpr bool PACK = EXT1 && G < 0;
     const int lane = threadIdx.x;
     const int nrows = EXT1 ? a.n_phase + 1 : a.n_phase;
     const int ncell = nrows * a.n_mag;
-    for (int b = 0; b < ncell; ++b) hist[b * kWave + lane] = 0u;
+    const int nword = PACK ? (ncell + 1) / 2 : ncell;
+    for (int b = 0; b < nword; ++b) hist[b * kWave + lane] = 0u;
 
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
@@ -307,15 +313,44 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
     const uint32_t nm = static_cast<uint32_t>(a.n_mag);
     uint32_t* __restrict__ col = hist + lane;
     const uint32_t row_bytes = nm * kPlaneBytes;
+    const bool nm_even = (nm & 1u) == 0;
+    const uint32_t half_row_bytes = (nm / 2) * kPlaneBytes;
 
+    // Scalar loads for the point stream here: no LDS reads wait inside a chunk.
     // The atomic variant takes whole chunks (the histogram is order-free), so
     // the bins of a chunk are computed stage by stage across its points.
     constexpr int kCeChunk = HAS_PDT ? kChunk / 2 : kChunk;
-    scan_points<HAS_PDT, EXT1 ? (G > 0 ? G : kCeChunk) : 1>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return mrow[i]; },
-        [&](const auto& ph, const auto& mb) {
+    // The point body for a compile-time choice of the packed-even layout
+    // (kept out of the loop: one straight-line loop per variant).
+    auto body_for = [&](auto even_tag) {
+        constexpr bool EVEN = decltype(even_tag)::value;
+        return [&](const auto& ph, const auto& mb) {
             constexpr int K = group_size_v<decltype(ph)>;
-            if constexpr (EXT1 && G == 0) {
+            if constexpr (PACK) {
+                uint32_t kb[K];
+                div_bins(ph, db, kb);
+                if constexpr (EVEN) {
+                    // Cell kb*nm + m: word kb*(nm/2) + m/2, half m & 1.  The
+                    // magnitude part is wave-uniform (scalar operations).
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        const uint32_t m = mb[j] / kPlaneBytes;
+                        char* cell = reinterpret_cast<char*>(col) +
+                                     __umul24(kb[j], half_row_bytes) + (m >> 1) * kPlaneBytes;
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(cell),
+                                               1u << ((m & 1u) << 4), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        const uint32_t c = __umul24(kb[j], nm) + mb[j] / kPlaneBytes;
+                        uint32_t* cell = col + (c >> 1) * kWave;
+                        __hip_atomic_fetch_add(cell, 1u << ((c & 1u) << 4), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                }
+            } else if constexpr (EXT1 && G == 0) {
                 uint32_t kb[K];
                 div_bins(ph, db, kb);
 #pragma unroll
@@ -349,11 +384,25 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
                     }
                 }
             }
-        });
-    if (EXT1) {
-        for (int i = 0; i < a.n_mag; ++i)
-            col[i * kWave] += col[(a.n_phase * a.n_mag + i) * kWave];
-    }
+        };
+    };
+    auto scan = [&](auto body) {
+        scan_points<HAS_PDT, EXT1 ? (G > 0 ? G : kCeChunk) : 1>(
+            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(mrow + i); }, body);
+    };
+    if (PACK && nm_even) scan(body_for(std::true_type{}));
+    else scan(body_for(std::false_type{}));
+    // Count of cell (row j, mag i); with EXT1 the spare row n_phase (phase
+    // 1.0) belongs to row 0 (ce.rs:57 wraps it there).
+    auto cell_count = [&](int c) -> uint32_t {
+        if constexpr (PACK) return (col[(c >> 1) * kWave] >> ((c & 1) << 4)) & 0xffffu;
+        else return col[c * kWave];
+    };
+    auto count = [&](int j, int i) -> uint32_t {
+        uint32_t v = cell_count(j * a.n_mag + i);
+        if (EXT1 && j == 0) v += cell_count(a.n_phase * a.n_mag + i);
+        return v;
+    };
 
     // ce.rs:66-86: p = count / (N*ext_p*ext_m); CE = sum_j sum_i p_ij ln(p_j/p_ij).
     const float norm = static_cast<float>(static_cast<uint64_t>(n) *
@@ -361,11 +410,10 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
                                           static_cast<uint64_t>(a.mag_ext));
     float ce = 0.0f;
     for (int j = 0; j < a.n_phase; ++j) {
-        const uint32_t* row = col + j * a.n_mag * kWave;
         float pj = 0.0f;
-        for (int i = 0; i < a.n_mag; ++i) pj += static_cast<float>(row[i * kWave]) / norm;
+        for (int i = 0; i < a.n_mag; ++i) pj += static_cast<float>(count(j, i)) / norm;
         for (int i = 0; i < a.n_mag; ++i) {
-            const float pij = static_cast<float>(row[i * kWave]) / norm;
+            const float pij = static_cast<float>(count(j, i)) / norm;
             if (pij != 0.0f) {
                 const float term = pij * logf(pj / pij);
                 ce += term;
@@ -393,6 +441,7 @@ __global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, Wor
                 continue;
             }
             switch (rank_group(q, it / g.tblocks)) {
+                case -1: ce_block<HAS_PDT, EXT1, -1>(g, a, lt); break;
                 case 0: ce_block<HAS_PDT, EXT1, 0>(g, a, lt); break;
                 case 2: ce_block<HAS_PDT, EXT1, 2>(g, a, lt); break;
                 case 4: ce_block<HAS_PDT, EXT1, 4>(g, a, lt); break;
@@ -441,7 +490,8 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
             });
     } else {
         scan_points<HAS_PDT, 1>(
-            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; },
+            g.t64 + p0, g.tt + p0, n, f,
+            [&](int64_t i) { return F2{uniform_load(y + i), uniform_load(yy + i)}; },
             [&](const auto& ph, const auto& v) {
                 const uint32_t k = div_bin(ph[0], db);
                 for (int o = 0; o < a.ext; ++o) {
@@ -518,7 +568,7 @@ __device__ __forceinline__ void ls_block(const GridArgs& g, const LsArgs& a, con
 
     float mc = 0.0f, ms = 0.0f, cc = 0.0f, cs = 0.0f;
     scan_points<HAS_PDT, 4>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return y[i]; },
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(y + i); },
         [&](const auto& ph, const auto& yv) {
             constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
@@ -825,7 +875,8 @@ void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQue
           hipStream_t s) {
     if (q) launch_persistent(ce_persistent<P, E>, g, a, lds, *q, s);
     else if (!E) launch_grid_kernel(ce_kernel<P, false, 1>, g, a, lds, s);
-    else if (grp <= 0) launch_grid_kernel(ce_kernel<P, E, 0>, g, a, lds, s);
+    else if (grp < 0) launch_grid_kernel(ce_kernel<P, E, -1>, g, a, lds, s);
+    else if (grp == 0) launch_grid_kernel(ce_kernel<P, E, 0>, g, a, lds, s);
     else if (grp >= 4) launch_grid_kernel(ce_kernel<P, E, 4>, g, a, lds, s);
     else if (grp == 2) launch_grid_kernel(ce_kernel<P, E, 2>, g, a, lds, s);
     else launch_grid_kernel(ce_kernel<P, E, 1>, g, a, lds, s);

@@ -85,6 +85,12 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);  // n <
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }
+// CE histogram with two 16-bit counts per 32-bit word (curves of at most
+// kCePackMaxLen points, so no count can carry into its neighbour).
+constexpr int64_t kCePackMaxLen = 65535;
+inline size_t lds_ce_packed(int n_phase, int n_mag) {
+    return size_t((n_phase + 1) * n_mag + 1) / 2 * 64 * 4;
+}
 inline size_t lds_aov(int nb) { return size_t(3) * nb * 64 * 4; }
 inline size_t lds_fpw(int nb) { return size_t(2) * nb * 64 * 4; }
 inline size_t lds_bls(int nb) { return size_t(2) * (nb + 1) * 64 * 4; }

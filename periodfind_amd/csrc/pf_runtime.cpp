@@ -514,7 +514,24 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
     switch (p.algo) {
         case PF_ALGO_CE: {
             pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
-            le = pf::launch_ce(g, a, pf::lds_ce(p.n_phase, p.n_mag), group, q, s);
+            // The atomic variant counts in 16-bit halves (half the LDS, twice
+            // the waves per CU) whenever no count can exceed 65535.
+            const bool can_pack = p.phase_ext == 1 && p.mag_ext == 1 &&
+                                  cs.max_len <= pf::kCePackMaxLen;
+            const size_t lds_full = pf::lds_ce(p.n_phase, p.n_mag);
+            const size_t lds_pack = pf::lds_ce_packed(p.n_phase, p.n_mag);
+            if (q) {
+                pf::WorkQueue qq = *q;
+                bool all_packed = true;
+                for (int b = 0; b < qq.n_buckets; ++b) {
+                    if (qq.bucket_group[b] == 0 && can_pack) qq.bucket_group[b] = -1;
+                    if (qq.bucket_group[b] >= 0) all_packed = false;
+                }
+                le = pf::launch_ce(g, a, all_packed ? lds_pack : lds_full, 0, &qq, s);
+            } else {
+                const int grp = (group == 0 && can_pack) ? -1 : group;
+                le = pf::launch_ce(g, a, grp < 0 ? lds_pack : lds_full, grp, nullptr, s);
+            }
             break;
         }
         case PF_ALGO_AOV: {

@@ -347,3 +347,22 @@ def test_tuning_table_buckets_keep_results(ctx, tmp_path):
         assert ctx.arch().startswith("gfx")
     finally:
         ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
+
+
+@pytest.mark.parametrize("n", [65535, 65536])
+def test_ce_counts_at_the_packing_limit(ctx, n):
+    """CE keeps two 16-bit counts per LDS word for curves of <= 65535 points.
+    At the limit: every point in one cell, and two cells of one word filled
+    40000 / rest; 65536 points take the 32-bit layout."""
+    rng = np.random.default_rng(41)
+    P = np.array([1.0, 2.0, 3.0, 0.7], np.float32)
+    D = np.zeros(1, np.float32)
+    t = np.arange(n, dtype=np.float32)  # integer days: phase 0 for P = 1
+    one_cell = np.full(n, 0.25, np.float32)
+    two = np.where(np.arange(n) < 40000, 0.35, 0.25).astype(np.float32)  # mag bins 3, 2: one word
+    noisy = rng.uniform(0.01, 0.99, n).astype(np.float32)
+    mags = [one_cell, two, noisy]
+    times = [t, t, t]
+    got = ctx.grid("ce", times, mags, P, D, n_phase=10, n_mag=10)
+    ref = O.ce_grid(times, mags, P, D, n_phase=10, n_mag=10)
+    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-7)
