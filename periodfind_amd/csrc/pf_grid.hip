@@ -408,8 +408,18 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
     const float norm = static_cast<float>(static_cast<uint64_t>(n) *
                                           static_cast<uint64_t>(a.phase_ext) *
                                           static_cast<uint64_t>(a.mag_ext));
+    // Packed counts of a 65536-point curve overflow only if all points share
+    // one cell (see kCePackMaxLen); then p = 1 there and CE = 1 * ln(1/1) = 0.
+    bool one_cell = false;
+    if constexpr (PACK) {
+        if (n > 65535) {
+            uint32_t total = 0;
+            for (int c = 0; c < ncell; ++c) total += cell_count(c);
+            one_cell = total != static_cast<uint32_t>(n);
+        }
+    }
     float ce = 0.0f;
-    for (int j = 0; j < a.n_phase; ++j) {
+    for (int j = 0; j < a.n_phase && !one_cell; ++j) {
         float pj = 0.0f;
         for (int i = 0; i < a.n_mag; ++i) pj += static_cast<float>(count(j, i)) / norm;
         for (int i = 0; i < a.n_mag; ++i) {

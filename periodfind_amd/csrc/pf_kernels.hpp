@@ -85,9 +85,11 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);  // n <
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }
-// CE histogram with two 16-bit counts per 32-bit word (curves of at most
-// kCePackMaxLen points, so no count can carry into its neighbour).
-constexpr int64_t kCePackMaxLen = 65535;
+// CE histogram with two 16-bit counts per 32-bit word.  Up to 65535 points no
+// count can carry into its neighbour; at exactly 65536 a carry happens only
+// when every point falls in one cell, which the kernel detects (the decoded
+// counts no longer sum to N) and which has CE = 0 exactly.
+constexpr int64_t kCePackMaxLen = 65536;
 inline size_t lds_ce_packed(int n_phase, int n_mag) {
     return size_t((n_phase + 1) * n_mag + 1) / 2 * 64 * 4;
 }

@@ -349,11 +349,11 @@ def test_tuning_table_buckets_keep_results(ctx, tmp_path):
         ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
 
 
-@pytest.mark.parametrize("n", [65535, 65536])
+@pytest.mark.parametrize("n", [65535, 65536, 65537])
 def test_ce_counts_at_the_packing_limit(ctx, n):
-    """CE keeps two 16-bit counts per LDS word for curves of <= 65535 points.
-    At the limit: every point in one cell, and two cells of one word filled
-    40000 / rest; 65536 points take the 32-bit layout."""
+    """CE keeps two 16-bit counts per LDS word for curves of <= 65536 points.
+    At the limit: every point in one cell (at 65536 the count wraps, which the
+    kernel detects: CE = 0), and two cells of one word filled 40000 / rest."""
     rng = np.random.default_rng(41)
     P = np.array([1.0, 2.0, 3.0, 0.7], np.float32)
     D = np.zeros(1, np.float32)
