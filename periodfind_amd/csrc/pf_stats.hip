@@ -323,7 +323,14 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(const float* __restrict
     const float med = block_median(k, n, hist, sel, wave_tot);
     uint32_t thr_key = 0;
     int n_better = 0;
-    if (n_top > 0) {
+    if (n_top == 1) {
+        // The best key by a block reduction; nothing is strictly better.
+        uint32_t best = use_max ? 0u : 0xffffffffu;
+        for (int i = threadIdx.x; i < n; i += kThreads)
+            best = use_max ? (k[i] > best ? k[i] : best) : (k[i] < best ? k[i] : best);
+        thr_key = use_max ? ~block_min_u32(~best, reinterpret_cast<uint32_t*>(wave_tot))
+                          : block_min_u32(best, reinterpret_cast<uint32_t*>(wave_tot));
+    } else if (n_top > 1) {
         thr_key = radix_select(k, n, use_max ? n - n_top : n_top - 1, hist, sel);
         int b = 0;
         for (int i = threadIdx.x; i < n; i += kThreads)
