@@ -747,9 +747,27 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
             }
         });
     // Prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
+    // Eight slots are read before any is added, so the LDS latency is paid
+    // once per eight slots rather than once per slot.
     {
         float pw = 0.0f, py = 0.0f;
-        for (int k = 1; k <= nb; ++k) {
+        int k = 1;
+        for (; k + 7 <= nb; k += 8) {
+            float bw[8], by[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                bw[j] = col[(2 * (k + j)) * kWave];
+                by[j] = col[(2 * (k + j) + 1) * kWave];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                pw = pw + bw[j];
+                py = py + by[j];
+                col[(2 * (k + j)) * kWave] = pw;
+                col[(2 * (k + j) + 1) * kWave] = py;
+            }
+        }
+        for (; k <= nb; ++k) {
             pw = pw + col[(2 * k) * kWave];
             py = py + col[(2 * k + 1) * kWave];
             col[(2 * k) * kWave] = pw;
@@ -767,59 +785,83 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
         // window that would improve the best, so it sits on that path.
         const double rlo = static_cast<double>(wt) * 0x1p-150;
         double mid = bls_midpoint_up(best);
-        // One candidate window [p, p+d): Rust's window sums (bls.rs:104-117),
-        // then the exact-max test.  The filter num >= M*den is a superset of
-        // fl(num/den) > best (it also admits ties and den == 0); admitted
-        // candidates are decided exactly by the IEEE quotient, as in Rust.
-        auto consider = [&](float r, float s) {
-            if (r > 0.0f && r < wt) {  // superset of 0 < fl(r/wt) < 1, see below
-                const float num = s * s;
-                const float den = r * (wt - r);
-                if (static_cast<double>(num) >= mid * static_cast<double>(den) &&
-                    static_cast<double>(r) > rlo) {
-                    const float v = num / den;
-                    if (v > best) {
-                        best = v;
-                        mid = bls_midpoint_up(best);
-                    }
+        // Candidate windows [p, p+d) are examined in groups of four: their
+        // sums (Rust's window sums, bls.rs:104-117) and the admission test
+        // num >= M*den are branch-free; only a group in which some lane
+        // admits a candidate takes the exact path.  Admission is a superset
+        // of fl(num/den) > best (it also admits ties and den == 0), and the
+        // exact path decides each admitted candidate, in duration order, by
+        // the IEEE quotient as Rust does.  A candidate not admitted under
+        // the group's starting best cannot beat a later, larger best.
+        // r in (0, wt) is min(r, wt - r) > 0: wt - r > 0 iff r < wt in f32.
+        auto admit = [&](float r, float s, float& num, float& den) -> bool {
+            const float t = wt - r;
+            num = s * s;
+            den = r * t;
+            return (fminf(r, t) > 0.0f) &
+                   (static_cast<double>(num) >= mid * static_cast<double>(den));
+        };
+        auto improve = [&](bool ok, float r, float num, float den) {
+            if (ok && static_cast<double>(r) > rlo) {
+                const float v = num / den;
+                if (v > best) {
+                    best = v;
+                    mid = bls_midpoint_up(best);
                 }
             }
         };
+        constexpr int kG = 4;  // candidates per group
+        auto group = [&](const float (&r)[kG], const float (&s)[kG]) {
+            float num[kG], den[kG];
+            bool ok[kG];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                ok[j] = admit(r[j], s[j], num[j], den[j]);
+                any |= ok[j];
+            }
+            if (any) {
+#pragma unroll
+                for (int j = 0; j < kG; ++j) improve(ok[j], r[j], num[j], den[j]);
+            }
+        };
+        auto one = [&](float r, float s) {
+            float num, den;
+            const bool ok = admit(r, s, num, den);
+            if (ok) improve(ok, r, num, den);
+        };
+        const float* pw = col;          // W plane of slot k at pw[2k*64]
+        const float* py = col + kWave;  // Y plane
         for (int p = 0; p < nb; ++p) {
-            const float w0 = col[(2 * p) * kWave];
-            const float y0 = col[(2 * p + 1) * kWave];
+            const float w0 = pw[(2 * p) * kWave];
+            const float y0 = py[(2 * p) * kWave];
             const float wtail = wt - w0;    // wp[nb] - wp[phi]
             const float ytail = ytot - y0;  // yp[nb] - yp[phi]
-            auto window = [&](int d, float& r, float& s) {
-                const int e = p + d;
-                if (e <= nb) {
-                    r = col[(2 * e) * kWave] - w0;
-                    s = col[(2 * e + 1) * kWave] - y0;
-                } else {
-                    const int wr = e - nb;
-                    r = wtail + col[(2 * wr) * kWave];
-                    s = ytail + col[(2 * wr + 1) * kWave];
-                }
-            };
+            // Durations up to nb - p stay inside the bins: r = wp[p+d] - wp[p].
+            // Longer ones wrap: r = (wp[nb] - wp[p]) + wp[p+d-nb].
+            const int d_in = min(a.d_hi, nb - p);
             int d = a.d_lo;
-            // Four durations per step: their LDS reads are issued before any
-            // of them is evaluated, so the read latency is paid once.
-            for (; d + 3 <= a.d_hi; d += 4) {
-                float r0, s0, r1, s1, r2, s2, r3, s3;
-                window(d, r0, s0);
-                window(d + 1, r1, s1);
-                window(d + 2, r2, s2);
-                window(d + 3, r3, s3);
-                consider(r0, s0);
-                consider(r1, s1);
-                consider(r2, s2);
-                consider(r3, s3);
+            for (; d + kG - 1 <= d_in; d += kG) {
+                float r[kG], s[kG];
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    r[j] = pw[(2 * (p + d + j)) * kWave] - w0;
+                    s[j] = py[(2 * (p + d + j)) * kWave] - y0;
+                }
+                group(r, s);
             }
-            for (; d <= a.d_hi; ++d) {
-                float r, s;
-                window(d, r, s);
-                consider(r, s);
+            for (; d <= d_in; ++d) one(pw[(2 * (p + d)) * kWave] - w0, py[(2 * (p + d)) * kWave] - y0);
+            for (; d + kG - 1 <= a.d_hi; d += kG) {
+                float r[kG], s[kG];
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    r[j] = wtail + pw[(2 * (p + d + j - nb)) * kWave];
+                    s[j] = ytail + py[(2 * (p + d + j - nb)) * kWave];
+                }
+                group(r, s);
             }
+            for (; d <= a.d_hi; ++d)
+                one(wtail + pw[(2 * (p + d - nb)) * kWave], ytail + py[(2 * (p + d - nb)) * kWave]);
         }
     }
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = best;
