@@ -61,6 +61,19 @@ def measured_traffic(kernel):
         return None
 
 
+def measured_util(kernel):
+    """rocprofv3 VALUBusy / LdsUtil (percent) of `kernel` from the committed
+    counter summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "util_batch.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        return {"valu_busy_pct": k["VALUBusy"], "lds_util_pct": k["LdsUtil"],
+                "source": "profiles/batch_util_pmc.md"}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def prep_mags(algo, mags):
     from periodfind_amd._utils import prepare_magnitudes
 
@@ -574,7 +587,8 @@ def main():
                    "n_curves": N_CURVES, "sizes": list(SIZES), "n_periods": 1000},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(ach, 3),
                      "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom)},
+                     "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom),
+                     "counters": measured_util(dom)},
         "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
         "kernels": detail,
     }
