@@ -156,6 +156,17 @@ int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const flo
 /* Algorithm selector for the generic entry points below. */
 enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4 };
 
+/* Magnitude preparation applied on the device before evaluation
+ * (pf_params.mag_prep): prepare_magnitudes of periodfind/_utils.py:12-47,
+ * which the reference runs on the host before every call, reproduced in
+ * numpy float32 arithmetic bit for bit.
+ *   PF_MAG_AS_IS      magnitudes are used as given;
+ *   PF_MAG_CENTER     m - np.mean(m) per curve (LS's default);
+ *   PF_MAG_NORMALIZE  ((m - min) / (max - min)) * 0.999 + 5e-4 per curve
+ *                     (CE's default); NaN anywhere in a curve, or a constant
+ *                     curve, gives NaN magnitudes, as numpy does. */
+enum { PF_MAG_AS_IS = 0, PF_MAG_CENTER = 1, PF_MAG_NORMALIZE = 2 };
+
 /* Algorithm parameters; unused fields are ignored. */
 typedef struct pf_params {
     int algo;        /* PF_ALGO_* */
@@ -166,6 +177,7 @@ typedef struct pf_params {
     int n_bins;      /* FPW / BLS bins */
     float qmin;      /* BLS */
     float qmax;      /* BLS */
+    int mag_prep;    /* PF_MAG_*; 0 = as given */
 } pf_params;
 
 /* Grid evaluation followed by on-device peak selection (peaks.rs:18-238).
@@ -195,6 +207,19 @@ typedef struct pf_curves pf_curves;
 int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves, const int64_t* offsets,
                      const float* t, const float* y, const float* err);
 void pf_curves_destroy(pf_curves* cv);
+
+/* Refill an existing set from per-curve host arrays: t[i], y[i] (and err[i]
+ * when err != NULL) hold lengths[i] float32 values each.  This is the
+ * reference's own boundary form (vector<float*> + lengths, ce.pyx:159-175
+ * and CalcCEValsBatched in cuda/ce.h), replacing its pageable per-call
+ * copies (cuda/ce.cu:513-518): the curves are packed CSR into the context's
+ * reusable pinned staging buffers chunk by chunk, and each chunk's
+ * host-to-device copy is issued before the next chunk is packed, so packing
+ * overlaps the transfer.  The set's device buffers grow as needed and its
+ * cached derived arrays are dropped.  Returns when the data are on the
+ * device (the caller may then reuse its arrays). */
+int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float* const* t,
+                   const float* const* y, const float* const* err, const int64_t* lengths);
 
 /* Full grid over a resident set.  The periodogram stays in device memory
  * (pf_curves_result) and is also copied to `out` when out != NULL. */

@@ -14,6 +14,7 @@ import numpy as np
 
 cimport numpy as cnp
 from libc.stdint cimport int64_t
+from libc.stdlib cimport free, malloc
 
 cnp.import_array()
 
@@ -29,6 +30,9 @@ cdef extern from "periodfind_amd.h":
         int n_bins
         float qmin
         float qmax
+        int mag_prep
+    ctypedef struct pf_curves:
+        pass
 
     int PF_OK
     int pf_device_count(int* count)
@@ -73,8 +77,26 @@ cdef extern from "periodfind_amd.h":
                  int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
                  int* fallback) nogil
     int pf_last_kernel_ms(pf_ctx* ctx, float* ms)
+    int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves,
+                         const int64_t* offsets, const float* t, const float* y,
+                         const float* err) nogil
+    void pf_curves_destroy(pf_curves* cv) nogil
+    int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float** t,
+                       const float** y, const float** err, const int64_t* lengths) nogil
+    int pf_curves_grid(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
+                       const float* periods, int64_t n_periods, const float* period_dts,
+                       int64_t n_pdts, float* out) nogil
+    int pf_curves_peaks(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
+                        const float* periods, int64_t n_periods, const float* period_dts,
+                        int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                        float* val) nogil
+    int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
+                        const float* periods, int64_t n_periods, const float* period_dts,
+                        int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
+                        int* fallback) nogil
 
 ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
+MAG_PREP = {None: 0, "as_is": 0, "center": 1, "normalize": 2}
 
 
 def device_count():
@@ -104,26 +126,55 @@ cdef pf_params _params(str algo, dict p):
     prm.n_bins = p.get("n_bins", 10)
     prm.qmin = p.get("qmin", 0.01)
     prm.qmax = p.get("qmax", 0.5)
+    prm.mag_prep = MAG_PREP[p.get("mag_prep")]
     return prm
+
+
+cdef list _f32_arrays(object arrays, str name):
+    """The arrays as C-contiguous 1-D float32 ndarrays (no copy when they
+    already are)."""
+    out = []
+    for i, a in enumerate(arrays):
+        if not isinstance(a, np.ndarray) or (<cnp.ndarray> a).dtype != np.float32:
+            raise TypeError(f"{name}[{i}] must be a float32 ndarray")
+        if (<cnp.ndarray> a).ndim != 1:
+            a = a.reshape(-1)
+        if not (<cnp.ndarray> a).flags.c_contiguous:
+            a = np.ascontiguousarray(a)
+        out.append(a)
+    return out
 
 
 cdef class Context:
     """A device context (stream, reusable device buffers) on one GPU.
-    Calls on one Context are serialised by a lock."""
+    Calls on one Context are serialised by a re-entrant lock; a caller that
+    loads curves and then evaluates them holds `lock` across both."""
 
     cdef pf_ctx* ctx
-    cdef object lock
+    cdef pf_curves* cv  # the curve set of the list-based calls (load / set_*)
+    cdef readonly object lock  # re-entrant: hold it across load + set_* calls
     cdef readonly int device
+    cdef readonly int64_t n_loaded
 
     def __cinit__(self, int device=0):
         self.ctx = NULL
+        self.cv = NULL
+        self.n_loaded = 0
         self.device = device
-        self.lock = threading.Lock()
+        self.lock = threading.RLock()
         cdef int rc = pf_ctx_create(&self.ctx, device)
+        if rc != 0:
+            _raise(rc)
+        cdef int64_t zero = 0
+        rc = pf_curves_create(self.ctx, &self.cv, 0, &zero, NULL, NULL, NULL)
         if rc != 0:
             _raise(rc)
 
     def __dealloc__(self):
+        if self.cv != NULL:
+            with nogil:
+                pf_curves_destroy(self.cv)
+            self.cv = NULL
         if self.ctx != NULL:
             with nogil:
                 pf_ctx_destroy(self.ctx)
@@ -133,6 +184,113 @@ cdef class Context:
         cdef float ms = 0.0
         pf_last_kernel_ms(self.ctx, &ms)
         return ms
+
+    def load(self, times, mags, errs=None):
+        """Per-curve float32 arrays (the reference's list-of-arrays input)
+        into this context's curve set with pf_curves_load: packed into pinned
+        staging chunk by chunk, each chunk's copy overlapping the packing of
+        the next.  Later set_grid / set_stats / set_peaks calls evaluate it."""
+        cdef Py_ssize_t n = len(times)
+        ts = _f32_arrays(times, "times")
+        ms = _f32_arrays(mags, "mags")
+        es = _f32_arrays(errs, "errs") if errs is not None else None
+        if len(ms) != n or (es is not None and len(es) != n):
+            raise ValueError("times, mags and errs must hold the same number of curves")
+        cdef Py_ssize_t k = n if n > 0 else 1
+        cdef const float** tp = <const float**> malloc(k * sizeof(float*))
+        cdef const float** yp = <const float**> malloc(k * sizeof(float*))
+        cdef const float** ep = <const float**> malloc(k * sizeof(float*)) if es is not None else NULL
+        cdef int64_t* lens = <int64_t*> malloc(k * sizeof(int64_t))
+        cdef Py_ssize_t i
+        cdef int rc = 0
+        try:
+            if tp == NULL or yp == NULL or lens == NULL or (es is not None and ep == NULL):
+                raise MemoryError()
+            for i in range(n):
+                lens[i] = (<cnp.ndarray> ts[i]).shape[0]
+                if (<cnp.ndarray> ms[i]).shape[0] != lens[i] or (
+                        es is not None and (<cnp.ndarray> es[i]).shape[0] != lens[i]):
+                    raise ValueError(f"curve {i}: times, mags and errs lengths differ")
+                tp[i] = <const float*> cnp.PyArray_DATA(ts[i])
+                yp[i] = <const float*> cnp.PyArray_DATA(ms[i])
+                if es is not None:
+                    ep[i] = <const float*> cnp.PyArray_DATA(es[i])
+            with self.lock:
+                with nogil:
+                    rc = pf_curves_load(self.ctx, self.cv, n, tp, yp, ep, lens)
+                if rc == 0:
+                    self.n_loaded = n
+        finally:
+            free(tp)
+            free(yp)
+            free(ep)
+            free(lens)
+        if rc != 0:
+            _raise(rc)
+
+    def set_grid(self, str algo, dict p, cnp.ndarray periods, cnp.ndarray pdts):
+        """Full periodogram of the loaded set -> float32[C, n_periods, n_pdts]."""
+        cdef pf_params prm = _params(algo, p)
+        out = np.zeros((self.n_loaded, periods.shape[0], pdts.shape[0]), dtype=np.float32)
+        cdef float* op = <float*> cnp.PyArray_DATA(out)
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_curves_grid(self.ctx, self.cv, &prm, pp, NP, dp, ND, op)
+        if rc != 0:
+            _raise(rc)
+        return out
+
+    def set_stats(self, str algo, dict p, cnp.ndarray periods, cnp.ndarray pdts, int n_stats):
+        """Device 'stats' of the loaded set (see stats)."""
+        cdef int64_t C = self.n_loaded
+        cdef pf_params prm = _params(algo, p)
+        summary = np.zeros((C, 4), dtype=np.float32)
+        idx = np.zeros((C, n_stats), dtype=np.int64)
+        val = np.zeros((C, n_stats), dtype=np.float32)
+        fb = np.zeros(C, dtype=np.int32)
+        cdef float* sp = <float*> cnp.PyArray_DATA(summary)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef int* fp = <int*> cnp.PyArray_DATA(fb)
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_curves_stats(self.ctx, self.cv, &prm, pp, NP, dp, ND, n_stats, sp, ip,
+                                     vp, fp)
+        if rc != 0:
+            _raise(rc)
+        return summary, idx, val, fb
+
+    def set_peaks(self, str algo, dict p, cnp.ndarray periods, cnp.ndarray pdts,
+                  int64_t n_peaks, int64_t min_distance):
+        """Device top-N peaks of the loaded set (see peaks)."""
+        cdef int64_t C = self.n_loaded
+        cdef pf_params prm = _params(algo, p)
+        idx = np.full((C, n_peaks), -1, dtype=np.int64)
+        val = np.zeros((C, n_peaks), dtype=np.float32)
+        cdef int64_t* ip = <int64_t*> cnp.PyArray_DATA(idx)
+        cdef float* vp = <float*> cnp.PyArray_DATA(val)
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts)
+        cdef int64_t NP = periods.shape[0]
+        cdef int64_t ND = pdts.shape[0]
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_curves_peaks(self.ctx, self.cv, &prm, pp, NP, dp, ND, n_peaks,
+                                     min_distance, ip, vp)
+        if rc != 0:
+            _raise(rc)
+        return idx, val
 
     def grid(self, str algo, dict p, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
              cnp.ndarray periods, cnp.ndarray pdts, cnp.ndarray out):

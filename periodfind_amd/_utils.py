@@ -31,6 +31,29 @@ def prepare_magnitudes(mags, center, normalize):
     return mags
 
 
+def magnitude_mode(mags, center, normalize):
+    """The preparation prepare_magnitudes(mags, center, normalize) applies,
+    as a mode for the device to apply instead: "center", "normalize" or None.
+    Warns as prepare_magnitudes does when both are set, and lets numpy raise
+    or warn on empty curves exactly as it would there (np.min of an empty
+    array raises ValueError; np.mean of one warns)."""
+    if center and normalize:
+        warnings.warn(
+            "Center and normalize are conflicting settings. Normalize will be ignored.",
+            RuntimeWarning,
+            stacklevel=3,
+        )
+    mode = "center" if center else ("normalize" if normalize else None)
+    if mode is not None:
+        for mag in mags:
+            if len(mag) == 0:
+                if mode == "center":
+                    np.mean(mag)
+                else:
+                    np.min(mag)
+    return mode
+
+
 def validate_inputs(times, mags):
     """Same number of curves and equal per-curve lengths (_utils.py:50-76)."""
     if len(times) != len(mags):

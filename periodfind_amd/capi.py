@@ -55,6 +55,7 @@ class PfParams(ctypes.Structure):
         ("n_bins", ctypes.c_int),
         ("qmin", ctypes.c_float),
         ("qmax", ctypes.c_float),
+        ("mag_prep", ctypes.c_int),
     ]
 
 
@@ -93,6 +94,8 @@ SIGNATURES = {
     "pf_curves_create": (_int, [_vp, ctypes.POINTER(_vp), _i64, _i64p, _f32p, _f32p,
                                 _f32p_or_null]),
     "pf_curves_destroy": (None, [_vp]),
+    "pf_curves_load": (_int, [_vp, _vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                              ctypes.POINTER(_vp), _i64p]),
     "pf_curves_grid": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
                               _f32p_or_null]),
     "pf_curves_result": (_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
@@ -310,8 +313,11 @@ def _refine_outputs(C, n):
             np.zeros((C, n), np.float64), np.zeros((C, n), np.int32))
 
 
+PF_MAG = {"as_is": 0, "center": 1, "normalize": 2}
+
+
 def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
-                qmax=0.5, ext=None):
+                qmax=0.5, ext=None, mag_prep="as_is"):
     prm = PfParams()
     prm.algo = PF_ALGO[algo]
     if n_phase is None:
@@ -325,6 +331,7 @@ def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=Non
     prm.n_bins = n_bins
     prm.qmin = qmin
     prm.qmax = qmax
+    prm.mag_prep = PF_MAG[mag_prep] if isinstance(mag_prep, str) else int(mag_prep)
     return prm
 
 
@@ -339,6 +346,23 @@ class CurveSet:
         e = pack(errs)[1] if errs is not None else None
         self.n_curves = len(times)
         check(lib().pf_curves_create(ctx.h, ctypes.byref(self.h), len(times), off, t, y, e))
+
+    def load(self, times, mags, errs=None):
+        """Refill the set from per-curve float32 arrays (pf_curves_load:
+        pinned, chunked, packing overlapped with the transfer)."""
+        n = len(times)
+        arrs = [[np.ascontiguousarray(a, dtype=np.float32) for a in lst]
+                for lst in (times, mags, errs if errs is not None else [])]
+        lens = np.array([a.size for a in arrs[0]], dtype=np.int64)
+        for lst, name in ((arrs[1], "mags"), (arrs[2], "errs")):
+            if lst and (len(lst) != n or any(a.size != L for a, L in zip(lst, lens))):
+                raise ValueError(f"{name} must match times curve by curve")
+        if len(arrs[1]) != n:
+            raise ValueError("mags must match times curve by curve")
+        ptrs = [(_vp * max(n, 1))(*[a.ctypes.data for a in lst]) if lst else None for lst in arrs]
+        check(lib().pf_curves_load(self.ctx.h, self.h, n, ptrs[0], ptrs[1], ptrs[2], lens))
+        self.n_curves = n
+        return self
 
     def close(self):
         if self.h:

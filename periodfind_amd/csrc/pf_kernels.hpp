@@ -106,6 +106,10 @@ hipError_t launch_square(const float* y, float* yy, int64_t npts, hipStream_t s)
 hipError_t launch_inv_var(const float* e, float* w, int64_t npts, hipStream_t s);
 hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, hipStream_t s);
 // Serial per-curve reductions in the reference's order (one lane per curve).
+// prepare_magnitudes on the device (mode PF_MAG_CENTER or PF_MAG_NORMALIZE),
+// y -> yp per curve, numpy float32 semantics.
+hipError_t launch_prep_mags(const int64_t* off, const float* y, float* yp, int64_t n_curves,
+                            int mode, hipStream_t s);
 hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int64_t n_curves,
                              hipStream_t s);
 hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,

@@ -4,8 +4,10 @@
 // reductions (AOV mean, BLS weighted mean) must reproduce the reference's
 // serial f32 summation order, so they run one lane per curve; a curve set
 // computes them once and caches the result.
+#include "periodfind_amd.h"
 #include "pf_device.hpp"
 #include "pf_kernels.hpp"
+#include "pf_numpy.hpp"
 
 namespace pf {
 
@@ -145,6 +147,73 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
     if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }
 
+// prepare_magnitudes (periodfind/_utils.py:12-47) in numpy float32
+// arithmetic, one workgroup per curve:
+//   center:    m - np.mean(m), the mean from numpy's blocked pairwise sum
+//              divided by n (f64 quotient rounded to f32, which equals the
+//              correctly rounded f32 quotient for f32 operands);
+//   normalize: ((m - min) / (max - min)) * 0.999 + 5e-4, the constants as
+//              float32 (numpy 2 casts Python float scalars to the array's
+//              dtype).  np.min / np.max propagate NaN, so a curve holding NaN
+//              becomes all NaN; a constant curve divides 0 by 0 (NaN), as
+//              numpy does.
+// Every operation is a single IEEE f32 operation (no contraction), so the
+// result is bit-identical to numpy's.
+__global__ __launch_bounds__(kBlock) void prep_mags_kernel(const int64_t* __restrict__ off,
+                                                           const float* __restrict__ y,
+                                                           float* __restrict__ yp, int mode) {
+    const int64_t c = blockIdx.x;
+    const int64_t p0 = off[c];
+    const int64_t n = off[c + 1] - p0;
+    if (n <= 0) return;
+    const float* x = y + p0;
+    float* o = yp + p0;
+    if (mode == PF_MAG_CENTER) {
+        __shared__ npsum::LeafPlan plan;
+        __shared__ float s_sum;
+        npsum::block_numpy_sum(x, n, plan, &s_sum);
+        const float mean = static_cast<float>(static_cast<double>(s_sum) / static_cast<double>(n));
+        for (int64_t i = threadIdx.x; i < n; i += kBlock) o[i] = x[i] - mean;
+        return;
+    }
+    // normalize
+    float mn = __builtin_inff(), mx = -__builtin_inff();
+    int nan = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+        const float v = x[i];
+        nan |= v != v;
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+#pragma unroll
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, o2));
+        mx = fmaxf(mx, __shfl_xor(mx, o2));
+        nan |= __shfl_xor(nan, o2);
+    }
+    __shared__ float w_mn[kBlock / 64], w_mx[kBlock / 64];
+    __shared__ int w_nan[kBlock / 64];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        w_mn[wv] = mn;
+        w_mx[wv] = mx;
+        w_nan[wv] = nan;
+    }
+    __syncthreads();
+    mn = w_mn[0];
+    mx = w_mx[0];
+    nan = w_nan[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+        mn = fminf(mn, w_mn[w]);
+        mx = fmaxf(mx, w_mx[w]);
+        nan |= w_nan[w];
+    }
+    if (nan) mn = mx = __builtin_nanf("");
+    const float range = mx - mn;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) o[i] = ((x[i] - mn) / range) * 0.999f + 5e-4f;
+}
+
 }  // namespace
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
@@ -207,6 +276,15 @@ hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w
     if (bx > 64) bx = 64;
     hipLaunchKernelGGL(bls_center_kernel, dim3(static_cast<uint32_t>(bx * n_curves)),
                        dim3(kBlock), 0, s, off, y, w, mean, yw, bx);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_mags(const int64_t* off, const float* y, float* yp, int64_t n_curves,
+                            int mode, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    if (mode != PF_MAG_CENTER && mode != PF_MAG_NORMALIZE) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(prep_mags_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock), 0,
+                       s, off, y, yp, mode);
     return hipGetLastError();
 }
 

@@ -89,6 +89,12 @@ struct CurveSet {
     int64_t max_len = 0;
     bool has_err = false;
     DevBuf off, t, y, e;
+    // Prepared magnitudes (PF_MAG_CENTER / PF_MAG_NORMALIZE of y) and the
+    // mode they hold (-1: none); derived arrays below were built from the
+    // magnitudes of mode `derived_mode`.
+    DevBuf yp;
+    int yp_mode = -1;
+    int derived_mode = PF_MAG_AS_IS;
     // Derived, with validity flags.
     DevBuf t64, tt;
     bool t64_ok = false, tt_ok = false;
@@ -124,11 +130,19 @@ struct CurveSet {
         t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = order_ok = false;
         ce_nmag = -1;
         out_T = 0;
+        yp_mode = -1;
+        derived_mode = PF_MAG_AS_IS;
+        drop_graph();
+    }
+    // Drop what was built from the magnitudes (a new preparation mode).
+    void invalidate_mags() {
+        aov_ok = fpw_ok = bls_ok = false;
+        ce_nmag = -1;
         drop_graph();
     }
     void release() {
         drop_graph();
-        DevBuf* bufs[] = {&off, &t, &y, &e, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
+        DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
                           &fpw_wy, &bls_yw, &bls_mean, &out, &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
@@ -165,6 +179,11 @@ struct pf_ctx {
     DevBuf outbuf;
     void* host_stage = nullptr;
     size_t host_cap = 0;
+    // Input staging of pf_curves_load: two pinned chunks, each reused once
+    // the copy issued from it has completed (its event).
+    void* in_stage[2] = {nullptr, nullptr};
+    hipEvent_t in_ev[2] = {nullptr, nullptr};
+    bool in_busy[2] = {false, false};
     // LS refinement: frequency grid (f64) and per-peak brackets.
     DevBuf ls_freq, ls_lo, ls_hi;
 };
@@ -180,6 +199,7 @@ struct Params {
     int algo;
     int n_phase, n_mag, phase_ext, mag_ext, n_bins;
     float qmin, qmax;
+    int mag_prep;  // PF_MAG_*
 };
 
 // --- Launch tuning -----------------------------------------------------------
@@ -294,6 +314,8 @@ std::string default_tune_path(const std::string& arch) {
 }
 
 int check_params(const Params& p, size_t lds_max) {
+    if (p.mag_prep != PF_MAG_AS_IS && p.mag_prep != PF_MAG_CENTER && p.mag_prep != PF_MAG_NORMALIZE)
+        return fail(PF_E_ARG, "mag_prep must be PF_MAG_AS_IS, PF_MAG_CENTER or PF_MAG_NORMALIZE");
     switch (p.algo) {
         case PF_ALGO_CE:
             if (p.n_phase < 1 || p.n_mag < 1 || p.phase_ext < 1 || p.mag_ext < 1)
@@ -387,27 +409,139 @@ int validate_offsets(int64_t n_curves, const int64_t* off, int64_t* npts, int64_
     return PF_OK;
 }
 
+// Host per-curve arrays -> one device CSR array, through the context's two
+// pinned staging chunks: a chunk is filled on the host and its copy issued
+// on the stream, then the other chunk is filled while that copy runs; a
+// chunk is refilled only once its previous copy has completed (its event).
+constexpr size_t kInChunk = size_t(4) << 20;
+
+int stage_init(pf_ctx* ctx) {
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->in_stage[i] &&
+            hipHostMalloc(&ctx->in_stage[i], kInChunk, hipHostMallocDefault) != hipSuccess) {
+            ctx->in_stage[i] = nullptr;
+            return fail(PF_E_NOMEM, "hipHostMalloc(%zu) failed", kInChunk);
+        }
+        if (!ctx->in_ev[i]) PF_HIP(hipEventCreateWithFlags(&ctx->in_ev[i], hipEventDisableTiming));
+    }
+    return PF_OK;
+}
+
+struct Stager {
+    pf_ctx* ctx;
+    int slot = 0;
+    size_t fill = 0;  // bytes in the current chunk
+    char* dst = nullptr;
+    size_t pos = 0;  // bytes of dst already issued
+
+    void begin(void* d) {
+        dst = static_cast<char*>(d);
+        pos = 0;
+    }
+    int wait(int i) {
+        if (ctx->in_busy[i]) {
+            PF_HIP(hipEventSynchronize(ctx->in_ev[i]));
+            ctx->in_busy[i] = false;
+        }
+        return PF_OK;
+    }
+    int flush() {
+        if (!fill) return PF_OK;
+        PF_HIP(hipMemcpyAsync(dst + pos, ctx->in_stage[slot], fill, hipMemcpyHostToDevice,
+                              ctx->stream));
+        PF_HIP(hipEventRecord(ctx->in_ev[slot], ctx->stream));
+        ctx->in_busy[slot] = true;
+        pos += fill;
+        fill = 0;
+        slot ^= 1;
+        return wait(slot);
+    }
+    int put(const void* src, size_t bytes) {
+        const char* p = static_cast<const char*>(src);
+        while (bytes) {
+            const size_t take = bytes < kInChunk - fill ? bytes : kInChunk - fill;
+            std::memcpy(static_cast<char*>(ctx->in_stage[slot]) + fill, p, take);
+            fill += take;
+            p += take;
+            bytes -= take;
+            if (fill == kInChunk) PF_TRY(flush());
+        }
+        return PF_OK;
+    }
+    int drain() {
+        PF_TRY(wait(0));
+        return wait(1);
+    }
+};
+
+// Shape of a set from its offsets (host); drops everything derived.
+void set_layout(CurveSet& cs, int64_t n_curves, const int64_t* off, int64_t npts, int64_t mx,
+                bool has_err) {
+    cs.invalidate();
+    cs.n_curves = n_curves;
+    cs.npts = npts;
+    cs.max_len = mx;
+    cs.off_h.assign(off, off + n_curves + 1);
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t c = 0; c < n_curves; ++c) {
+        const double len = static_cast<double>(off[c + 1] - off[c]);
+        s1 += len;
+        s2 += len * len;
+    }
+    cs.rep_len = s1 > 0.0 ? static_cast<int64_t>(s2 / s1) : 0;
+    cs.has_err = has_err;
+}
+
+// Per-curve host arrays (t[i], y[i], err[i] of lengths[i]) into cs.
+int upload_lists(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const float* const* t,
+                 const float* const* y, const float* const* e, const int64_t* len) {
+    if (n_curves < 0) return fail(PF_E_ARG, "n_curves < 0");
+    if (n_curves > 0 && (!len || !t || !y)) return fail(PF_E_ARG, "t, y and lengths must not be NULL");
+    std::vector<int64_t> off(static_cast<size_t>(n_curves) + 1, 0);
+    int64_t mx = 0;
+    for (int64_t c = 0; c < n_curves; ++c) {
+        const int64_t L = len[c];
+        if (L < 0) return fail(PF_E_ARG, "lengths[%lld] < 0", static_cast<long long>(c));
+        if (L > 0 && (!t[c] || !y[c] || (e && !e[c])))
+            return fail(PF_E_ARG, "curve %lld: array pointer is NULL", static_cast<long long>(c));
+        off[c + 1] = off[c] + L;
+        if (L > mx) mx = L;
+    }
+    const int64_t npts = off[n_curves];
+    set_layout(cs, n_curves, off.data(), npts, mx, e != nullptr);
+    const size_t nb = static_cast<size_t>(npts) * sizeof(float);
+    PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
+    PF_TRY(ensure(cs.t, nb));
+    PF_TRY(ensure(cs.y, nb));
+    if (e) PF_TRY(ensure(cs.e, nb));
+    PF_TRY(stage_init(ctx));
+    Stager st{ctx};
+    int rc = PF_OK;
+    auto copy = [&](void* dst, const float* const* src) {
+        st.begin(dst);
+        for (int64_t c = 0; c < n_curves && rc == PF_OK; ++c)
+            if (len[c] > 0) rc = st.put(src[c], static_cast<size_t>(len[c]) * sizeof(float));
+        if (rc == PF_OK) rc = st.flush();
+    };
+    st.begin(cs.off.p);
+    rc = st.put(off.data(), (n_curves + 1) * sizeof(int64_t));
+    if (rc == PF_OK) rc = st.flush();
+    if (rc == PF_OK) copy(cs.t.p, t);
+    if (rc == PF_OK) copy(cs.y.p, y);
+    if (rc == PF_OK && e) copy(cs.e.p, e);
+    const int rd = st.drain();
+    if (rc == PF_OK) rc = rd;
+    if (rc != PF_OK) set_layout(cs, 0, off.data(), 0, 0, false);  // an empty, consistent set
+    return rc;
+}
+
 // Copy host CSR arrays into cs (device), invalidating derived arrays.
 int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* off,
                   const float* t, const float* y, const float* e) {
     int64_t npts = 0, mx = 0;
     PF_TRY(validate_offsets(n_curves, off, &npts, &mx));
     if (npts > 0 && (!t || !y)) return fail(PF_E_ARG, "t and y must not be NULL");
-    cs.invalidate();
-    cs.n_curves = n_curves;
-    cs.npts = npts;
-    cs.max_len = mx;
-    cs.off_h.assign(off, off + n_curves + 1);
-    {
-        double s1 = 0.0, s2 = 0.0;
-        for (int64_t c = 0; c < n_curves; ++c) {
-            const double len = static_cast<double>(off[c + 1] - off[c]);
-            s1 += len;
-            s2 += len * len;
-        }
-        cs.rep_len = s1 > 0.0 ? static_cast<int64_t>(s2 / s1) : 0;
-    }
-    cs.has_err = e != nullptr;
+    set_layout(cs, n_curves, off, npts, mx, e != nullptr);
     const size_t nb = static_cast<size_t>(npts) * sizeof(float);
     PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
     PF_TRY(ensure(cs.t, nb));
@@ -425,13 +559,31 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
     return PF_OK;
 }
 
+// The magnitudes a call evaluates: as uploaded, or prepared on the device
+// (prepare() must have run for p).
+const float* mags(CurveSet& cs, const Params& p) {
+    return p.mag_prep == PF_MAG_AS_IS ? as<float>(cs.y) : as<float>(cs.yp);
+}
+
 // Build (or reuse) the derived arrays params.algo needs.
 int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
     hipStream_t s = ctx->stream;
     const int64_t n = cs.npts;
-    const float* y = as<float>(cs.y);
     const float* e = cs.has_err ? as<float>(cs.e) : nullptr;
     const int64_t* off = as<int64_t>(cs.off);
+    // Magnitudes in the requested preparation; what was derived from another
+    // preparation is rebuilt.
+    if (p.mag_prep != PF_MAG_AS_IS && cs.yp_mode != p.mag_prep) {
+        PF_TRY(ensure(cs.yp, n * sizeof(float)));
+        PF_HIP(pf::launch_prep_mags(off, as<float>(cs.y), as<float>(cs.yp), cs.n_curves,
+                                    p.mag_prep, s));
+        cs.yp_mode = p.mag_prep;
+    }
+    if (cs.derived_mode != p.mag_prep) {
+        cs.invalidate_mags();
+        cs.derived_mode = p.mag_prep;
+    }
+    const float* y = mags(cs, p);
     if (!cs.t64_ok || (has_pdt && !cs.tt_ok)) {
         PF_TRY(ensure(cs.t64, n * sizeof(double)));
         if (has_pdt) PF_TRY(ensure(cs.tt, n * sizeof(double)));
@@ -535,13 +687,13 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             break;
         }
         case PF_ALGO_AOV: {
-            pf::AovArgs a{as<float>(cs.y), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
+            pf::AovArgs a{mags(cs, p), as<float>(cs.aov_yy), as<float>(cs.aov_avg),
                           p.n_phase, p.phase_ext};
             le = pf::launch_aov(g, a, pf::lds_aov(p.n_phase), group, q, s);
             break;
         }
         case PF_ALGO_LS: {
-            pf::LsArgs a{as<float>(cs.y), std::getenv("PF_LS_PRECISE") ? 0 : 1};
+            pf::LsArgs a{mags(cs, p), std::getenv("PF_LS_PRECISE") ? 0 : 1};
             le = pf::launch_ls(g, a, q, s);
             break;
         }
@@ -653,6 +805,7 @@ Params to_params(const pf_params* p) {
     q.n_bins = p->n_bins;
     q.qmin = p->qmin;
     q.qmax = p->qmax;
+    q.mag_prep = p->mag_prep;
     return q;
 }
 
@@ -822,6 +975,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         static_cast<uint64_t>(p.mag_ext), static_cast<uint64_t>(p.n_bins),
         static_cast<uint64_t>(__builtin_bit_cast(uint32_t, p.qmin)),
         static_cast<uint64_t>(__builtin_bit_cast(uint32_t, p.qmax)),
+        static_cast<uint64_t>(p.mag_prep), reinterpret_cast<uint64_t>(mags(cs, p)),
         static_cast<uint64_t>(np), static_cast<uint64_t>(nd), static_cast<uint64_t>(n_peaks),
         static_cast<uint64_t>(min_distance), static_cast<uint64_t>(ctx->has_pdt),
         reinterpret_cast<uint64_t>(ctx->periods.p), reinterpret_cast<uint64_t>(ctx->pdts.p),
@@ -1083,6 +1237,10 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->in_stage[i]) (void)hipHostFree(ctx->in_stage[i]);
+        if (ctx->in_ev[i]) (void)hipEventDestroy(ctx->in_ev[i]);
+    }
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1197,6 +1355,15 @@ int pf_curves_create(pf_ctx* ctx, pf_curves** out, int64_t n_curves, const int64
     }
     *out = cv;
     return PF_OK;
+}
+
+int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float* const* t,
+                   const float* const* y, const float* const* err, const int64_t* lengths) {
+    if (!ctx || !cv) return fail(PF_E_ARG, "ctx/cv is NULL");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "the curve set belongs to another context");
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_lists(ctx, cv->cs, n_curves, t, y, err, lengths));
+    return finish(ctx);
 }
 
 void pf_curves_destroy(pf_curves* cv) {

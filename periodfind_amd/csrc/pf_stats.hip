@@ -18,6 +18,7 @@
 // containing NaN is flagged so the caller can defer to numpy.
 #include "pf_device.hpp"
 #include "pf_kernels.hpp"
+#include "pf_numpy.hpp"
 
 namespace pf {
 
@@ -35,117 +36,14 @@ __device__ __forceinline__ float key_value(uint32_t k) {
     return __uint_as_float(u);
 }
 
-// numpy's pairwise summation of x[0..n), n <= 8192 (one numpy block).  The
-// split tree (halves rounded down to a multiple of 8, leaves of <= 128
-// elements) is enumerated once; leaf sums run in parallel, one thread per
-// leaf; thread 0 then combines them in the tree's order.
-constexpr int kMaxLeaves = 72;  // the tree of n <= 8192 has at most 65 leaves
+using npsum::LeafPlan;
+using npsum::plan_leaves;
 
-struct LeafPlan {
-    int start[kMaxLeaves];
-    int len[kMaxLeaves];
-    float sum[kMaxLeaves];
-    int count;
-};
-
-// Left-to-right leaves of the split tree of [0, n) (thread 0).  The explicit
-// stack lives in LDS: a dynamically indexed private array would be placed in
-// scratch memory, one global-memory round trip per push and pop.
-__device__ void plan_leaves(int n, LeafPlan& L) {
-    __shared__ int st_s[16], st_n[16];
-    int sp = 0;
-    st_s[0] = 0;
-    st_n[0] = n;
-    sp = 1;
-    int count = 0;
-    while (sp > 0) {
-        --sp;
-        const int s0 = st_s[sp], m = st_n[sp];
-        if (m <= 128) {
-            L.start[count] = s0;
-            L.len[count] = m;
-            ++count;
-            continue;
-        }
-        int n2 = m / 2;
-        n2 -= n2 % 8;
-        // push right first so the left half is visited first
-        st_s[sp] = s0 + n2;
-        st_n[sp] = m - n2;
-        ++sp;
-        st_s[sp] = s0;
-        st_n[sp] = n2;
-        ++sp;
-    }
-    L.count = count;
-}
-
-// numpy's leaf: < 8 elements serial; else 8 accumulators, combined
-// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the remainder serially.
-__device__ float leaf_sum(const float* x, int n) {
-    if (n < 8) {
-        float res = -0.0f;
-        for (int i = 0; i < n; ++i) res += x[i];
-        return res;
-    }
-    float r[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = x[j];
-    int i = 8;
-    for (; i < n - (n % 8); i += 8) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) r[j] += x[i + j];
-    }
-    float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-    for (; i < n; ++i) res += x[i];
-    return res;
-}
-
-// Post-order combination of the leaf sums (thread 0; stack in LDS, as above).
-__device__ float combine_leaves(int n, const LeafPlan& L) {
-    __shared__ int st_n[16], st_state[16];
-    __shared__ float st_left[16];
-    int sp = 0;
-    st_n[0] = n;
-    st_state[0] = 0;
-    int next = 0;
-    float ret = 0.0f;
-    while (true) {
-        const int fn = st_n[sp];
-        if (fn <= 128) {
-            ret = L.sum[next++];
-            if (sp == 0) return ret;
-            --sp;
-            continue;
-        }
-        int n2 = fn / 2;
-        n2 -= n2 % 8;
-        const int state = st_state[sp];
-        if (state == 0) {
-            st_state[sp] = 1;
-            st_n[sp + 1] = n2;
-            st_state[sp + 1] = 0;
-            ++sp;
-        } else if (state == 1) {
-            st_left[sp] = ret;
-            st_state[sp] = 2;
-            st_n[sp + 1] = fn - n2;
-            st_state[sp + 1] = 0;
-            ++sp;
-        } else {
-            ret = st_left[sp] + ret;
-            if (sp == 0) return ret;
-            --sp;
-        }
-    }
-}
-
-// np.add.reduce of x[0..n) (n <= 8192) by the whole block; result in *out.
-// L must hold the plan of n (plan_leaves, then a barrier).
+// np.add.reduce of x[0..n), n <= 8192 (one numpy block), by the whole block;
+// result in *out.  L must hold the plan of n (plan_leaves, then a barrier).
 __device__ void block_numpy_sum(const float* x, int n, LeafPlan& L, float* out) {
-    for (int i = threadIdx.x; i < L.count; i += kThreads) L.sum[i] = leaf_sum(x + L.start[i], L.len[i]);
-    __syncthreads();
-    if (threadIdx.x == 0) *out = 0.0f + combine_leaves(n, L);
+    const float s = npsum::block_pairwise(x, n, L);
+    if (threadIdx.x == 0) *out = 0.0f + s;
     __syncthreads();
 }
 

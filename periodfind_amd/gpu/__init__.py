@@ -6,9 +6,12 @@ classes' superset signature (periodfind/cpu/__init__.py:73-672): besides
 output='stats' | 'periodogram', output='peaks' returns the top `n_peaks`
 local extrema per curve (peaks.rs semantics), selected on the device.
 
-Each calc() validates inputs with the reference's messages, prepares
-magnitudes as the reference does (CE normalizes, LS centers), CSR-packs the
-curves and calls the HIP library through the Cython binding.
+Each calc() validates inputs with the reference's messages and hands the
+per-curve arrays to the HIP library through the Cython binding
+(pf_curves_load: packed into pinned staging, packing overlapped with the
+transfer).  The magnitude preparation the reference runs on the host
+(CE normalizes, LS centers; periodfind/_utils.py:12-47) runs on the device,
+in numpy float32 arithmetic bit for bit.
 """
 
 import threading
@@ -20,7 +23,8 @@ from periodfind_amd import _hipruntime
 _hipruntime.preload()
 
 from periodfind_amd import _pfhip  # noqa: E402  (after the runtime preload)
-from periodfind_amd._utils import ensure_float32, prepare_magnitudes, validate_errs, validate_inputs
+from periodfind_amd._utils import (ensure_float32, magnitude_mode, prepare_magnitudes,
+                                   validate_errs, validate_inputs)
 from periodfind_amd.results import Periodogram, Statistics
 
 N_PEAKS_DEFAULT = 32
@@ -87,58 +91,53 @@ class _Algorithm:
     def _params(self):
         return {}
 
-    def _run(self, times, mags_use, errs, periods, period_dts, output, n_stats,
+    def _run(self, times, mags, errs, mag_mode, periods, period_dts, output, n_stats,
              significance_type, n_peaks, min_distance):
-        P, D = _grid_arrays(periods, period_dts)
-        off, t = pack(times)
-        _, y = pack(mags_use)
-        e = pack(errs)[1] if errs is not None else None
-        ctx = context(self.device)
-        if output == "peaks":
-            idx, val = ctx.peaks(self._algo, self._params(), off, t, y, e, P, D, int(n_peaks),
-                                 int(min_distance))
-            return _unravel_peaks(idx, val, periods, period_dts)
-        if output not in ("stats", "periodogram"):
+        """Load the curves into the context's set, then evaluate it; `mag_mode`
+        ("center", "normalize" or None) is applied on the device."""
+        if output not in ("stats", "periodogram", "peaks"):
             raise NotImplementedError(
                 f'Output type "{output}" is not implemented. '
                 f'Use "stats", "periodogram", or "peaks".')
+        P, D = _grid_arrays(periods, period_dts)
+        ctx = context(self.device)
+        prm = dict(self._params(), mag_prep=mag_mode)
         params = [periods, period_dts]
         T = P.size * D.size
-        if output == "stats" and 1 <= n_stats <= T <= DEVICE_STATS_MAX_T and len(times) > 0:
-            return self._device_stats(ctx, off, t, y, e, P, D, params, n_stats,
-                                      significance_type, times, mags_use, errs)
-        out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
-        ctx.grid(self._algo, self._params(), off, t, y, e, P, D, out)
+        with ctx.lock:  # the load and the evaluation of it, as one unit
+            ctx.load(times, mags, errs)
+            if output == "peaks":
+                idx, val = ctx.set_peaks(self._algo, prm, P, D, int(n_peaks), int(min_distance))
+                return _unravel_peaks(idx, val, periods, period_dts)
+            if output == "stats" and 1 <= n_stats <= T <= DEVICE_STATS_MAX_T and len(times) > 0:
+                return self._device_stats(ctx, prm, P, D, params, n_stats, significance_type,
+                                          len(times))
+            out = ctx.set_grid(self._algo, prm, P, D)
         if output == "stats":
             return [Statistics.statistics_from_data(out[i], params, self._use_max, n=n_stats,
                                                     significance_type=significance_type)
                     for i in range(len(times))]
         return [Periodogram(d, params, self._use_max) for d in out]
 
-
-    def _device_stats(self, ctx, off, t, y, e, P, D, params, n_stats, significance_type,
-                      times, mags_use, errs):
+    def _device_stats(self, ctx, prm, P, D, params, n_stats, significance_type, n_curves):
         """'stats' computed on the device (numpy float32 semantics); curves whose
         periodogram holds NaN are summarised by numpy from the full grid."""
         try:
-            summ, idx, val, fb = ctx.stats(self._algo, self._params(), off, t, y, e, P, D,
-                                           int(n_stats))
+            summ, idx, val, fb = ctx.set_stats(self._algo, prm, P, D, int(n_stats))
         except RuntimeError as exc:
             if "(code -5)" not in str(exc):  # PF_E_LIMIT: grid too large to stage in LDS
                 raise
-            out = np.zeros((len(times), P.size, D.size), dtype=np.float32)
-            ctx.grid(self._algo, self._params(), off, t, y, e, P, D, out)
+            out = ctx.set_grid(self._algo, prm, P, D)
             return [Statistics.statistics_from_data(out[i], params, self._use_max, n=n_stats,
                                                     significance_type=significance_type)
-                    for i in range(len(times))]
+                    for i in range(n_curves)]
         nd = D.size
         grid = None
         res = []
-        for c in range(len(times)):
+        for c in range(n_curves):
             if fb[c]:
                 if grid is None:
-                    grid = np.zeros((len(times), P.size, D.size), dtype=np.float32)
-                    ctx.grid(self._algo, self._params(), off, t, y, e, P, D, grid)
+                    grid = ctx.set_grid(self._algo, prm, P, D)
                 res.append(Statistics.statistics_from_data(
                     grid[c], params, self._use_max, n=n_stats, significance_type=significance_type))
                 continue
@@ -175,38 +174,10 @@ class ConditionalEntropy(_Algorithm):
         validate_inputs(times, mags)
         ensure_float32(times, "times")
         ensure_float32(mags, "mags")
-        mags_use = prepare_magnitudes(mags, center, normalize)
-        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+        mode = magnitude_mode(mags, center, normalize)
+        return self._run(times, mags, None, mode, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
-    def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
-                     normalize=False, center=True, rtol=1e-13, max_iter=60):
-        """Top `n_peaks` LS maxima per curve over the frequency grid `freqs`
-        (cycles per time unit, monotone), each refined to sub-grid precision.
-
-        The grid is the float32 periodogram at periods 1/f (period_dts [0]);
-        each peak k is then refined in FP64 by safeguarded Newton on dP/dw
-        inside [f(k-1), f(k+1)].  Returns a list (per curve) of dicts with
-        arrays 'grid_index', 'grid_power' (float32), 'frequency', 'period',
-        'power' (FP64 power at the refined frequency) and 'status' (0 refined,
-        1 no bracketed maximum: left at the grid frequency, 2 not converged,
-        3 no peak in this slot)."""
-        validate_inputs(times, mags)
-        ensure_float32(times, "times")
-        ensure_float32(mags, "mags")
-        mags_use = prepare_magnitudes(mags, center, normalize)
-        off, t = pack(times)
-        _, y = pack(mags_use)
-        F = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
-        idx, val, fr, pw, st = context(self.device).ls_peaks_refined(
-            off, t, y, F, int(n_peaks), int(min_distance), float(rtol), int(max_iter))
-        out = []
-        for c in range(idx.shape[0]):
-            with np.errstate(divide="ignore"):
-                period = np.where(fr[c] > 0, 1.0 / np.where(fr[c] > 0, fr[c], 1.0), 0.0)
-            out.append({"grid_index": idx[c], "grid_power": val[c], "frequency": fr[c],
-                        "period": period, "power": pw[c], "status": st[c]})
-        return out
 
 
 class AOV(_Algorithm):
@@ -228,38 +199,10 @@ class AOV(_Algorithm):
         validate_inputs(times, mags)
         ensure_float32(times, "times")
         ensure_float32(mags, "mags")
-        mags_use = prepare_magnitudes(mags, center, normalize)
-        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+        mode = magnitude_mode(mags, center, normalize)
+        return self._run(times, mags, None, mode, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
-    def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
-                     normalize=False, center=True, rtol=1e-13, max_iter=60):
-        """Top `n_peaks` LS maxima per curve over the frequency grid `freqs`
-        (cycles per time unit, monotone), each refined to sub-grid precision.
-
-        The grid is the float32 periodogram at periods 1/f (period_dts [0]);
-        each peak k is then refined in FP64 by safeguarded Newton on dP/dw
-        inside [f(k-1), f(k+1)].  Returns a list (per curve) of dicts with
-        arrays 'grid_index', 'grid_power' (float32), 'frequency', 'period',
-        'power' (FP64 power at the refined frequency) and 'status' (0 refined,
-        1 no bracketed maximum: left at the grid frequency, 2 not converged,
-        3 no peak in this slot)."""
-        validate_inputs(times, mags)
-        ensure_float32(times, "times")
-        ensure_float32(mags, "mags")
-        mags_use = prepare_magnitudes(mags, center, normalize)
-        off, t = pack(times)
-        _, y = pack(mags_use)
-        F = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
-        idx, val, fr, pw, st = context(self.device).ls_peaks_refined(
-            off, t, y, F, int(n_peaks), int(min_distance), float(rtol), int(max_iter))
-        out = []
-        for c in range(idx.shape[0]):
-            with np.errstate(divide="ignore"):
-                period = np.where(fr[c] > 0, 1.0 / np.where(fr[c] > 0, fr[c], 1.0), 0.0)
-            out.append({"grid_index": idx[c], "grid_power": val[c], "frequency": fr[c],
-                        "period": period, "power": pw[c], "status": st[c]})
-        return out
 
 
 class LombScargle(_Algorithm):
@@ -277,8 +220,8 @@ class LombScargle(_Algorithm):
         validate_inputs(times, mags)
         ensure_float32(times, "times")
         ensure_float32(mags, "mags")
-        mags_use = prepare_magnitudes(mags, center, normalize)
-        return self._run(times, mags_use, None, periods, period_dts, output, n_stats,
+        mode = magnitude_mode(mags, center, normalize)
+        return self._run(times, mags, None, mode, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
     def refine_peaks(self, times, mags, freqs, n_peaks=8, min_distance=MIN_DISTANCE_DEFAULT,
@@ -331,7 +274,7 @@ class FPW(_Algorithm):
         ensure_float32(times, "times")
         ensure_float32(mags, "mags")
         errs = validate_errs(errs, times)
-        return self._run(times, mags, errs, periods, period_dts, output, n_stats,
+        return self._run(times, mags, errs, None, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
 
@@ -357,7 +300,7 @@ class BoxLeastSquares(_Algorithm):
         ensure_float32(times, "times")
         ensure_float32(mags, "mags")
         errs = validate_errs(errs, times)
-        return self._run(times, mags, errs, periods, period_dts, output, n_stats,
+        return self._run(times, mags, errs, None, periods, period_dts, output, n_stats,
                          significance_type, n_peaks, min_distance)
 
 

@@ -12,11 +12,29 @@ import pytest
 
 import periodfind_amd
 from periodfind_amd import Periodogram, Statistics
-from periodfind_amd._utils import (ensure_float32, prepare_magnitudes, validate_errs,
-                                   validate_inputs)
+from periodfind_amd._utils import (ensure_float32, magnitude_mode, prepare_magnitudes,
+                                   validate_errs, validate_inputs)
 
 
 class TestUtils:
+    def test_magnitude_mode_mirrors_prepare_magnitudes(self):
+        # The drop-in classes prepare on the device; what the host still does
+        # (mode choice, warning, numpy's errors on empty curves) must match.
+        one = [np.ones(3, np.float32)]
+        assert magnitude_mode(one, False, False) is None
+        assert magnitude_mode(one, False, True) == "normalize"
+        assert magnitude_mode(one, True, False) == "center"
+        with pytest.warns(RuntimeWarning, match="conflicting settings"):
+            assert magnitude_mode(one, True, True) == "center"
+        empty = [np.ones(3, np.float32), np.zeros(0, np.float32)]
+        with pytest.raises(ValueError) as want:
+            prepare_magnitudes(empty, False, True)
+        with pytest.raises(ValueError) as got:
+            magnitude_mode(empty, False, True)
+        assert str(got.value) == str(want.value)
+        with pytest.warns(RuntimeWarning, match="Mean of empty slice"):
+            magnitude_mode(empty, True, False)
+
     def test_normalize_range_and_formula(self):
         m = [np.array([1.0, 2.0, 3.0], dtype=np.float32)]
         r = prepare_magnitudes(m, center=False, normalize=True)[0]
