@@ -39,11 +39,20 @@ PARAMS = {
     "ls": dict(),
     "fpw": dict(n_bins=10),
     "bls": dict(n_bins=50, qmin=0.01, qmax=0.5),
+    "mf": dict(n_bins=20),
 }
 # Algorithmic flops per point-period evaluation (inner loop) and per trial
 # (epilogue): fold 6, bin/accumulate as counted per statistic.
-FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0}
-FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0}
+FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0,
+               # MF: fold 6, bin 3, three bin sums 3, the R^2 pass 4 (sub, mul, mul, add)
+               "mf": 16.0}
+FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0,
+               # MF(20): 14 templates x 20 shifts x 20 bins (mul + add), 280 divisions,
+               # ~12 per bin for the profile, R^2 and RMS terms
+               "mf": 2.0 * 14 * 20 * 20 + 14 * 20 + 12 * 20}
+# Fold-based statistics beside the headline five (measured by --workload siblings).
+SIBLINGS = ("mf",)
+ERR_ALGOS = ("fpw", "bls", "mf")
 PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
 # Published 1x P100 pts/sec at 8192 points (benchmarks/throughput_results_1gpu.csv:49-81).
 P100_8192 = {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510}
@@ -96,7 +105,7 @@ class Resident:
         self.h = ctypes.c_void_p()
         off, t = capi.pack(times)
         _, y = capi.pack(prep_mags(algo, mags))
-        e = capi.pack(errs)[1] if algo in ("fpw", "bls") else None
+        e = capi.pack(errs)[1] if algo in ERR_ALGOS else None
         capi.check(capi.lib().pf_curves_create(ctx.h, ctypes.byref(self.h), len(times), off, t,
                                                y, e))
         self.prm = capi.make_params(algo, **PARAMS[algo])
@@ -479,6 +488,87 @@ def run_heavy(args):
     print(json.dumps(out), flush=True)
 
 
+def run_siblings(args):
+    """The fold-based statistics beside the headline five (Matched Filter),
+    at the headline shape: 100 curves x {1024, 4096, 8192} points, 1000
+    periods, output 'stats', resident inputs; GPU kernel roofline and the
+    oracle as CPU baseline on a bounded sample."""
+    from periodfind_amd import capi, synth
+    from oracle import oracle as O
+
+    datasets = {n: synth.config2(n, N_CURVES) for n in SIZES}
+    ctx = capi.Context(0)
+    sets = {}
+    for n in SIZES:
+        t, m, e, P, D = datasets[n]
+        for algo in SIBLINGS:
+            sets[(algo, n)] = Resident(ctx, algo, t, m, e, P, D)
+    print("bench: siblings warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        for r in sets.values():
+            r.step()
+    kms = {k: [] for k in sets}
+    walls = {k: [] for k in sets}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for k, r in sets.items():
+            tc = time.perf_counter()
+            kms[k].append(r.step())
+            walls[k].append(time.perf_counter() - tc)
+    wall = time.perf_counter() - t0
+    pts_per_step = sum(N_CURVES * n for (_, n) in sets)
+    detail = {}
+    best = None
+    for (algo, n), v in kms.items():
+        kt = float(np.mean(v)) * 1e-3
+        evals = N_CURVES * n * 1000
+        ach = evals * (FLOPS_INNER[algo] + FLOPS_TRIAL[algo] / n) / kt / 1e12
+        detail[f"{algo}@{n}"] = {"kernel_ms": round(kt * 1e3, 4),
+                                 "call_ms": round(float(np.median(walls[(algo, n)])) * 1e3, 4),
+                                 "evals_per_s": evals / kt, "tflops": round(ach, 3),
+                                 "frac": round(ach / PEAK_VALU_TFLOPS, 4)}
+        if best is None or kt > best[1]:
+            best = (f"{algo}@{n}", kt, ach)
+    for r in sets.values():
+        r.close()
+    ctx.close()
+    out = {"metric": "pts/sec @1000 periods per algo", "value": pts_per_step / (wall / args.steps),
+           "unit": "pts/sec", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 (fold in f64)", "data": "synthetic",
+           "config": {"workload": "siblings: " + ", ".join(SIBLINGS) + " x 100 curves x "
+                                  "{1024,4096,8192} points, 1000 periods, output=stats",
+                      "params": {a: PARAMS[a] for a in SIBLINGS}},
+           "roofline": {"bound": "valu", "kernel": best[0], "achieved": round(best[2], 3),
+                        "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(best[2] / PEAK_VALU_TFLOPS, 4), "traffic": None},
+           "kernels": detail}
+    if args.cpu_seconds > 0:
+        print("bench: siblings CPU baseline", file=sys.stderr, flush=True)
+        threads = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS")
+        if omp and omp.isdigit():
+            threads = min(threads, int(omp))
+        t, m, e, P, D = datasets[8192]
+        cpu = {}
+        for algo in SIBLINGS:
+            ta = time.perf_counter()
+            O.calc_grid(algo, t[:1], m[:1], P, D, errs=e[:1], threads=threads, **PARAMS[algo])
+            per_curve = max(time.perf_counter() - ta, 1e-6)
+            k = int(max(1, min(N_CURVES, args.cpu_seconds / len(SIBLINGS) / per_curve)))
+            ta = time.perf_counter()
+            O.calc_grid(algo, t[:k], m[:k], P, D, errs=e[:k], threads=threads, **PARAMS[algo])
+            dt = time.perf_counter() - ta
+            cpu[algo] = {"pts_per_s_8192": k * 8192 / dt, "curves": k, "seconds": round(dt, 2)}
+        out["cpu_baseline"] = {"kind": "port", "cores": threads, "unit": "pts/sec @1000 periods",
+                               "per_algo_8192": cpu,
+                               "sample": "first k of the 100 curves at 8192 points, 1000 periods"}
+        out["gpu_over_cpu_8192"] = {
+            a: round(N_CURVES * 8192 / (detail[f"{a}@8192"]["call_ms"] * 1e-3)
+                     / cpu[a]["pts_per_s_8192"], 1) for a in SIBLINGS}
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of `nbytes` (torch), the reference kernel for
     scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -500,10 +590,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU-baseline budget (0 disables)")
     ap.add_argument("--no-drop-in", action="store_true")
-    ap.add_argument("--workload", default="batch", choices=["batch", "ragged", "lsrefine", "heavy"],
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "ragged", "lsrefine", "heavy", "siblings"],
                     help="batch: the headline 5-statistic workload; ragged: 10k ragged curves; "
                          "lsrefine: differentiable LS peak refinement; heavy: 1000 x 65536 "
-                         "points, 1e5 periods")
+                         "points, 1e5 periods; siblings: the fold-based statistics beside the "
+                         "five (Matched Filter) at the batch shape")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (profiler counter calibration)")
     ap.add_argument("--oracle-curves", type=int, default=100,
@@ -522,6 +614,8 @@ def main():
         return run_lsrefine(args)
     if args.workload == "heavy":
         return run_heavy(args)
+    if args.workload == "siblings":
+        return run_siblings(args)
 
     from periodfind_amd import capi, synth
 

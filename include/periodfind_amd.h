@@ -144,6 +144,22 @@ int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const flo
                 const float* period_dts, int64_t n_pdts, int n_bins, float qmin, float qmax,
                 float* out);
 
+/* MatchedFilter(num_bins).CalcMFBatched — cuda/mf.h, periodfind/mf.pyx:19-31;
+ * semantics of rust/src/mf.rs:344-393 (calc_mf): per trial, the combined
+ * score max_corr * R^2 * coverage of the phase-binned profile against the
+ * 14 shape templates.  err == NULL means all ones.  Higher is better. */
+int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* err, const float* periods, int64_t n_periods,
+               const float* period_dts, int64_t n_pdts, int num_bins, float* out);
+
+/* calc_mf_features_batched (rust/src/lib.rs:866-900, mf.rs:404-451): each
+ * curve at its own period (periods[n_curves], period_dt 0) ->
+ * out[n_curves][7] = best_sawtooth, best_sinusoidal, best_eclipsing, R^2,
+ * amp_snr, n_filled, combined.  err == NULL means all ones. */
+int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* err, const float* periods, int num_bins,
+                   float* out);
+
 /* ------------------------------------------------------------------------ */
 /* Device top-N peaks.  Replaces the CPU-only fused entry points             */
 /* calc_{ce,aov,ls,fpw,bls}_peaks_batched (rust/src/lib.rs:559-754) and the  */
@@ -154,7 +170,8 @@ int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const flo
 /* ------------------------------------------------------------------------ */
 
 /* Algorithm selector for the generic entry points below. */
-enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4 };
+enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4,
+       PF_ALGO_MF = 5 };
 
 /* Magnitude preparation applied on the device before evaluation
  * (pf_params.mag_prep): prepare_magnitudes of periodfind/_utils.py:12-47,
@@ -174,7 +191,7 @@ typedef struct pf_params {
     int n_mag;       /* CE magnitude bins */
     int phase_ext;   /* CE / AOV phase_bin_extent */
     int mag_ext;     /* CE mag_bin_extent */
-    int n_bins;      /* FPW / BLS bins */
+    int n_bins;      /* FPW / BLS bins; MF num_bins */
     float qmin;      /* BLS */
     float qmax;      /* BLS */
     int mag_prep;    /* PF_MAG_*; 0 = as given */

@@ -34,6 +34,15 @@ class _F64OrNull:
 
 
 _f64p_or_null = _F64OrNull
+
+
+class _F32OrNull:
+    @classmethod
+    def from_param(cls, obj):
+        return None if obj is None else _f32p.from_param(obj)
+
+
+_f32p_or_null = _F32OrNull
 _flt = ctypes.c_float
 
 
@@ -76,6 +85,15 @@ def lib():
         L.or_find_top_peaks_batched.argtypes = [_f32p, _i64, _i64, _i64, _i64, _int, _i64p,
                                                 _f32p]
         L.or_max_threads.restype = _int
+        L.or_mf_templates.restype = _int
+        L.or_mf_templates.argtypes = [_int, _f32p, ctypes.POINTER(ctypes.c_int)]
+        L.or_mf_circular_pearson_max.restype = _flt
+        L.or_mf_circular_pearson_max.argtypes = [_f32p, _f32p, _int]
+        L.or_mf_batched.restype = _int
+        L.or_mf_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64,
+                                    _f32p, _i64, _int, _f32p, _int]
+        L.or_mf_features.restype = _int
+        L.or_mf_features.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _int, _f32p, _int]
         L.or_lsd_tau_power.restype = ctypes.c_double
         L.or_lsd_tau_power.argtypes = [_f64p, _f64p, _i64, ctypes.c_double]
         L.or_lsd_jet.restype = None
@@ -157,6 +175,48 @@ def bls_grid(times, mags, errs, periods, period_dts, n_bins=50, qmin=0.01, qmax=
     return out
 
 
+def mf_grid(times, mags, errs, periods, period_dts, num_bins=20, threads=0):
+    """calc_mf_batched (rust/src/lib.rs:765-821, mf.rs:344-393): the combined
+    score max_corr * R^2 * coverage per trial; errs None = ones."""
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    e = pack(errs)[1] if errs is not None else None
+    rc = lib().or_mf_batched(len(times), off, t, m, e, P, P.size, D, D.size, num_bins,
+                             out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("num_bins must be >= 1")
+    return out
+
+
+def mf_features(times, mags, errs, periods, num_bins=20, threads=0):
+    """calc_mf_features_batched (lib.rs:866-900, mf.rs:404-451): one period
+    per curve -> float32[n_curves, 7] = [best_sawtooth, best_sinusoidal,
+    best_eclipsing, R2, amp_snr, n_filled, combined]."""
+    off, t = pack(times)
+    _, m = pack(mags)
+    _, e = pack(errs)
+    P = np.ascontiguousarray(periods, dtype=np.float32)
+    out = np.zeros((len(times), 7), np.float32)
+    rc = lib().or_mf_features(len(times), off, t, m, e, P, num_bins, out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("num_bins must be >= 1")
+    return out
+
+
+def mf_templates(num_bins=20):
+    """generate_templates (mf.rs:49-112) -> (shapes float32[14, nb], kinds int[14])."""
+    shapes = np.zeros((14, num_bins), np.float32)
+    kinds = (ctypes.c_int * 14)()
+    n = lib().or_mf_templates(num_bins, shapes.reshape(-1), kinds)
+    return shapes[:n], np.array(kinds[:n])
+
+
+def mf_circular_pearson_max(profile, template):
+    p = np.ascontiguousarray(profile, dtype=np.float32)
+    t = np.ascontiguousarray(template, dtype=np.float32)
+    return float(lib().or_mf_circular_pearson_max(p, t, p.size))
+
+
 def bls_duration_range(n_bins, qmin, qmax):
     lo, hi = ctypes.c_int64(), ctypes.c_int64()
     lib().or_bls_duration_range(n_bins, qmin, qmax, ctypes.byref(lo), ctypes.byref(hi))
@@ -185,7 +245,7 @@ def find_top_peaks_batched(data, n_peaks=32, min_distance=1, use_max=True):
     return idx, val
 
 
-USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True}
+USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True, "mf": True}
 
 
 def prepare_magnitudes(mags, center, normalize):
@@ -222,6 +282,10 @@ def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **pa
         return fpw_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
     if algo == "bls":
         return bls_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
+    if algo == "mf":  # cpu/__init__.py:675-811: errs default to ones, mags as given
+        if "n_bins" in params:
+            params["num_bins"] = params.pop("n_bins")
+        return mf_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
     raise ValueError(algo)
 
 

@@ -19,7 +19,7 @@ from periodfind_amd.results import Periodogram, Statistics
 
 __all__ = [
     "set_device", "get_device", "ConditionalEntropy", "AOV", "LombScargle", "FPW",
-    "BoxLeastSquares", "Statistics", "Periodogram",
+    "BoxLeastSquares", "MatchedFilter", "Statistics", "Periodogram",
 ]
 
 _default_device = None  # None: auto-detect
@@ -97,3 +97,8 @@ def FPW(**kwargs):
 def BoxLeastSquares(**kwargs):
     """Box Least Squares on the resolved device."""
     return _make("BoxLeastSquares", kwargs)
+
+
+def MatchedFilter(**kwargs):
+    """Matched Filter morphology score on the resolved device."""
+    return _make("MatchedFilter", kwargs)

@@ -58,6 +58,12 @@ cdef extern from "periodfind_amd.h":
                     const float* y, const float* err, const float* periods,
                     int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_bins,
                     float qmin, float qmax, float* out) nogil
+    int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* err, const float* periods, int64_t n_periods,
+                   const float* period_dts, int64_t n_pdts, int num_bins, float* out) nogil
+    int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                       const float* y, const float* err, const float* periods, int num_bins,
+                       float* out) nogil
     int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
                  const int64_t* offsets, const float* t, const float* y, const float* err,
                  const float* periods, int64_t n_periods, const float* period_dts,
@@ -95,7 +101,7 @@ cdef extern from "periodfind_amd.h":
                         int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
                         int* fallback) nogil
 
-ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
+ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5}
 MAG_PREP = {None: 0, "as_is": 0, "center": 1, "normalize": 2}
 
 
@@ -313,7 +319,7 @@ cdef class Context:
             i0, i1, i2, i3 = p["n_phase"], p["n_mag"], p["phase_ext"], p["mag_ext"]
         elif a == 1:
             i0, i1 = p["n_phase"], p["phase_ext"]
-        elif a == 3:
+        elif a == 3 or a == 5:
             i0 = p["n_bins"]
         elif a == 4:
             i0, f0, f1 = p["n_bins"], p["qmin"], p["qmax"]
@@ -327,6 +333,8 @@ cdef class Context:
                     rc = pf_ls_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, op)
                 elif a == 3:
                     rc = pf_fpw_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
+                elif a == 5:
+                    rc = pf_mf_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
                 else:
                     rc = pf_bls_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, f0, f1,
                                      op)
@@ -422,6 +430,25 @@ cdef class Context:
         if rc != 0:
             _raise(rc)
         return idx, val, fr, pw, st
+
+    def mf_features(self, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+                    cnp.ndarray periods, int num_bins):
+        """Matched Filter features, curve c at periods[c] -> float32[C, 7]."""
+        cdef int64_t C = off.shape[0] - 1
+        out = np.zeros((C, 7), dtype=np.float32)
+        cdef float* op = <float*> cnp.PyArray_DATA(out)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_mf_features(self.ctx, C, o, tp, yp, ep, pp, num_bins, op)
+        if rc != 0:
+            _raise(rc)
+        return out
 
     def find_top_peaks(self, cnp.ndarray data, int64_t n_peaks, int64_t min_distance,
                        bint use_max):

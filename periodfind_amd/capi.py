@@ -21,7 +21,7 @@ PF_E_HIP = -3
 PF_E_NOMEM = -4
 PF_E_LIMIT = -5
 
-PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4}
+PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5}
 
 # Every symbol the header declares, with (restype, argtypes).
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -88,6 +88,10 @@ SIGNATURES = {
                            _i64, _int, _f32p]),
     "pf_bls_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
                            _i64, _int, _flt, _flt, _f32p]),
+    "pf_mf_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
+                          _i64, _int, _f32p]),
+    "pf_mf_features": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _int,
+                              _f32p]),
     "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
                         _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
     "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
@@ -250,8 +254,22 @@ class Context:
         elif algo == "bls":
             check(L.pf_bls_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
                                 p.get("n_bins", 50), p.get("qmin", 0.01), p.get("qmax", 0.5), o))
+        elif algo == "mf":
+            check(L.pf_mf_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
+                               p.get("n_bins", 20), o))
         else:
             raise ValueError(algo)
+        return out
+
+    def mf_features(self, times, mags, errs, periods, num_bins=20):
+        """Matched Filter features at one period per curve -> float32[C, 7]."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        out = np.zeros((len(times), 7), dtype=np.float32)
+        check(lib().pf_mf_features(self.h, len(times), off, t, y, e, P, num_bins,
+                                   out.reshape(-1)))
         return out
 
     def peaks(self, algo, times, mags, periods, period_dts, errs=None, n_peaks=32,
@@ -327,7 +345,7 @@ def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=Non
     prm.phase_ext = ext if ext is not None else phase_ext
     prm.mag_ext = mag_ext
     if n_bins is None:
-        n_bins = 50 if algo == "bls" else 10
+        n_bins = {"bls": 50, "mf": 20}.get(algo, 10)
     prm.n_bins = n_bins
     prm.qmin = qmin
     prm.qmax = qmax

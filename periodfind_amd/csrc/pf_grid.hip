@@ -696,6 +696,204 @@ __global__ __launch_bounds__(kWave) void fpw_persistent(GridArgs g, FpwArgs a, W
 }
 
 // ---------------------------------------------------------------------------
+// Matched Filter — rust/src/mf.rs:172-330 (eval_mf_at_period).
+// LDS: [bin][3][64] f32: sum w, sum w*y, sum w*y*y, in point order
+// (mf.rs:181-187); w = 1/(e*e), w*y and (w*y)*y are prepared per curve set.
+// Epilogue per lane, every f32 operation as in Rust:
+//   * one pass over the bins (k order) for the profile Y/W, filled bins,
+//     total weights, within-bin sums of squares, extremes and RMS (mf.rs:
+//     196-291; separate accumulators, so one pass gives the same sums);
+//   * R^2 needs every point again: the weighted sum of squares about this
+//     trial's grand mean, in point order (mf.rs:230-234);
+//   * circular Pearson maxima against the 14 templates (mf.rs:124-153).  The
+//     profile mean, variance and denominator are the same for every template,
+//     so they are computed once; the centred profile c[k] is kept in the lane's
+//     LDS column.  Shifts are evaluated kMfShiftBlock at a time with one
+//     accumulator each, each summing c[k]*t[k+s] over k in ascending order
+//     (the reference's order).  As the denominator is positive and rounding
+//     is monotone, the best fl(dot/denom) over shifts is fl(max dot / denom):
+//     one division per template instead of one per shift.
+// ---------------------------------------------------------------------------
+struct F3 {
+    float a, b, c;
+};
+
+template <bool HAS_PDT, int G>
+__device__ __forceinline__ void mf_block(const GridArgs& g, const MfArgs& a, const LaneTrial lt) {
+    extern __shared__ float fl[];
+    const int lane = threadIdx.x;
+    const int nb = a.n_bins;
+    float* col = fl + lane;  // bin k: W at plane 3k, Y at 3k+1, V at 3k+2
+    for (int b = 0; b < 3 * nb; ++b) col[b * kWave] = 0.0f;
+    const FoldTrial f =
+        a.curve_period ? make_fold_trial(a.curve_period[lt.curve], 0.0f)
+                       : make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const float* __restrict__ w = a.w + p0;
+    const float* __restrict__ wy = a.wy + p0;
+    const float* __restrict__ wyy = a.wyy + p0;
+    const float nf = static_cast<float>(nb);
+    const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
+    constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
+
+    scan_points_rmw<HAS_PDT, G, 3>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F3{w[i], wy[i], wyy[i]}; }, col,
+        [&](const auto& ph, const auto& v, auto& off, auto& val) {
+            constexpr int K = group_size_v<decltype(ph)>;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                off[j] = mul_bin(ph[j], nf, nm1) * kBinBytes;
+                val[0][j] = v[j].a;
+                val[1][j] = v[j].b;
+                val[2][j] = v[j].c;
+            }
+        });
+
+    // Bins (mf.rs:196-291).  The profile replaces Y in LDS (0 for an empty bin).
+    uint32_t n_filled = 0, n_rms = 0;
+    float total_wt = 0.0f, total_wy = 0.0f, ss_within = 0.0f, sum_rms = 0.0f;
+    float min_p = __builtin_inff(), max_p = -__builtin_inff();
+    for (int k = 0; k < nb; ++k) {
+        const float W = col[(3 * k) * kWave];
+        float prof = 0.0f;
+        if (W > 0.0f) {
+            const float Y = col[(3 * k + 1) * kWave];
+            const float V = col[(3 * k + 2) * kWave];
+            prof = Y / W;
+            n_filled += 1;
+            total_wt += W;
+            total_wy += Y;
+            const float yyw = Y * Y / W;
+            ss_within += V - yyw;
+            if (prof < min_p) min_p = prof;
+            if (prof > max_p) max_p = prof;
+            const float wv = (V - yyw) / W;
+            if (wv > 0.0f) {
+                sum_rms += __builtin_sqrtf(wv);
+                n_rms += 1;
+            }
+        }
+        col[(3 * k + 1) * kWave] = prof;
+    }
+    const float coverage = static_cast<float>(n_filled) / nf;
+    const bool scored = n_filled >= 3;  // mf.rs:208-219
+
+    // R^2 (mf.rs:225-245): a second pass over the points, wave-uniform loads.
+    float r2 = 0.0f;
+    if (scored) {
+        const float gm = total_wy / total_wt;
+        const float* __restrict__ yv = a.y + p0;
+        float ss_total = 0.0f;
+        int64_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            float yy[8], ww[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                yy[j] = uniform_load(yv + i + j);
+                ww[j] = uniform_load(w + i + j);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float d = yy[j] - gm;
+                ss_total += ww[j] * d * d;
+            }
+        }
+        for (; i < n; ++i) {
+            const float d = uniform_load(yv + i) - gm;
+            ss_total += uniform_load(w + i) * d * d;
+        }
+        if (ss_total > 0.0f) {
+            const float q = 1.0f - ss_within / ss_total;
+            r2 = q > 0.0f ? q : 0.0f;  // f32::max(q, 0.0), NaN -> 0.0
+        }
+    }
+
+    // Template correlations (mf.rs:124-153, 293-316).
+    float best[3] = {0.0f, 0.0f, 0.0f};  // sawtooth, sinusoidal, eclipsing
+    if (scored) {
+        float p_mean = -0.0f;
+        for (int k = 0; k < nb; ++k) p_mean = p_mean + col[(3 * k + 1) * kWave];
+        p_mean = p_mean / nf;
+        float p_var = -0.0f;
+        for (int k = 0; k < nb; ++k) {
+            const float c = col[(3 * k + 1) * kWave] - p_mean;
+            p_var = p_var + c * c;
+            col[(3 * k + 2) * kWave] = c;  // centred profile
+        }
+        const float denom = __builtin_sqrtf(p_var) * __builtin_sqrtf(nf);
+        const bool flat = p_var < 1e-20f || denom < 1e-20f;  // correlation 0
+        const int stride = mf_template_stride(nb);  // indices below reach 2*nb + 6
+        for (int tp = 0; tp < kMfShapes; ++tp) {
+            const float* __restrict__ tt = a.tmpl + tp * stride;
+            float maxdot = -__builtin_inff();
+            for (int s0 = 0; s0 < nb; s0 += kMfShiftBlock) {
+                float dot[kMfShiftBlock];
+#pragma unroll
+                for (int j = 0; j < kMfShiftBlock; ++j) dot[j] = 0.0f;
+                for (int k = 0; k < nb; ++k) {
+                    const float c = col[(3 * k + 2) * kWave];
+#pragma unroll
+                    for (int j = 0; j < kMfShiftBlock; ++j)
+                        dot[j] = dot[j] + c * uniform_load(tt + s0 + k + j);
+                }
+#pragma unroll
+                for (int j = 0; j < kMfShiftBlock; ++j)
+                    if (s0 + j < nb && dot[j] > maxdot) maxdot = dot[j];
+            }
+            float corr = 0.0f;
+            if (!flat) {
+                const float r = maxdot / denom;
+                corr = r > -1.0f ? r : -1.0f;  // best starts at -1 (mf.rs:140)
+            }
+            const int kind = tp < 5 ? 0 : (tp == 5 ? 1 : 2);
+            if (corr > best[kind]) best[kind] = corr;
+        }
+    }
+    const float max_corr = fmaxf(fmaxf(best[0], best[1]), best[2]);
+    const float combined = max_corr * r2 * coverage;
+
+    if (!lt.active) return;
+    if (a.feat) {
+        float amp_snr = 0.0f;
+        if (scored) {
+            const float amplitude = max_p - min_p;
+            const float mean_rms = n_rms > 0 ? sum_rms / static_cast<float>(n_rms) : 1.0f;
+            amp_snr = mean_rms > 1e-12f ? amplitude / mean_rms : 0.0f;
+        }
+        float* o = a.feat + lt.curve * kMfFeatures;
+        o[0] = best[0];
+        o[1] = best[1];
+        o[2] = best[2];
+        o[3] = r2;
+        o[4] = amp_snr;
+        o[5] = static_cast<float>(n_filled);
+        o[6] = combined;
+    } else {
+        g.out[lt.curve * g.T + lt.trial] = combined;
+    }
+}
+
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave) void mf_kernel(GridArgs g, MfArgs a) {
+    mf_block<HAS_PDT, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+template <bool HAS_PDT>
+__global__ __launch_bounds__(kWave) void mf_persistent(GridArgs g, MfArgs a, WorkQueue q) {
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: mf_block<HAS_PDT, 2>(g, a, lt); break;
+                case 4: mf_block<HAS_PDT, 4>(g, a, lt); break;
+                default: mf_block<HAS_PDT, 1>(g, a, lt); break;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // BLS — rust/src/bls.rs:20-134.
 // LDS: [slot 0..nb][2][64] f32 (W, Y).  Binning adds bin k into slot k+1; the
 // prefix pass then turns slot k into prefix[k] in place (bls.rs:85-91).  The
@@ -954,6 +1152,15 @@ void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, const WorkQ
 }
 
 template <bool P>
+void mf_g(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQueue* q,
+          hipStream_t s) {
+    if (q) launch_persistent(mf_persistent<P>, g, a, lds, *q, s);
+    else if (grp >= 4) launch_grid_kernel(mf_kernel<P, 4>, g, a, lds, s);
+    else if (grp == 2) launch_grid_kernel(mf_kernel<P, 2>, g, a, lds, s);
+    else launch_grid_kernel(mf_kernel<P, 1>, g, a, lds, s);
+}
+
+template <bool P>
 void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
     if (q) launch_persistent(bls_persistent<P>, g, a, lds, *q, s);
@@ -990,6 +1197,12 @@ hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hip
         else a.fast ? launch_grid_kernel(ls_kernel<false, true>, g, a, 0, s)
                     : launch_grid_kernel(ls_kernel<false, false>, g, a, 0, s);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_mf(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s) {
+    g.has_pdt ? mf_g<true>(g, a, lds, grp, q, s) : mf_g<false>(g, a, lds, grp, q, s);
     return hipGetLastError();
 }
 

@@ -56,6 +56,27 @@ struct LsArgs {
     int fast;               // hardware sin/cos on revolutions
 };
 
+// Matched Filter (rust/src/mf.rs).  Templates: kMfShapes shapes in the order
+// of generate_templates (5 sawtooth, 1 sinusoidal, 8 eclipsing), each stored
+// as a row of mf_template_stride(n_bins) values t[j % n_bins]: the kernel
+// reads t[k + s] for k < n_bins and shifts s in blocks of kMfShiftBlock
+// starting below n_bins, i.e. indices up to 2*n_bins + kMfShiftBlock - 2.
+constexpr int kMfShapes = 14;
+constexpr int kMfShiftBlock = 8;
+__host__ __device__ constexpr int mf_template_stride(int nb) { return 2 * nb + kMfShiftBlock; }
+constexpr int kMfFeatures = 7;  // sawtooth, sinusoidal, eclipsing, R2, amp_snr, n_filled, combined
+
+struct MfArgs {
+    const float* w;             // [npts] 1/(e*e)
+    const float* wy;            // [npts] w*y
+    const float* wyy;           // [npts] (w*y)*y
+    const float* y;             // [npts] magnitudes (R^2 pass)
+    const float* tmpl;          // [kMfShapes][n_bins + kMfShiftBlock]
+    int n_bins;
+    const float* curve_period;  // features: one period per curve (period_dt 0), else null
+    float* feat;                // features: [n_curves][kMfFeatures]
+};
+
 struct FpwArgs {
     const float* w;         // [npts] 1/(e*e)
     const float* wy;        // [npts] w*y
@@ -72,6 +93,8 @@ struct BlsArgs {
 // q == nullptr: one workgroup per (curve, trial block); else a persistent
 // launch pulling items from *q.
 hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s);
+hipError_t launch_mf(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQueue* q,
                      hipStream_t s);
 hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
                       const WorkQueue* q, hipStream_t s);
@@ -95,6 +118,7 @@ inline size_t lds_ce_packed(int n_phase, int n_mag) {
 }
 inline size_t lds_aov(int nb) { return size_t(3) * nb * 64 * 4; }
 inline size_t lds_fpw(int nb) { return size_t(2) * nb * 64 * 4; }
+inline size_t lds_mf(int nb) { return size_t(3) * nb * 64 * 4; }
 inline size_t lds_bls(int nb) { return size_t(2) * (nb + 1) * 64 * 4; }
 
 // Per-point / per-curve preparation kernels (pf_prep.hip).

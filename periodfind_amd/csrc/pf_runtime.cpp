@@ -108,6 +108,8 @@ struct CurveSet {
     bool fpw_ok = false;
     DevBuf bls_yw, bls_mean;
     bool bls_ok = false;
+    DevBuf mf_wyy;  // (w*y)*y (MF); w and w*y are shared with FPW
+    bool mf_ok = false;
     // Last periodogram.
     DevBuf out;
     int64_t out_T = 0;
@@ -127,7 +129,7 @@ struct CurveSet {
         graph_key.clear();
     }
     void invalidate() {
-        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = order_ok = false;
+        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = order_ok = false;
         ce_nmag = -1;
         out_T = 0;
         yp_mode = -1;
@@ -136,14 +138,14 @@ struct CurveSet {
     }
     // Drop what was built from the magnitudes (a new preparation mode).
     void invalidate_mags() {
-        aov_ok = fpw_ok = bls_ok = false;
+        aov_ok = fpw_ok = bls_ok = mf_ok = false;
         ce_nmag = -1;
         drop_graph();
     }
     void release() {
         drop_graph();
         DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                          &fpw_wy, &bls_yw, &bls_mean, &out, &order, &counter};
+                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &out, &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
                 (void)hipFree(b->p);
@@ -186,6 +188,10 @@ struct pf_ctx {
     bool in_busy[2] = {false, false};
     // LS refinement: frequency grid (f64) and per-peak brackets.
     DevBuf ls_freq, ls_lo, ls_hi;
+    // Matched Filter templates of mf_nb bins (rows of mf_template_stride).
+    DevBuf mf_tmpl;
+    int mf_nb = -1;
+    std::vector<float> mf_tmpl_h;
 };
 
 struct pf_curves {
@@ -233,14 +239,15 @@ std::vector<int64_t> tune_edges(const pf_ctx* ctx, int algo) {
 }
 
 constexpr int kMaxTuneBuckets = 32;
-const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls"};
+const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls", "mf"};
+constexpr int kNumAlgos = 6;
 
 int validate_tune(const pf_tune_entry* t, int64_t n) {
     if (n < 0 || (n > 0 && !t)) return fail(PF_E_ARG, "tuning table pointer/size invalid");
-    int per_algo[5] = {0, 0, 0, 0, 0};
+    int per_algo[kNumAlgos] = {};
     for (int64_t i = 0; i < n; ++i) {
         const pf_tune_entry& e = t[i];
-        if (e.algo < PF_ALGO_CE || e.algo > PF_ALGO_BLS)
+        if (e.algo < PF_ALGO_CE || e.algo >= kNumAlgos)
             return fail(PF_E_ARG, "tuning entry %lld: unknown algorithm %d",
                         static_cast<long long>(i), e.algo);
         if (e.min_len < 0)
@@ -261,7 +268,7 @@ int validate_tune(const pf_tune_entry* t, int64_t n) {
 }
 
 // Text table: one entry per line, "<algo> <min_len> <group> <claim>",
-// algo in {ce, aov, ls, fpw, bls}; '#' starts a comment.
+// algo in {ce, aov, ls, fpw, bls, mf}; '#' starts a comment.
 int parse_tune_file(const char* path, std::vector<pf_tune_entry>* out) {
     std::ifstream f(path);
     if (!f) return fail(PF_E_ARG, "cannot open tuning table %s", path);
@@ -277,7 +284,7 @@ int parse_tune_file(const char* path, std::vector<pf_tune_entry>* out) {
         if (!(ss >> name)) continue;
         pf_tune_entry e{};
         e.algo = -1;
-        for (int a = 0; a < 5; ++a)
+        for (int a = 0; a < kNumAlgos; ++a)
             if (name == kAlgoNames[a]) e.algo = a;
         long long ml = 0;
         if (e.algo < 0 || !(ss >> ml >> e.group >> e.claim))
@@ -336,6 +343,11 @@ int check_params(const Params& p, size_t lds_max) {
             if (p.n_bins < 1) return fail(PF_E_ARG, "FPW needs n_bins >= 1");
             if (pf::lds_fpw(p.n_bins) > lds_max)
                 return fail(PF_E_LIMIT, "FPW n_bins=%d exceeds LDS", p.n_bins);
+            break;
+        case PF_ALGO_MF:
+            if (p.n_bins < 1) return fail(PF_E_ARG, "MF needs num_bins >= 1");
+            if (pf::lds_mf(p.n_bins) > lds_max)
+                return fail(PF_E_LIMIT, "MF num_bins=%d exceeds LDS", p.n_bins);
             break;
         case PF_ALGO_BLS:
             if (p.n_bins < 2) return fail(PF_E_ARG, "BLS needs n_bins >= 2");
@@ -559,6 +571,76 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
     return PF_OK;
 }
 
+// Matched Filter templates, generate_templates (rust/src/mf.rs:29-112):
+// 5 sawtooth (rise fractions), 1 sinusoid, 4 dip widths x {no, half-depth}
+// secondary eclipse, each normalised to zero mean and unit variance in f32
+// (sums fold from -0.0, as Rust's f32 Sum does; sin is the C library's sinf,
+// as Rust std calls it).  Built on the host once per bin count and stored as
+// rows of t[j % nb], j < mf_template_stride(nb).
+void mf_normalize(std::vector<float>& t) {
+    const float n = static_cast<float>(t.size());
+    float sum = -0.0f;
+    for (float x : t) sum = sum + x;
+    const float mean = sum / n;
+    for (float& x : t) x -= mean;
+    float var = -0.0f;
+    for (float x : t) var = var + x * x;
+    const float sd = std::sqrt(var / n);
+    if (sd > 1e-12f)
+        for (float& x : t) x /= sd;
+}
+
+std::vector<std::vector<float>> mf_shapes(int nb) {
+    std::vector<std::vector<float>> out;
+    const float nbf = static_cast<float>(nb);
+    auto phase_of = [&](int k) { return (static_cast<float>(k) + 0.5f) / nbf; };
+    for (float rf : {0.10f, 0.20f, 0.35f, 0.50f, 0.65f}) {
+        std::vector<float> t(nb);
+        for (int k = 0; k < nb; ++k) {
+            const float ph = phase_of(k);
+            t[k] = ph < rf ? ph / rf : 1.0f - (ph - rf) / (1.0f - rf);
+        }
+        mf_normalize(t);
+        out.push_back(std::move(t));
+    }
+    {
+        const float two_pi = 2.0f * static_cast<float>(M_PI);
+        std::vector<float> t(nb);
+        for (int k = 0; k < nb; ++k) t[k] = sinf(two_pi * phase_of(k));
+        mf_normalize(t);
+        out.push_back(std::move(t));
+    }
+    for (float dw : {0.05f, 0.10f, 0.15f, 0.20f}) {
+        for (int sec = 0; sec < 2; ++sec) {
+            std::vector<float> t(nb, 1.0f);
+            const float half = dw / 2.0f;
+            for (int k = 0; k < nb; ++k) {
+                const float ph = phase_of(k);
+                if ((ph < 0.5f ? ph : 1.0f - ph) < half) t[k] = 0.0f;
+                if (sec && std::fabs(ph - 0.5f) < half) t[k] = 1.0f - 0.5f;
+            }
+            mf_normalize(t);
+            out.push_back(std::move(t));
+        }
+    }
+    return out;
+}
+
+int ensure_mf_templates(pf_ctx* ctx, int nb) {
+    if (ctx->mf_nb == nb) return PF_OK;
+    const std::vector<std::vector<float>> sh = mf_shapes(nb);
+    const int stride = pf::mf_template_stride(nb);
+    ctx->mf_tmpl_h.assign(static_cast<size_t>(pf::kMfShapes) * stride, 0.0f);
+    for (int s = 0; s < pf::kMfShapes; ++s)
+        for (int j = 0; j < stride; ++j) ctx->mf_tmpl_h[s * stride + j] = sh[s][j % nb];
+    PF_TRY(ensure(ctx->mf_tmpl, ctx->mf_tmpl_h.size() * sizeof(float)));
+    PF_HIP(hipMemcpyAsync(ctx->mf_tmpl.p, ctx->mf_tmpl_h.data(),
+                          ctx->mf_tmpl_h.size() * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    ctx->mf_nb = nb;
+    return PF_OK;
+}
+
 // The magnitudes a call evaluates: as uploaded, or prepared on the device
 // (prepare() must have run for p).
 const float* mags(CurveSet& cs, const Params& p) {
@@ -611,16 +693,23 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
             break;
         case PF_ALGO_FPW:
         case PF_ALGO_BLS:
+        case PF_ALGO_MF:
             if (!cs.w_ok) {
                 PF_TRY(ensure(cs.w, n * sizeof(float)));
                 PF_HIP(pf::launch_inv_var(e, as<float>(cs.w), n, s));
                 cs.w_ok = true;
             }
-            if (p.algo == PF_ALGO_FPW && !cs.fpw_ok) {
+            if ((p.algo == PF_ALGO_FPW || p.algo == PF_ALGO_MF) && !cs.fpw_ok) {
                 PF_TRY(ensure(cs.fpw_wy, n * sizeof(float)));
                 PF_HIP(pf::launch_mul(as<float>(cs.w), y, as<float>(cs.fpw_wy), n, s));
                 cs.fpw_ok = true;
             }
+            if (p.algo == PF_ALGO_MF && !cs.mf_ok) {  // (w*y)*y, mf.rs:187
+                PF_TRY(ensure(cs.mf_wyy, n * sizeof(float)));
+                PF_HIP(pf::launch_mul(as<float>(cs.fpw_wy), y, as<float>(cs.mf_wyy), n, s));
+                cs.mf_ok = true;
+            }
+            if (p.algo == PF_ALGO_MF) PF_TRY(ensure_mf_templates(ctx, p.n_bins));
             if (p.algo == PF_ALGO_BLS && !cs.bls_ok) {
                 PF_TRY(ensure(cs.bls_yw, n * sizeof(float)));
                 PF_TRY(ensure(cs.bls_mean, cs.n_curves * sizeof(float)));
@@ -700,6 +789,12 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
         case PF_ALGO_FPW: {
             pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
             le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), group, q, s);
+            break;
+        }
+        case PF_ALGO_MF: {
+            pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
+                         as<float>(ctx->mf_tmpl), p.n_bins, nullptr, nullptr};
+            le = pf::launch_mf(g, a, pf::lds_mf(p.n_bins), group, q, s);
             break;
         }
         case PF_ALGO_BLS: {
@@ -984,7 +1079,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
-        reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p)};
+        reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
+        reinterpret_cast<uint64_t>(cs.mf_wyy.p), reinterpret_cast<uint64_t>(ctx->mf_tmpl.p)};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
         key.push_back(static_cast<uint64_t>(b.r1));
@@ -1233,7 +1329,7 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->outbuf, &ctx->ls_freq, &ctx->ls_lo,
-                      &ctx->ls_hi};
+                      &ctx->ls_hi, &ctx->mf_tmpl};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
@@ -1279,6 +1375,57 @@ int pf_fpw_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const flo
     Params p{PF_ALGO_FPW, 0, 0, 1, 1, n_bins, 0.0f, 0.0f};
     return grid_host(ctx, p, n_curves, offsets, t, y, err, periods, n_periods, period_dts,
                      n_pdts, out);
+}
+
+int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* err, const float* periods, int64_t n_periods,
+               const float* period_dts, int64_t n_pdts, int num_bins, float* out) {
+    Params p{PF_ALGO_MF, 0, 0, 1, 1, num_bins, 0.0f, 0.0f};
+    return grid_host(ctx, p, n_curves, offsets, t, y, err, periods, n_periods, period_dts,
+                     n_pdts, out);
+}
+
+int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* err, const float* periods, int num_bins,
+                   float* out) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    Params p{PF_ALGO_MF, 0, 0, 1, 1, num_bins, 0.0f, 0.0f};
+    PF_TRY(check_params(p, ctx->lds_max));
+    if (n_curves > 0 && (!periods || !out)) return fail(PF_E_ARG, "periods/out is NULL");
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
+    CurveSet& cs = ctx->scratch;
+    if (n_curves == 0) return finish(ctx);
+    // One trial per curve: the per-curve periods are the trial grid (period_dt
+    // 0), and the kernel folds curve c at periods[c].
+    const float zero = 0.0f;
+    PF_TRY(upload_grid(ctx, periods, n_curves, &zero, 1));
+    PF_TRY(prepare(ctx, cs, p, false));
+    const size_t ob = static_cast<size_t>(n_curves) * pf::kMfFeatures * sizeof(float);
+    PF_TRY(ensure(ctx->outbuf, ob));
+    pf::GridArgs g;
+    g.off = as<int64_t>(cs.off);
+    g.t64 = as<double>(cs.t64);
+    g.tt = as<double>(cs.t64);
+    g.periods = as<float>(ctx->periods);
+    g.pdts = as<float>(ctx->pdts);
+    g.nd = 1;
+    g.T = 1;
+    g.tblocks = 1;
+    g.n_blocks = static_cast<uint64_t>(n_curves);
+    g.out = nullptr;
+    g.has_pdt = 0;
+    if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
+    pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
+                 as<float>(ctx->mf_tmpl), num_bins, as<float>(ctx->periods),
+                 as<float>(ctx->outbuf)};
+    PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const hipError_t le = pf::launch_mf(g, a, pf::lds_mf(num_bins), 1, nullptr, ctx->stream);
+    if (le != hipSuccess) return fail(PF_E_HIP, "MF features launch failed: %s", hipGetErrorString(le));
+    PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    PF_HIP(hipMemcpyAsync(out, ctx->outbuf.p, ob, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
 }
 
 int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,

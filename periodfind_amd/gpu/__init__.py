@@ -304,6 +304,63 @@ class BoxLeastSquares(_Algorithm):
                          significance_type, n_peaks, min_distance)
 
 
+class MatchedFilter(_Algorithm):
+    """Matched Filter morphology score (periodfind/mf.pyx; rust/src/mf.rs).
+
+    Per trial period the light curve is folded into `num_bins` inverse-variance
+    weighted phase bins and the bin means are correlated, over all circular
+    shifts, with 14 shape templates (sawtooth, sinusoid, eclipsing binary);
+    the score is max_corr * R^2 * coverage.  Higher is better; errs default
+    to ones; magnitudes are used as given (normalize/center are accepted and
+    ignored, as in the reference)."""
+
+    _algo = "mf"
+
+    def __init__(self, num_bins=20, device=0):
+        self.num_bins = num_bins
+        self.device = device
+
+    def _params(self):
+        return dict(n_bins=int(self.num_bins))
+
+    def calc(self, times, mags, periods, period_dts, errs=None, output="stats", normalize=False,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        errs = validate_errs(errs, times)
+        return self._run(times, mags, errs, None, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+    def calc_features(self, times, mags, errs, periods):
+        """Features at one period per curve (periodfind/cpu/__init__.py:813-851,
+        rust/src/mf.rs:404-451) -> float32[n_curves, 7]: best_sawtooth,
+        best_sinusoidal, best_eclipsing, R^2, amp_snr, n_filled, combined."""
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        ensure_float32(errs, "errs")
+        if len(errs) != len(times):
+            raise ValueError(
+                f"errs must have the same number of arrays as times, "
+                f"got {len(errs)} and {len(times)}")
+        periods = np.asarray(periods, dtype=np.float32)
+        if periods.ndim != 1 or len(periods) != len(times):
+            raise ValueError(
+                f"periods must be a 1D array with one entry per curve, "
+                f"got shape {periods.shape} for {len(times)} curves")
+        for i, (e, t) in enumerate(zip(errs, times)):
+            if len(e) != len(t):
+                raise ValueError(
+                    f"errs[{i}] and times[{i}] have different lengths: {len(e)} vs {len(t)}")
+        off, t = pack(times)
+        _, y = pack(mags)
+        _, e = pack(errs)
+        return context(self.device).mf_features(off, t, y, e, np.ascontiguousarray(periods),
+                                                int(self.num_bins))
+
+
 def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTANCE_DEFAULT,
                            use_max=True, device=0):
     """Peaks of host periodograms data[n_curves, n_periods, n_pdts] on the device
@@ -312,5 +369,5 @@ def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTA
     return context(device).find_top_peaks(data, int(n_peaks), int(min_distance), bool(use_max))
 
 
-__all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares",
+__all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares", "MatchedFilter",
            "find_top_peaks_batched", "context"]
