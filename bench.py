@@ -40,18 +40,24 @@ PARAMS = {
     "fpw": dict(n_bins=10),
     "bls": dict(n_bins=50, qmin=0.01, qmax=0.5),
     "mf": dict(n_bins=20),
+    "vn": dict(n_phase=20, n_mag=20, phase_ext=1, mag_ext=1, bandwidth=2, margin=1),
 }
 # Algorithmic flops per point-period evaluation (inner loop) and per trial
 # (epilogue): fold 6, bin/accumulate as counted per statistic.
 FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0,
                # MF: fold 6, bin 3, three bin sums 3, the R^2 pass 4 (sub, mul, mul, add)
-               "mf": 16.0}
+               "mf": 16.0,
+               # VN: as CE (fold 6, bin 2, index 2)
+               "vn": 10.0}
 FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0,
                # MF(20): 14 templates x 20 shifts x 20 bins (mul + add), 280 divisions,
                # ~12 per bin for the profile, R^2 and RMS terms
-               "mf": 2.0 * 14 * 20 * 20 + 14 * 20 + 12 * 20}
+               "mf": 2.0 * 14 * 20 * 20 + 14 * 20 + 12 * 20,
+               # VN(20, 20, bw 2): 20 start states x 19 transitions x 20 states x (5 compares
+               # + 1 add), 400 logs and 400 divisions, ~3 per cell for the concentration
+               "vn": 20.0 * 19 * 20 * 6 + 400 + 400 + 3 * 400}
 # Fold-based statistics beside the headline five (measured by --workload siblings).
-SIBLINGS = ("mf",)
+SIBLINGS = ("mf", "vn")
 ERR_ALGOS = ("fpw", "bls", "mf")
 PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
 # Published 1x P100 pts/sec at 8192 points (benchmarks/throughput_results_1gpu.csv:49-81).
@@ -86,7 +92,7 @@ def measured_util(kernel):
 def prep_mags(algo, mags):
     from periodfind_amd._utils import prepare_magnitudes
 
-    if algo == "ce":
+    if algo in ("ce", "vn"):
         return prepare_magnitudes(mags, False, True)
     if algo == "ls":
         return prepare_magnitudes(mags, True, False)
@@ -489,7 +495,8 @@ def run_heavy(args):
 
 
 def run_siblings(args):
-    """The fold-based statistics beside the headline five (Matched Filter),
+    """The fold-based statistics beside the headline five (Matched Filter,
+    Viterbi Narrowband),
     at the headline shape: 100 curves x {1024, 4096, 8192} points, 1000
     periods, output 'stats', resident inputs; GPU kernel roofline and the
     oracle as CPU baseline on a bounded sample."""
@@ -595,7 +602,7 @@ def main():
                     help="batch: the headline 5-statistic workload; ragged: 10k ragged curves; "
                          "lsrefine: differentiable LS peak refinement; heavy: 1000 x 65536 "
                          "points, 1e5 periods; siblings: the fold-based statistics beside the "
-                         "five (Matched Filter) at the batch shape")
+                         "five (Matched Filter, Viterbi Narrowband) at the batch shape")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (profiler counter calibration)")
     ap.add_argument("--oracle-curves", type=int, default=100,

@@ -152,6 +152,16 @@ int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const floa
                const float* y, const float* err, const float* periods, int64_t n_periods,
                const float* period_dts, int64_t n_pdts, int num_bins, float* out);
 
+/* ViterbiNarrowband(n_phase, n_mag, phase_ext, mag_ext, bandwidth,
+ * margin).CalcVNValsBatched — cuda/vn.h, periodfind/vn.pyx:19-34; semantics of
+ * rust/src/vn.rs:17-219: CE's phase-magnitude histogram, then the share of
+ * its mass within `margin` bins of the best circular Viterbi path, rescaled
+ * against a uniform histogram.  Higher is better.  n_mag must be <= 32. */
+int pf_vn_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* periods, int64_t n_periods, const float* period_dts,
+               int64_t n_pdts, int n_phase, int n_mag, int phase_ext, int mag_ext, int bandwidth,
+               int margin, float* out);
+
 /* calc_mf_features_batched (rust/src/lib.rs:866-900, mf.rs:404-451): each
  * curve at its own period (periods[n_curves], period_dt 0) ->
  * out[n_curves][7] = best_sawtooth, best_sinusoidal, best_eclipsing, R^2,
@@ -171,7 +181,7 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
 
 /* Algorithm selector for the generic entry points below. */
 enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4,
-       PF_ALGO_MF = 5 };
+       PF_ALGO_MF = 5, PF_ALGO_VN = 6 };
 
 /* Magnitude preparation applied on the device before evaluation
  * (pf_params.mag_prep): prepare_magnitudes of periodfind/_utils.py:12-47,
@@ -187,14 +197,16 @@ enum { PF_MAG_AS_IS = 0, PF_MAG_CENTER = 1, PF_MAG_NORMALIZE = 2 };
 /* Algorithm parameters; unused fields are ignored. */
 typedef struct pf_params {
     int algo;        /* PF_ALGO_* */
-    int n_phase;     /* CE / AOV phase bins */
-    int n_mag;       /* CE magnitude bins */
-    int phase_ext;   /* CE / AOV phase_bin_extent */
-    int mag_ext;     /* CE mag_bin_extent */
+    int n_phase;     /* CE / AOV / VN phase bins */
+    int n_mag;       /* CE / VN magnitude bins (VN: at most 32) */
+    int phase_ext;   /* CE / AOV / VN phase_bin_extent */
+    int mag_ext;     /* CE / VN mag_bin_extent */
     int n_bins;      /* FPW / BLS bins; MF num_bins */
     float qmin;      /* BLS */
     float qmax;      /* BLS */
     int mag_prep;    /* PF_MAG_*; 0 = as given */
+    int bandwidth;   /* VN: largest magnitude-bin step between phase bins */
+    int margin;      /* VN: bins either side of the path counted as on it */
 } pf_params;
 
 /* Grid evaluation followed by on-device peak selection (peaks.rs:18-238).

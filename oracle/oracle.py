@@ -94,6 +94,11 @@ def lib():
                                     _f32p, _i64, _int, _f32p, _int]
         L.or_mf_features.restype = _int
         L.or_mf_features.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _int, _f32p, _int]
+        L.or_vn_batched.restype = _int
+        L.or_vn_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64, _int,
+                                    _int, _int, _int, _int, _int, _f32p, _int, _int]
+        L.or_vn_concentration.restype = _flt
+        L.or_vn_concentration.argtypes = [_f32p, _int, _int, _int, _int]
         L.or_lsd_tau_power.restype = ctypes.c_double
         L.or_lsd_tau_power.argtypes = [_f64p, _f64p, _i64, ctypes.c_double]
         L.or_lsd_jet.restype = None
@@ -217,6 +222,28 @@ def mf_circular_pearson_max(profile, template):
     return float(lib().or_mf_circular_pearson_max(p, t, p.size))
 
 
+def vn_grid(times, mags, periods, period_dts, n_phase=20, n_mag=20, phase_ext=1, mag_ext=1,
+            bandwidth=2, margin=1, threads=0, dlog=False):
+    """calc_vn_batched (rust/src/lib.rs, vn.rs:156-219) on magnitudes as given
+    (the CPU class normalizes them first, cpu/__init__.py:1093-1150).  dlog:
+    emission logs as the double log rounded to f32 instead of libm logf."""
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    rc = lib().or_vn_batched(len(times), off, t, m, P, P.size, D, D.size, n_phase, n_mag,
+                             phase_ext, mag_ext, bandwidth, margin, out.reshape(-1), threads,
+                             int(bool(dlog)))
+    if rc != 0:
+        raise ValueError("invalid Viterbi Narrowband parameters")
+    return out
+
+
+def vn_concentration(hist, bandwidth=2, margin=1):
+    """circular_viterbi_concentration (vn.rs:17-150) of hist[n_phase, n_mag]."""
+    h = np.ascontiguousarray(hist, dtype=np.float32)
+    return float(lib().or_vn_concentration(h.reshape(-1), h.shape[0], h.shape[1], bandwidth,
+                                           margin))
+
+
 def bls_duration_range(n_bins, qmin, qmax):
     lo, hi = ctypes.c_int64(), ctypes.c_int64()
     lib().or_bls_duration_range(n_bins, qmin, qmax, ctypes.byref(lo), ctypes.byref(hi))
@@ -245,7 +272,8 @@ def find_top_peaks_batched(data, n_peaks=32, min_distance=1, use_max=True):
     return idx, val
 
 
-USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True, "mf": True}
+USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True, "mf": True,
+           "vn": True}
 
 
 def prepare_magnitudes(mags, center, normalize):
@@ -282,6 +310,9 @@ def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **pa
         return fpw_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
     if algo == "bls":
         return bls_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
+    if algo == "vn":  # cpu/__init__.py:1093-1150: normalize by default
+        mm = prepare_magnitudes(mags, params.pop("center", False), params.pop("normalize", True))
+        return vn_grid(times, mm, periods, period_dts, threads=threads, **params)
     if algo == "mf":  # cpu/__init__.py:675-811: errs default to ones, mags as given
         if "n_bins" in params:
             params["num_bins"] = params.pop("n_bins")

@@ -19,7 +19,7 @@ from periodfind_amd.results import Periodogram, Statistics
 
 __all__ = [
     "set_device", "get_device", "ConditionalEntropy", "AOV", "LombScargle", "FPW",
-    "BoxLeastSquares", "MatchedFilter", "Statistics", "Periodogram",
+    "BoxLeastSquares", "MatchedFilter", "ViterbiNarrowband", "Statistics", "Periodogram",
 ]
 
 _default_device = None  # None: auto-detect
@@ -102,3 +102,8 @@ def BoxLeastSquares(**kwargs):
 def MatchedFilter(**kwargs):
     """Matched Filter morphology score on the resolved device."""
     return _make("MatchedFilter", kwargs)
+
+
+def ViterbiNarrowband(**kwargs):
+    """Viterbi Narrowband score on the resolved device."""
+    return _make("ViterbiNarrowband", kwargs)

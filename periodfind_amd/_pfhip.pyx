@@ -31,6 +31,8 @@ cdef extern from "periodfind_amd.h":
         float qmin
         float qmax
         int mag_prep
+        int bandwidth
+        int margin
     ctypedef struct pf_curves:
         pass
 
@@ -58,6 +60,10 @@ cdef extern from "periodfind_amd.h":
                     const float* y, const float* err, const float* periods,
                     int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_bins,
                     float qmin, float qmax, float* out) nogil
+    int pf_vn_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                   const float* y, const float* periods, int64_t n_periods,
+                   const float* period_dts, int64_t n_pdts, int n_phase, int n_mag,
+                   int phase_ext, int mag_ext, int bandwidth, int margin, float* out) nogil
     int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
                    const float* y, const float* err, const float* periods, int64_t n_periods,
                    const float* period_dts, int64_t n_pdts, int num_bins, float* out) nogil
@@ -101,7 +107,7 @@ cdef extern from "periodfind_amd.h":
                         int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
                         int* fallback) nogil
 
-ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5}
+ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6}
 MAG_PREP = {None: 0, "as_is": 0, "center": 1, "normalize": 2}
 
 
@@ -133,6 +139,8 @@ cdef pf_params _params(str algo, dict p):
     prm.qmin = p.get("qmin", 0.01)
     prm.qmax = p.get("qmax", 0.5)
     prm.mag_prep = MAG_PREP[p.get("mag_prep")]
+    prm.bandwidth = p.get("bandwidth", 2)
+    prm.margin = p.get("margin", 1)
     return prm
 
 
@@ -313,10 +321,13 @@ cdef class Context:
         cdef float* op = <float*> cnp.PyArray_DATA(out)
         cdef int rc = 0
         cdef int a = ALGOS[algo]
-        cdef int i0 = 0, i1 = 0, i2 = 0, i3 = 0
+        cdef int i0 = 0, i1 = 0, i2 = 0, i3 = 0, i4 = 0, i5 = 0
         cdef float f0 = 0.0, f1 = 0.0
         if a == 0:
             i0, i1, i2, i3 = p["n_phase"], p["n_mag"], p["phase_ext"], p["mag_ext"]
+        elif a == 6:
+            i0, i1, i2, i3 = p["n_phase"], p["n_mag"], p["phase_ext"], p["mag_ext"]
+            i4, i5 = p["bandwidth"], p["margin"]
         elif a == 1:
             i0, i1 = p["n_phase"], p["phase_ext"]
         elif a == 3 or a == 5:
@@ -335,6 +346,9 @@ cdef class Context:
                     rc = pf_fpw_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
                 elif a == 5:
                     rc = pf_mf_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
+                elif a == 6:
+                    rc = pf_vn_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, i0, i1, i2, i3, i4,
+                                    i5, op)
                 else:
                     rc = pf_bls_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, f0, f1,
                                      op)

@@ -21,7 +21,7 @@ PF_E_HIP = -3
 PF_E_NOMEM = -4
 PF_E_LIMIT = -5
 
-PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5}
+PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6}
 
 # Every symbol the header declares, with (restype, argtypes).
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -56,6 +56,8 @@ class PfParams(ctypes.Structure):
         ("qmin", ctypes.c_float),
         ("qmax", ctypes.c_float),
         ("mag_prep", ctypes.c_int),
+        ("bandwidth", ctypes.c_int),
+        ("margin", ctypes.c_int),
     ]
 
 
@@ -88,6 +90,8 @@ SIGNATURES = {
                            _i64, _int, _f32p]),
     "pf_bls_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
                            _i64, _int, _flt, _flt, _f32p]),
+    "pf_vn_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64, _int, _int,
+                          _int, _int, _int, _int, _f32p]),
     "pf_mf_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
                           _i64, _int, _f32p]),
     "pf_mf_features": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _int,
@@ -254,6 +258,10 @@ class Context:
         elif algo == "bls":
             check(L.pf_bls_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
                                 p.get("n_bins", 50), p.get("qmin", 0.01), p.get("qmax", 0.5), o))
+        elif algo == "vn":
+            check(L.pf_vn_grid(self.h, C, off, t, y, P, P.size, D, D.size,
+                               p.get("n_phase", 20), p.get("n_mag", 20), p.get("phase_ext", 1),
+                               p.get("mag_ext", 1), p.get("bandwidth", 2), p.get("margin", 1), o))
         elif algo == "mf":
             check(L.pf_mf_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
                                p.get("n_bins", 20), o))
@@ -334,12 +342,14 @@ def _refine_outputs(C, n):
 PF_MAG = {"as_is": 0, "center": 1, "normalize": 2}
 
 
-def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
-                qmax=0.5, ext=None, mag_prep="as_is"):
+def make_params(algo, n_phase=None, n_mag=None, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
+                qmax=0.5, ext=None, mag_prep="as_is", bandwidth=2, margin=1):
     prm = PfParams()
     prm.algo = PF_ALGO[algo]
     if n_phase is None:
-        n_phase = 10
+        n_phase = 20 if algo == "vn" else 10
+    if n_mag is None:
+        n_mag = 20 if algo == "vn" else 10
     prm.n_phase = n_phase
     prm.n_mag = n_mag
     prm.phase_ext = ext if ext is not None else phase_ext
@@ -350,6 +360,8 @@ def make_params(algo, n_phase=None, n_mag=10, phase_ext=1, mag_ext=1, n_bins=Non
     prm.qmin = qmin
     prm.qmax = qmax
     prm.mag_prep = PF_MAG[mag_prep] if isinstance(mag_prep, str) else int(mag_prep)
+    prm.bandwidth = bandwidth
+    prm.margin = margin
     return prm
 
 

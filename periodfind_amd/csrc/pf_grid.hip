@@ -293,8 +293,13 @@ struct F2 {
 // hot loop needs no wrap.  mrow[i] is the wave-uniform byte offset
 // (mag bin % n_mag) * 256 of the point's magnitude column.
 // ---------------------------------------------------------------------------
-template <bool HAS_PDT, bool EXT1, int G>
-__device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt) {
+// The (phase, magnitude) count histogram of one trial per lane, as CE builds
+// it (ce.rs:44-63); `epi(count, cell_count, ncell, n, col)` then turns it into
+// the lane's statistic.  count(j, i) is the count of phase row j, magnitude
+// column i (the spare row folded into row 0).
+template <bool HAS_PDT, bool EXT1, int G, class Epi>
+__device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt,
+                                           Epi epi) {
     extern __shared__ uint32_t hist[];
     // G < 0: packed atomics, cells c and c^1 share a word (16-bit counts).
     constexpr bool PACK = EXT1 && G < 0;
@@ -403,7 +408,14 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
         if (EXT1 && j == 0) v += cell_count(a.n_phase * a.n_mag + i);
         return v;
     };
+    epi(count, cell_count, ncell, n, col);
+}
 
+template <bool HAS_PDT, bool EXT1, int G>
+__device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt) {
+    constexpr bool PACK = EXT1 && G < 0;
+    hist_block<HAS_PDT, EXT1, G>(g, a, lt, [&](auto count, auto cell_count, int ncell, int64_t n,
+                                                 uint32_t*) {
     // ce.rs:66-86: p = count / (N*ext_p*ext_m); CE = sum_j sum_i p_ij ln(p_j/p_ij).
     const float norm = static_cast<float>(static_cast<uint64_t>(n) *
                                           static_cast<uint64_t>(a.phase_ext) *
@@ -431,6 +443,7 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
         }
     }
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = ce;
+    });
 }
 
 // One workgroup per (curve, 64-trial block), hardware-scheduled.
@@ -457,6 +470,181 @@ __global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, Wor
                 case 4: ce_block<HAS_PDT, EXT1, 4>(g, a, lt); break;
                 default: ce_block<HAS_PDT, EXT1, 1>(g, a, lt); break;
             }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Viterbi Narrowband — rust/src/vn.rs:17-219.
+// The histogram is CE's (hist_block).  Per lane, the circular Viterbi of
+// vn.rs:36-95 runs in registers: the state vector over the magnitude bins is
+// padded to MAXM with -inf (a padded cell never wins a max, so the clamped
+// transition window of vn.rs:55-61 equals the unclamped one), the window max
+// of radius `bandwidth` is `bandwidth` rounds of a radius-1 max (max is exact
+// and associative), and start states advance kVnStartGroup at a time so each
+// row of emissions is gathered once per group.  Emissions come from the
+// per-curve table lut[count].  The best start is the first strict maximum of
+// the circular totals (vn.rs:81).  Its path is traced back without stored
+// backpointers: for each phase, last to first, the winning pass is recomputed
+// up to the previous phase, and the previous state is the first strict
+// maximum over the transition window (vn.rs:63-68).  Each row's path-mass
+// fraction (vn.rs:120-145) is computed during the trace and stored over that
+// row's counts (the recomputations only read the rows before it), then the
+// fractions are summed in phase order.
+// ---------------------------------------------------------------------------
+constexpr int kVnStartGroup = 4;
+
+template <int MAXM>
+__device__ __forceinline__ void vn_window(float (&v)[MAXM], int bw) {
+    for (int r = 0; r < bw; ++r) {
+        float t[MAXM];
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) {
+            float x = v[m];
+            if (m > 0) x = fmaxf(x, v[m - 1]);
+            if (m + 1 < MAXM) x = fmaxf(x, v[m + 1]);
+            t[m] = x;
+        }
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) v[m] = t[m];
+    }
+}
+
+template <int MAXM, bool PACK, class Count>
+__device__ __forceinline__ float vn_score(const VnArgs& a, Count count, uint32_t* col, int64_t n,
+                                          const float* __restrict__ lut) {
+    constexpr float kNegInf = -__builtin_inff();
+    const int np_ = a.h.n_phase, nm = a.h.n_mag, bw = a.bandwidth, mg = a.margin;
+    auto load_row = [&](int phi, float (&e)[MAXM]) {
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) {
+            e[m] = kNegInf;
+            if (m < nm) e[m] = lut[count(phi, m)];
+        }
+    };
+    // One transition (vn.rs:52-75): window max, plus this row's emission.
+    auto step = [&](float (&v)[MAXM], const float (&e)[MAXM]) {
+        vn_window<MAXM>(v, bw);
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) v[m] = m < nm ? e[m] + v[m] : kNegInf;
+    };
+    float e0[MAXM];
+    load_row(0, e0);
+    float best_total = kNegInf;
+    int best_s = 0;
+    for (int s0 = 0; s0 < nm; s0 += kVnStartGroup) {
+        float v[kVnStartGroup][MAXM];
+#pragma unroll
+        for (int j = 0; j < kVnStartGroup; ++j)
+#pragma unroll
+            for (int m = 0; m < MAXM; ++m) v[j][m] = (m == s0 + j) ? e0[m] : kNegInf;
+        for (int phi = 1; phi < np_; ++phi) {
+            float e[MAXM];
+            load_row(phi, e);
+#pragma unroll
+            for (int j = 0; j < kVnStartGroup; ++j) step(v[j], e);
+        }
+#pragma unroll
+        for (int j = 0; j < kVnStartGroup; ++j) {
+            const int st = s0 + j;
+            float total = kNegInf;
+#pragma unroll
+            for (int m = 0; m < MAXM; ++m)
+                if (m == st) total = v[j][m];
+            if (st < nm && total > best_total) {
+                best_total = total;
+                best_s = st;
+            }
+        }
+    }
+    if (best_total == kNegInf) return 0.0f;  // vn.rs:109-111
+
+    // Trace back, the concentration fraction of each row on the way.
+    auto frac_word = [&](int phi) -> float* {
+        const int c0 = phi * nm;  // packed rows own the word holding cells c0+1.. (n_mag >= 3)
+        return reinterpret_cast<float*>(col + (PACK ? (c0 + 1) / 2 : c0) * kWave);
+    };
+    constexpr float kSkip = -1.0f;  // an empty column (vn.rs:127-129)
+    const float norm = static_cast<float>(static_cast<uint64_t>(n) *
+                                          static_cast<uint64_t>(a.h.phase_ext) *
+                                          static_cast<uint64_t>(a.h.mag_ext));
+    int pm = best_s;
+    for (int phi = np_ - 1; phi >= 0; --phi) {
+        float col_mass = 0.0f;
+        for (int m = 0; m < nm; ++m) col_mass += static_cast<float>(count(phi, m)) / norm;
+        float frac = kSkip;
+        if (!(col_mass <= 0.0f)) {
+            const int lo = pm >= mg ? pm - mg : 0;
+            const int hi = pm + mg < nm ? pm + mg : nm - 1;
+            float path_mass = 0.0f;
+            for (int m = lo; m <= hi; ++m) path_mass += static_cast<float>(count(phi, m)) / norm;
+            frac = path_mass / col_mass;
+        }
+        *frac_word(phi) = frac;
+        if (phi == 0) break;
+        float v[MAXM];
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) v[m] = (m == best_s) ? e0[m] : kNegInf;
+        for (int q = 1; q < phi; ++q) {
+            float e[MAXM];
+            load_row(q, e);
+            step(v, e);
+        }
+        const int lo = pm >= bw ? pm - bw : 0;
+        const int hi = pm + bw < nm ? pm + bw : nm - 1;
+        float best_prev = kNegInf;
+        int arg = 0;
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m)
+            if (m >= lo && m <= hi && v[m] > best_prev) {
+                best_prev = v[m];
+                arg = m;
+            }
+        pm = arg;
+    }
+    float sum_frac = 0.0f;
+    uint32_t n_cols = 0;
+    for (int phi = 0; phi < np_; ++phi) {
+        const float f = *frac_word(phi);
+        if (f != kSkip) {
+            sum_frac += f;
+            n_cols += 1;
+        }
+    }
+    if (n_cols == 0) return 0.0f;
+    const float avg = sum_frac / static_cast<float>(n_cols);
+    const int band = (2 * mg + 1) < nm ? (2 * mg + 1) : nm;
+    const float baseline = static_cast<float>(band) / static_cast<float>(nm);
+    float r = (avg - baseline) / (1.0f - baseline);
+    r = (r != r) ? 0.0f : (r > 0.0f ? r : 0.0f);  // f32::max(0.0), NaN -> 0.0
+    return r < 1.0f ? r : 1.0f;                    // f32::min(1.0)
+}
+
+template <bool HAS_PDT, bool EXT1, int G, int MAXM>
+__device__ __forceinline__ void vn_block(const GridArgs& g, const VnArgs& a, const LaneTrial lt) {
+    constexpr bool PACK = EXT1 && G < 0;
+    hist_block<HAS_PDT, EXT1, G>(g, a.h, lt, [&](auto count, auto, int, int64_t n, uint32_t* col) {
+        const float sc = vn_score<MAXM, PACK>(a, count, col, n, a.lut + a.lut_off[lt.curve]);
+        if (lt.active) g.out[lt.curve * g.T + lt.trial] = sc;
+    });
+}
+
+template <bool HAS_PDT, bool EXT1, int G, int MAXM>
+__global__ __launch_bounds__(kWave) void vn_kernel(GridArgs g, VnArgs a) {
+    vn_block<HAS_PDT, EXT1, G, MAXM>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+template <bool HAS_PDT, bool EXT1, int MAXM>
+__global__ __launch_bounds__(kWave) void vn_persistent(GridArgs g, VnArgs a, WorkQueue q) {
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it);
+            if (!EXT1) {
+                vn_block<HAS_PDT, false, 1, MAXM>(g, a, lt);
+                continue;
+            }
+            if (rank_group(q, it / g.tblocks) < 0) vn_block<HAS_PDT, EXT1, -1, MAXM>(g, a, lt);
+            else vn_block<HAS_PDT, EXT1, 0, MAXM>(g, a, lt);
         }
 }
 
@@ -1151,6 +1339,25 @@ void fpw_g(const GridArgs& g, const FpwArgs& a, size_t lds, int grp, const WorkQ
     else launch_grid_kernel(fpw_kernel<P, 1>, g, a, lds, s);
 }
 
+template <bool P, bool E, int MAXM>
+void vn_g2(const GridArgs& g, const VnArgs& a, size_t lds, int grp, const WorkQueue* q,
+           hipStream_t s) {
+    if (q) launch_persistent(vn_persistent<P, E, MAXM>, g, a, lds, *q, s);
+    else if (!E) launch_grid_kernel(vn_kernel<P, false, 1, MAXM>, g, a, lds, s);
+    else if (grp < 0) launch_grid_kernel(vn_kernel<P, E, -1, MAXM>, g, a, lds, s);
+    else launch_grid_kernel(vn_kernel<P, E, 0, MAXM>, g, a, lds, s);
+}
+
+template <bool P, bool E>
+void vn_g(const GridArgs& g, const VnArgs& a, size_t lds, int grp, const WorkQueue* q,
+          hipStream_t s) {
+    const int nm = a.h.n_mag;
+    if (nm <= 8) vn_g2<P, E, 8>(g, a, lds, grp, q, s);
+    else if (nm <= 16) vn_g2<P, E, 16>(g, a, lds, grp, q, s);
+    else if (nm <= 24) vn_g2<P, E, 24>(g, a, lds, grp, q, s);
+    else vn_g2<P, E, kVnMaxMag>(g, a, lds, grp, q, s);
+}
+
 template <bool P>
 void mf_g(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQueue* q,
           hipStream_t s) {
@@ -1197,6 +1404,15 @@ hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hip
         else a.fast ? launch_grid_kernel(ls_kernel<false, true>, g, a, 0, s)
                     : launch_grid_kernel(ls_kernel<false, false>, g, a, 0, s);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_vn(const GridArgs& g, const VnArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s) {
+    if (a.h.n_mag < 1 || a.h.n_mag > kVnMaxMag) return hipErrorInvalidValue;
+    const bool ext1 = a.h.phase_ext == 1 && a.h.mag_ext == 1;
+    if (g.has_pdt) ext1 ? vn_g<true, true>(g, a, lds, grp, q, s) : vn_g<true, false>(g, a, lds, grp, q, s);
+    else ext1 ? vn_g<false, true>(g, a, lds, grp, q, s) : vn_g<false, false>(g, a, lds, grp, q, s);
     return hipGetLastError();
 }
 

@@ -44,6 +44,18 @@ struct CeArgs {
     int n_phase, n_mag, phase_ext, mag_ext;
 };
 
+// Viterbi Narrowband (rust/src/vn.rs): the CE histogram, then a circular
+// Viterbi per trial.  lut[lut_off[c] + k] = ln(k / (n_c*phase_ext*mag_ext) +
+// 1e-10) for every count k of curve c (the emission log of a cell holding k
+// points, vn.rs:28-33), built once per curve set.
+constexpr int kVnMaxMag = 32;  // magnitude bins the register Viterbi holds
+struct VnArgs {
+    CeArgs h;
+    int bandwidth, margin;
+    const float* lut;
+    const int64_t* lut_off;   // [n_curves]
+};
+
 struct AovArgs {
     const float* y;         // [npts]
     const float* yy;        // [npts] y*y
@@ -96,6 +108,11 @@ hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, co
                      hipStream_t s);
 hipError_t launch_mf(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQueue* q,
                      hipStream_t s);
+hipError_t launch_vn(const GridArgs& g, const VnArgs& a, size_t lds, int grp, const WorkQueue* q,
+                     hipStream_t s);
+// Emission table of every curve (see VnArgs): one workgroup per curve.
+hipError_t launch_vn_lut(const int64_t* off, const int64_t* lut_off, float* lut, int64_t n_curves,
+                         int ext, hipStream_t s);
 hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
                       const WorkQueue* q, hipStream_t s);
 hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hipStream_t s);

@@ -214,6 +214,25 @@ __global__ __launch_bounds__(kBlock) void prep_mags_kernel(const int64_t* __rest
     for (int64_t i = threadIdx.x; i < n; i += kBlock) o[i] = ((x[i] - mn) / range) * 0.999f + 5e-4f;
 }
 
+// Viterbi Narrowband emission table (rust/src/vn.rs:28-33, 208-212): for
+// curve c of n points and every count k in [0, n*ext], ln(fl(fl(k / norm) +
+// 1e-10)) with norm = f32(n*ext).  The f32 log is the double log rounded to
+// f32 (correctly rounded but for inputs within a few double ulps of an f32
+// rounding boundary).  One workgroup per curve.
+__global__ __launch_bounds__(kBlock) void vn_lut_kernel(const int64_t* __restrict__ off,
+                                                        const int64_t* __restrict__ lut_off,
+                                                        float* __restrict__ lut, int ext) {
+    const int64_t c = blockIdx.x;
+    const int64_t kmax = (off[c + 1] - off[c]) * ext;
+    const float norm = static_cast<float>(static_cast<uint64_t>(kmax));
+    float* o = lut + lut_off[c];
+    for (int64_t k = threadIdx.x; k <= kmax; k += kBlock) {
+        const float h = static_cast<float>(static_cast<uint32_t>(k)) / norm;
+        const float x = h + 1e-10f;
+        o[k] = static_cast<float>(log(static_cast<double>(x)));
+    }
+}
+
 }  // namespace
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
@@ -285,6 +304,14 @@ hipError_t launch_prep_mags(const int64_t* off, const float* y, float* yp, int64
     if (mode != PF_MAG_CENTER && mode != PF_MAG_NORMALIZE) return hipErrorInvalidValue;
     hipLaunchKernelGGL(prep_mags_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock), 0,
                        s, off, y, yp, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_vn_lut(const int64_t* off, const int64_t* lut_off, float* lut, int64_t n_curves,
+                         int ext, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(vn_lut_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock), 0, s,
+                       off, lut_off, lut, ext);
     return hipGetLastError();
 }
 

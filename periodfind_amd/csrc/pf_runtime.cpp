@@ -110,6 +110,9 @@ struct CurveSet {
     bool bls_ok = false;
     DevBuf mf_wyy;  // (w*y)*y (MF); w and w*y are shared with FPW
     bool mf_ok = false;
+    DevBuf vn_lut, vn_lut_off;  // VN emission tables, for extents product vn_ext
+    int vn_ext = -1;
+    std::vector<int64_t> vn_lut_off_h;  // host copy (source of the async upload)
     // Last periodogram.
     DevBuf out;
     int64_t out_T = 0;
@@ -130,6 +133,7 @@ struct CurveSet {
     }
     void invalidate() {
         t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = order_ok = false;
+        vn_ext = -1;
         ce_nmag = -1;
         out_T = 0;
         yp_mode = -1;
@@ -145,7 +149,8 @@ struct CurveSet {
     void release() {
         drop_graph();
         DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &out, &order, &counter};
+                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &out,
+                          &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
                 (void)hipFree(b->p);
@@ -206,6 +211,7 @@ struct Params {
     int n_phase, n_mag, phase_ext, mag_ext, n_bins;
     float qmin, qmax;
     int mag_prep;  // PF_MAG_*
+    int bandwidth, margin;  // VN
 };
 
 // --- Launch tuning -----------------------------------------------------------
@@ -239,8 +245,8 @@ std::vector<int64_t> tune_edges(const pf_ctx* ctx, int algo) {
 }
 
 constexpr int kMaxTuneBuckets = 32;
-const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls", "mf"};
-constexpr int kNumAlgos = 6;
+const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls", "mf", "vn"};
+constexpr int kNumAlgos = 7;
 
 int validate_tune(const pf_tune_entry* t, int64_t n) {
     if (n < 0 || (n > 0 && !t)) return fail(PF_E_ARG, "tuning table pointer/size invalid");
@@ -268,7 +274,7 @@ int validate_tune(const pf_tune_entry* t, int64_t n) {
 }
 
 // Text table: one entry per line, "<algo> <min_len> <group> <claim>",
-// algo in {ce, aov, ls, fpw, bls, mf}; '#' starts a comment.
+// algo in {ce, aov, ls, fpw, bls, mf, vn}; '#' starts a comment.
 int parse_tune_file(const char* path, std::vector<pf_tune_entry>* out) {
     std::ifstream f(path);
     if (!f) return fail(PF_E_ARG, "cannot open tuning table %s", path);
@@ -343,6 +349,17 @@ int check_params(const Params& p, size_t lds_max) {
             if (p.n_bins < 1) return fail(PF_E_ARG, "FPW needs n_bins >= 1");
             if (pf::lds_fpw(p.n_bins) > lds_max)
                 return fail(PF_E_LIMIT, "FPW n_bins=%d exceeds LDS", p.n_bins);
+            break;
+        case PF_ALGO_VN:
+            if (p.n_phase < 1 || p.n_mag < 1 || p.phase_ext < 1 || p.mag_ext < 1)
+                return fail(PF_E_ARG, "VN needs n_phase, n_mag, extents >= 1");
+            if (p.bandwidth < 0 || p.margin < 0)
+                return fail(PF_E_ARG, "VN needs bandwidth, margin >= 0");
+            if (p.n_mag > pf::kVnMaxMag)
+                return fail(PF_E_LIMIT, "VN n_mag=%d exceeds %d", p.n_mag, pf::kVnMaxMag);
+            if (pf::lds_ce(p.n_phase, p.n_mag) > lds_max)
+                return fail(PF_E_LIMIT, "VN histogram %dx%d exceeds LDS (%zu B per wave)",
+                            p.n_phase, p.n_mag, lds_max);
             break;
         case PF_ALGO_MF:
             if (p.n_bins < 1) return fail(PF_E_ARG, "MF needs num_bins >= 1");
@@ -675,6 +692,26 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
         cs.tt_ok = cs.tt_ok || has_pdt;
     }
     switch (p.algo) {
+        case PF_ALGO_VN:
+            if (cs.vn_ext != p.phase_ext * p.mag_ext) {
+                const int ext = p.phase_ext * p.mag_ext;
+                std::vector<int64_t>& lo = cs.vn_lut_off_h;
+                lo.assign(static_cast<size_t>(cs.n_curves), 0);
+                int64_t total = 0;
+                for (int64_t c = 0; c < cs.n_curves; ++c) {
+                    lo[c] = total;
+                    total += (cs.off_h[c + 1] - cs.off_h[c]) * ext + 1;
+                }
+                PF_TRY(ensure(cs.vn_lut_off, lo.size() * sizeof(int64_t)));
+                PF_TRY(ensure(cs.vn_lut, static_cast<size_t>(total) * sizeof(float)));
+                if (!lo.empty())
+                    PF_HIP(hipMemcpyAsync(cs.vn_lut_off.p, lo.data(), lo.size() * sizeof(int64_t),
+                                          hipMemcpyHostToDevice, s));
+                PF_HIP(pf::launch_vn_lut(off, as<int64_t>(cs.vn_lut_off), as<float>(cs.vn_lut),
+                                         cs.n_curves, ext, s));
+                cs.vn_ext = ext;
+            }
+            [[fallthrough]];
         case PF_ALGO_CE:
             if (cs.ce_nmag != p.n_mag) {
                 PF_TRY(ensure(cs.ce_mrow, n * sizeof(uint32_t)));
@@ -791,6 +828,24 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), group, q, s);
             break;
         }
+        case PF_ALGO_VN: {
+            pf::VnArgs a{{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext},
+                         p.bandwidth, p.margin, as<float>(cs.vn_lut), as<int64_t>(cs.vn_lut_off)};
+            // 16-bit packed counts when no count can exceed 65535 and every row
+            // owns a whole word for the trace's per-row fraction (n_mag >= 3).
+            const bool can_pack = p.phase_ext == 1 && p.mag_ext == 1 && cs.max_len <= 65535 &&
+                                  p.n_mag >= 3;
+            const size_t lds = can_pack ? pf::lds_ce_packed(p.n_phase, p.n_mag)
+                                        : pf::lds_ce(p.n_phase, p.n_mag);
+            if (q) {
+                pf::WorkQueue qq = *q;
+                for (int b = 0; b < qq.n_buckets; ++b) qq.bucket_group[b] = can_pack ? -1 : 0;
+                le = pf::launch_vn(g, a, lds, 0, &qq, s);
+            } else {
+                le = pf::launch_vn(g, a, lds, can_pack ? -1 : 0, nullptr, s);
+            }
+            break;
+        }
         case PF_ALGO_MF: {
             pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
                          as<float>(ctx->mf_tmpl), p.n_bins, nullptr, nullptr};
@@ -901,6 +956,8 @@ Params to_params(const pf_params* p) {
     q.qmin = p->qmin;
     q.qmax = p->qmax;
     q.mag_prep = p->mag_prep;
+    q.bandwidth = p->bandwidth;
+    q.margin = p->margin;
     return q;
 }
 
@@ -1080,7 +1137,9 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
         reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
-        reinterpret_cast<uint64_t>(cs.mf_wyy.p), reinterpret_cast<uint64_t>(ctx->mf_tmpl.p)};
+        reinterpret_cast<uint64_t>(cs.mf_wyy.p), reinterpret_cast<uint64_t>(ctx->mf_tmpl.p),
+        reinterpret_cast<uint64_t>(cs.vn_lut.p), reinterpret_cast<uint64_t>(cs.vn_lut_off.p),
+        static_cast<uint64_t>(p.bandwidth), static_cast<uint64_t>(p.margin)};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
         key.push_back(static_cast<uint64_t>(b.r1));
@@ -1382,6 +1441,16 @@ int pf_mf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const floa
                const float* period_dts, int64_t n_pdts, int num_bins, float* out) {
     Params p{PF_ALGO_MF, 0, 0, 1, 1, num_bins, 0.0f, 0.0f};
     return grid_host(ctx, p, n_curves, offsets, t, y, err, periods, n_periods, period_dts,
+                     n_pdts, out);
+}
+
+int pf_vn_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+               const float* y, const float* periods, int64_t n_periods, const float* period_dts,
+               int64_t n_pdts, int n_phase, int n_mag, int phase_ext, int mag_ext, int bandwidth,
+               int margin, float* out) {
+    Params p{PF_ALGO_VN, n_phase, n_mag, phase_ext, mag_ext, 0, 0.0f, 0.0f, PF_MAG_AS_IS,
+             bandwidth, margin};
+    return grid_host(ctx, p, n_curves, offsets, t, y, nullptr, periods, n_periods, period_dts,
                      n_pdts, out);
 }
 

@@ -304,6 +304,44 @@ class BoxLeastSquares(_Algorithm):
                          significance_type, n_peaks, min_distance)
 
 
+class ViterbiNarrowband(_Algorithm):
+    """Viterbi Narrowband score (periodfind/vn.pyx; rust/src/vn.rs).
+
+    The Conditional Entropy phase-magnitude histogram, then the best circular
+    Viterbi path through it (magnitude steps of at most `bandwidth` bins
+    between phase bins); the score is the share of each column's mass within
+    `margin` bins of the path, averaged and rescaled against a uniform
+    histogram.  Higher is better; magnitudes are normalized by default (on
+    the device, as prepare_magnitudes does)."""
+
+    _algo = "vn"
+
+    def __init__(self, n_phase=20, n_mag=20, phase_bin_extent=1, mag_bin_extent=1, bandwidth=2,
+                 margin=1, device=0):
+        self.n_phase = n_phase
+        self.n_mag = n_mag
+        self.phase_bin_extent = phase_bin_extent
+        self.mag_bin_extent = mag_bin_extent
+        self.bandwidth = bandwidth
+        self.margin = margin
+        self.device = device
+
+    def _params(self):
+        return dict(n_phase=int(self.n_phase), n_mag=int(self.n_mag),
+                    phase_ext=int(self.phase_bin_extent), mag_ext=int(self.mag_bin_extent),
+                    bandwidth=int(self.bandwidth), margin=int(self.margin))
+
+    def calc(self, times, mags, periods, period_dts, output="stats", normalize=True,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        mode = magnitude_mode(mags, center, normalize)
+        return self._run(times, mags, None, mode, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+
 class MatchedFilter(_Algorithm):
     """Matched Filter morphology score (periodfind/mf.pyx; rust/src/mf.rs).
 
@@ -370,4 +408,5 @@ def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTA
 
 
 __all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares", "MatchedFilter",
+           "ViterbiNarrowband",
            "find_top_peaks_batched", "context"]
