@@ -514,6 +514,9 @@ __device__ __forceinline__ float vn_score(const VnArgs& a, Count count, uint32_t
                                           const float* __restrict__ lut) {
     constexpr float kNegInf = -__builtin_inff();
     const int np_ = a.h.n_phase, nm = a.h.n_mag, bw = a.bandwidth, mg = a.margin;
+    // Emissions of a row, gathered from the curve's table.  (An LDS copy of
+    // the table's head measured slower: the extra LDS costs the third wave
+    // per CU that the histogram leaves room for.)
     auto load_row = [&](int phi, float (&e)[MAXM]) {
 #pragma unroll
         for (int m = 0; m < MAXM; ++m) {
@@ -619,6 +622,7 @@ __device__ __forceinline__ float vn_score(const VnArgs& a, Count count, uint32_t
     return r < 1.0f ? r : 1.0f;                    // f32::min(1.0)
 }
 
+// LDS: the histogram, as CE's.
 template <bool HAS_PDT, bool EXT1, int G, int MAXM>
 __device__ __forceinline__ void vn_block(const GridArgs& g, const VnArgs& a, const LaneTrial lt) {
     constexpr bool PACK = EXT1 && G < 0;
