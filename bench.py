@@ -41,6 +41,7 @@ PARAMS = {
     "bls": dict(n_bins=50, qmin=0.01, qmax=0.5),
     "mf": dict(n_bins=20),
     "vn": dict(n_phase=20, n_mag=20, phase_ext=1, mag_ext=1, bandwidth=2, margin=1),
+    "mhf": dict(max_harmonics=5),
 }
 # Algorithmic flops per point-period evaluation (inner loop) and per trial
 # (epilogue): fold 6, bin/accumulate as counted per statistic.
@@ -48,18 +49,27 @@ FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0,
                # MF: fold 6, bin 3, three bin sums 3, the R^2 pass 4 (sub, mul, mul, add)
                "mf": 16.0,
                # VN: as CE (fold 6, bin 2, index 2)
-               "vn": 10.0}
+               "vn": 10.0,
+               # MHF(5), FP64: fold 6, harmonics 2..10 by recurrence (2 x 9 FMA), the
+               # 8K = 40 harmonic sums (FMA); the sincos itself is not counted
+               "mhf": 6.0 + 2.0 * 18 + 2.0 * 40}
 FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0,
                # MF(20): 14 templates x 20 shifts x 20 bins (mul + add), 280 divisions,
                # ~12 per bin for the profile, R^2 and RMS terms
                "mf": 2.0 * 14 * 20 * 20 + 14 * 20 + 12 * 20,
                # VN(20, 20, bw 2): 20 start states x 19 transitions x 20 states x (5 compares
                # + 1 add), 400 logs and 400 divisions, ~3 per cell for the concentration
-               "vn": 20.0 * 19 * 20 * 6 + 400 + 400 + 3 * 400}
+               "vn": 20.0 * 19 * 20 * 6 + 400 + 400 + 3 * 400,
+               # MHF(5): normal matrix assembly (78 entries), 12x12 Cholesky (~12^3/3 FMA)
+               # with forward substitution, BIC of 6 models
+               "mhf": 2.0 * 78 + 2.0 * 576 + 2.0 * 78 + 30}
 # Fold-based statistics beside the headline five (measured by --workload siblings).
-SIBLINGS = ("mf", "vn")
-ERR_ALGOS = ("fpw", "bls", "mf")
+SIBLINGS = ("mf", "vn", "mhf")
+ERR_ALGOS = ("fpw", "bls", "mf", "mhf")
 PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
+PEAK_F64_TFLOPS = 78.6    # MI355X FP64 vector, published
+# Arithmetic of each statistic's hot loop, and the peak it is held against.
+PEAK_OF = {"mhf": PEAK_F64_TFLOPS}
 # Published 1x P100 pts/sec at 8192 points (benchmarks/throughput_results_1gpu.csv:49-81).
 P100_8192 = {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510}
 
@@ -496,7 +506,7 @@ def run_heavy(args):
 
 def run_siblings(args):
     """The fold-based statistics beside the headline five (Matched Filter,
-    Viterbi Narrowband),
+    Viterbi Narrowband, Multi-Harmonic Fourier),
     at the headline shape: 100 curves x {1024, 4096, 8192} points, 1000
     periods, output 'stats', resident inputs; GPU kernel roofline and the
     oracle as CPU baseline on a bounded sample."""
@@ -530,25 +540,26 @@ def run_siblings(args):
         kt = float(np.mean(v)) * 1e-3
         evals = N_CURVES * n * 1000
         ach = evals * (FLOPS_INNER[algo] + FLOPS_TRIAL[algo] / n) / kt / 1e12
+        peak = PEAK_OF.get(algo, PEAK_VALU_TFLOPS)
         detail[f"{algo}@{n}"] = {"kernel_ms": round(kt * 1e3, 4),
                                  "call_ms": round(float(np.median(walls[(algo, n)])) * 1e3, 4),
                                  "evals_per_s": evals / kt, "tflops": round(ach, 3),
-                                 "frac": round(ach / PEAK_VALU_TFLOPS, 4)}
+                                 "peak": peak, "frac": round(ach / peak, 4)}
         if best is None or kt > best[1]:
-            best = (f"{algo}@{n}", kt, ach)
+            best = (f"{algo}@{n}", kt, ach, peak)
     for r in sets.values():
         r.close()
     ctx.close()
     out = {"metric": "pts/sec @1000 periods per algo", "value": pts_per_step / (wall / args.steps),
            "unit": "pts/sec", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 (fold in f64)", "data": "synthetic",
+           "vs_baseline": None, "dtype": "f32 (fold in f64); mhf f64", "data": "synthetic",
            "config": {"workload": "siblings: " + ", ".join(SIBLINGS) + " x 100 curves x "
                                   "{1024,4096,8192} points, 1000 periods, output=stats",
                       "params": {a: PARAMS[a] for a in SIBLINGS}},
            "roofline": {"bound": "valu", "kernel": best[0], "achieved": round(best[2], 3),
-                        "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(best[2] / PEAK_VALU_TFLOPS, 4), "traffic": None},
+                        "peak": best[3], "unit": "TFLOP/s",
+                        "frac": round(best[2] / best[3], 4), "traffic": None},
            "kernels": detail}
     if args.cpu_seconds > 0:
         print("bench: siblings CPU baseline", file=sys.stderr, flush=True)
@@ -561,8 +572,10 @@ def run_siblings(args):
         for algo in SIBLINGS:
             ta = time.perf_counter()
             O.calc_grid(algo, t[:1], m[:1], P, D, errs=e[:1], threads=threads, **PARAMS[algo])
-            per_curve = max(time.perf_counter() - ta, 1e-6)
-            k = int(max(1, min(N_CURVES, args.cpu_seconds / len(SIBLINGS) / per_curve)))
+            per_curve = max(time.perf_counter() - ta, 1e-6)  # one curve: one thread
+            # The oracle runs curves in parallel: size the sample for all threads.
+            k = int(max(min(threads, N_CURVES),
+                        min(N_CURVES, args.cpu_seconds / len(SIBLINGS) * threads / per_curve)))
             ta = time.perf_counter()
             O.calc_grid(algo, t[:k], m[:k], P, D, errs=e[:k], threads=threads, **PARAMS[algo])
             dt = time.perf_counter() - ta

@@ -170,6 +170,25 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
                    const float* y, const float* err, const float* periods, int num_bins,
                    float* out);
 
+/* MultiHarmonicFourier(max_harmonics).CalcMHFBatched — cuda/mhf.h,
+ * periodfind/mhf.pyx:19-31; semantics of rust/src/mhf.rs:68-291 (calc_mhf):
+ * per trial, weighted least-squares fits of offset + slope + k Fourier
+ * harmonics for k = 0..min(max_harmonics, 5), scored by the best BIC gain
+ * over the flat model (0 when no fit improves on it).  err == NULL means all
+ * ones; a point with err <= 0 has weight 0.  Higher is better.  Computed in
+ * FP64; max_harmonics < 0 is an error. */
+int pf_mhf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                const float* y, const float* err, const float* periods, int64_t n_periods,
+                const float* period_dts, int64_t n_pdts, int max_harmonics, float* out);
+
+/* calc_mhf_per_k_batched (rust/src/lib.rs:977-1037, mhf.rs:299-334): each
+ * curve at its own (periods[c], period_dts[c]) (period_dts == NULL: zeros) ->
+ * out[n_curves][K + 2] with K = min(max_harmonics, 5): the BIC gain of every
+ * model k = 0..K, then the best k (as a float). */
+int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                 const float* y, const float* err, const float* periods,
+                 const float* period_dts, int max_harmonics, float* out);
+
 /* ------------------------------------------------------------------------ */
 /* Device top-N peaks.  Replaces the CPU-only fused entry points             */
 /* calc_{ce,aov,ls,fpw,bls}_peaks_batched (rust/src/lib.rs:559-754) and the  */
@@ -181,7 +200,7 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
 
 /* Algorithm selector for the generic entry points below. */
 enum { PF_ALGO_CE = 0, PF_ALGO_AOV = 1, PF_ALGO_LS = 2, PF_ALGO_FPW = 3, PF_ALGO_BLS = 4,
-       PF_ALGO_MF = 5, PF_ALGO_VN = 6 };
+       PF_ALGO_MF = 5, PF_ALGO_VN = 6, PF_ALGO_MHF = 7 };
 
 /* Magnitude preparation applied on the device before evaluation
  * (pf_params.mag_prep): prepare_magnitudes of periodfind/_utils.py:12-47,
@@ -207,6 +226,7 @@ typedef struct pf_params {
     int mag_prep;    /* PF_MAG_*; 0 = as given */
     int bandwidth;   /* VN: largest magnitude-bin step between phase bins */
     int margin;      /* VN: bins either side of the path counted as on it */
+    int max_harmonics; /* MHF: largest model's harmonics (above 5 means 5) */
 } pf_params;
 
 /* Grid evaluation followed by on-device peak selection (peaks.rs:18-238).

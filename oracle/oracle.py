@@ -97,6 +97,12 @@ def lib():
         L.or_vn_batched.restype = _int
         L.or_vn_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64, _int,
                                     _int, _int, _int, _int, _int, _f32p, _int, _int]
+        L.or_mhf_batched.restype = _int
+        L.or_mhf_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64,
+                                     _f32p, _i64, _int, _f32p, _int]
+        L.or_mhf_per_k.restype = _int
+        L.or_mhf_per_k.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _f32p, _int,
+                                   _f32p, _int]
         L.or_vn_concentration.restype = _flt
         L.or_vn_concentration.argtypes = [_f32p, _int, _int, _int, _int]
         L.or_lsd_tau_power.restype = ctypes.c_double
@@ -244,6 +250,38 @@ def vn_concentration(hist, bandwidth=2, margin=1):
                                            margin))
 
 
+def mhf_grid(times, mags, errs, periods, period_dts, max_harmonics=5, threads=0):
+    """calc_mhf_batched (rust/src/lib.rs:914-970, mhf.rs:234-291): the best
+    BIC gain over the flat model per trial; errs None = ones."""
+    off, t, P, D, out = _grid_args(times, periods, period_dts)
+    _, m = pack(mags)
+    e = pack(errs)[1] if errs is not None else None
+    rc = lib().or_mhf_batched(len(times), off, t, m, e, P, P.size, D, D.size, max_harmonics,
+                              out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("max_harmonics must be >= 0")
+    return out
+
+
+def mhf_per_k(times, mags, errs, periods, period_dts=None, max_harmonics=5, threads=0):
+    """calc_mhf_per_k_batched (lib.rs:977-1037, mhf.rs:299-334): curve c at
+    (periods[c], period_dts[c]) -> float32[C, K + 2], K = min(max_harmonics, 5):
+    the BIC gain of each model k = 0..K, then the best k."""
+    off, t = pack(times)
+    _, m = pack(mags)
+    e = pack(errs)[1] if errs is not None else None
+    C = len(times)
+    P = np.ascontiguousarray(periods, dtype=np.float32)
+    D = (np.zeros(C, np.float32) if period_dts is None
+         else np.ascontiguousarray(period_dts, dtype=np.float32))
+    K = min(max_harmonics, 5)
+    out = np.zeros((C, max(K, 0) + 2), np.float32)
+    rc = lib().or_mhf_per_k(C, off, t, m, e, P, D, max_harmonics, out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("max_harmonics must be >= 0")
+    return out
+
+
 def bls_duration_range(n_bins, qmin, qmax):
     lo, hi = ctypes.c_int64(), ctypes.c_int64()
     lib().or_bls_duration_range(n_bins, qmin, qmax, ctypes.byref(lo), ctypes.byref(hi))
@@ -273,7 +311,7 @@ def find_top_peaks_batched(data, n_peaks=32, min_distance=1, use_max=True):
 
 
 USE_MAX = {"ce": False, "aov": True, "ls": True, "fpw": True, "bls": True, "mf": True,
-           "vn": True}
+           "vn": True, "mhf": True}
 
 
 def prepare_magnitudes(mags, center, normalize):
@@ -317,6 +355,8 @@ def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **pa
         if "n_bins" in params:
             params["num_bins"] = params.pop("n_bins")
         return mf_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
+    if algo == "mhf":  # cpu/__init__.py:874-979: errs default to ones, mags as given
+        return mhf_grid(times, mags, errs, periods, period_dts, threads=threads, **params)
     raise ValueError(algo)
 
 

@@ -19,7 +19,8 @@ from periodfind_amd.results import Periodogram, Statistics
 
 __all__ = [
     "set_device", "get_device", "ConditionalEntropy", "AOV", "LombScargle", "FPW",
-    "BoxLeastSquares", "MatchedFilter", "ViterbiNarrowband", "Statistics", "Periodogram",
+    "BoxLeastSquares", "MatchedFilter", "ViterbiNarrowband", "MultiHarmonicFourier",
+    "Statistics", "Periodogram",
 ]
 
 _default_device = None  # None: auto-detect
@@ -107,3 +108,8 @@ def MatchedFilter(**kwargs):
 def ViterbiNarrowband(**kwargs):
     """Viterbi Narrowband score on the resolved device."""
     return _make("ViterbiNarrowband", kwargs)
+
+
+def MultiHarmonicFourier(**kwargs):
+    """Multi-Harmonic Fourier periodogram on the resolved device."""
+    return _make("MultiHarmonicFourier", kwargs)

@@ -33,6 +33,7 @@ cdef extern from "periodfind_amd.h":
         int mag_prep
         int bandwidth
         int margin
+        int max_harmonics
     ctypedef struct pf_curves:
         pass
 
@@ -70,6 +71,13 @@ cdef extern from "periodfind_amd.h":
     int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
                        const float* y, const float* err, const float* periods, int num_bins,
                        float* out) nogil
+    int pf_mhf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                    const float* y, const float* err, const float* periods, int64_t n_periods,
+                    const float* period_dts, int64_t n_pdts, int max_harmonics,
+                    float* out) nogil
+    int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                     const float* y, const float* err, const float* periods,
+                     const float* period_dts, int max_harmonics, float* out) nogil
     int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
                  const int64_t* offsets, const float* t, const float* y, const float* err,
                  const float* periods, int64_t n_periods, const float* period_dts,
@@ -107,7 +115,7 @@ cdef extern from "periodfind_amd.h":
                         int64_t n_pdts, int n_stats, float* summary, int64_t* idx, float* val,
                         int* fallback) nogil
 
-ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6}
+ALGOS = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6, "mhf": 7}
 MAG_PREP = {None: 0, "as_is": 0, "center": 1, "normalize": 2}
 
 
@@ -141,6 +149,7 @@ cdef pf_params _params(str algo, dict p):
     prm.mag_prep = MAG_PREP[p.get("mag_prep")]
     prm.bandwidth = p.get("bandwidth", 2)
     prm.margin = p.get("margin", 1)
+    prm.max_harmonics = p.get("max_harmonics", 5)
     return prm
 
 
@@ -332,6 +341,8 @@ cdef class Context:
             i0, i1 = p["n_phase"], p["phase_ext"]
         elif a == 3 or a == 5:
             i0 = p["n_bins"]
+        elif a == 7:
+            i0 = p["max_harmonics"]
         elif a == 4:
             i0, f0, f1 = p["n_bins"], p["qmin"], p["qmax"]
         with self.lock:
@@ -349,6 +360,8 @@ cdef class Context:
                 elif a == 6:
                     rc = pf_vn_grid(self.ctx, C, o, tp, yp, pp, NP, dp, ND, i0, i1, i2, i3, i4,
                                     i5, op)
+                elif a == 7:
+                    rc = pf_mhf_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, op)
                 else:
                     rc = pf_bls_grid(self.ctx, C, o, tp, yp, ep, pp, NP, dp, ND, i0, f0, f1,
                                      op)
@@ -460,6 +473,30 @@ cdef class Context:
         with self.lock:
             with nogil:
                 rc = pf_mf_features(self.ctx, C, o, tp, yp, ep, pp, num_bins, op)
+        if rc != 0:
+            _raise(rc)
+        return out
+
+    def mhf_per_k(self, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+                  cnp.ndarray periods, pdts, int max_harmonics):
+        """Multi-Harmonic Fourier per model, curve c at (periods[c], pdts[c])
+        (pdts None: zeros) -> float32[C, min(max_harmonics, 5) + 2]."""
+        cdef int64_t C = off.shape[0] - 1
+        cdef int K = max_harmonics if max_harmonics < 5 else 5
+        if K < 0:
+            raise ValueError("max_harmonics must be >= 0")
+        out = np.zeros((C, K + 2), dtype=np.float32)
+        cdef float* op = <float*> cnp.PyArray_DATA(out)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef const float* dp = _fptr(pdts) if pdts is not None else NULL
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_mhf_per_k(self.ctx, C, o, tp, yp, ep, pp, dp, max_harmonics, op)
         if rc != 0:
             _raise(rc)
         return out

@@ -21,7 +21,7 @@ PF_E_HIP = -3
 PF_E_NOMEM = -4
 PF_E_LIMIT = -5
 
-PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6}
+PF_ALGO = {"ce": 0, "aov": 1, "ls": 2, "fpw": 3, "bls": 4, "mf": 5, "vn": 6, "mhf": 7}
 
 # Every symbol the header declares, with (restype, argtypes).
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -58,6 +58,7 @@ class PfParams(ctypes.Structure):
         ("mag_prep", ctypes.c_int),
         ("bandwidth", ctypes.c_int),
         ("margin", ctypes.c_int),
+        ("max_harmonics", ctypes.c_int),
     ]
 
 
@@ -96,6 +97,10 @@ SIGNATURES = {
                           _i64, _int, _f32p]),
     "pf_mf_features": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _int,
                               _f32p]),
+    "pf_mhf_grid": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64, _f32p,
+                           _i64, _int, _f32p]),
+    "pf_mhf_per_k": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p,
+                            _f32p_or_null, _int, _f32p]),
     "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
                         _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
     "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
@@ -265,6 +270,9 @@ class Context:
         elif algo == "mf":
             check(L.pf_mf_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
                                p.get("n_bins", 20), o))
+        elif algo == "mhf":
+            check(L.pf_mhf_grid(self.h, C, off, t, y, e, P, P.size, D, D.size,
+                                p.get("max_harmonics", 5), o))
         else:
             raise ValueError(algo)
         return out
@@ -278,6 +286,19 @@ class Context:
         out = np.zeros((len(times), 7), dtype=np.float32)
         check(lib().pf_mf_features(self.h, len(times), off, t, y, e, P, num_bins,
                                    out.reshape(-1)))
+        return out
+
+    def mhf_per_k(self, times, mags, errs, periods, period_dts=None, max_harmonics=5):
+        """Multi-Harmonic Fourier per model at one (period, period_dt) per
+        curve -> float32[C, min(max_harmonics, 5) + 2]."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        D = np.ascontiguousarray(period_dts, dtype=np.float32) if period_dts is not None else None
+        out = np.zeros((len(times), max(0, min(max_harmonics, 5)) + 2), dtype=np.float32)
+        check(lib().pf_mhf_per_k(self.h, len(times), off, t, y, e, P, D, max_harmonics,
+                                 out.reshape(-1)))
         return out
 
     def peaks(self, algo, times, mags, periods, period_dts, errs=None, n_peaks=32,
@@ -343,7 +364,7 @@ PF_MAG = {"as_is": 0, "center": 1, "normalize": 2}
 
 
 def make_params(algo, n_phase=None, n_mag=None, phase_ext=1, mag_ext=1, n_bins=None, qmin=0.01,
-                qmax=0.5, ext=None, mag_prep="as_is", bandwidth=2, margin=1):
+                qmax=0.5, ext=None, mag_prep="as_is", bandwidth=2, margin=1, max_harmonics=5):
     prm = PfParams()
     prm.algo = PF_ALGO[algo]
     if n_phase is None:
@@ -362,6 +383,7 @@ def make_params(algo, n_phase=None, n_mag=None, phase_ext=1, mag_ext=1, n_bins=N
     prm.mag_prep = PF_MAG[mag_prep] if isinstance(mag_prep, str) else int(mag_prep)
     prm.bandwidth = bandwidth
     prm.margin = margin
+    prm.max_harmonics = max_harmonics
     return prm
 
 

@@ -1086,6 +1086,233 @@ __global__ __launch_bounds__(kWave) void mf_persistent(GridArgs g, MfArgs a, Wor
 }
 
 // ---------------------------------------------------------------------------
+// Multi-Harmonic Fourier — rust/src/mhf.rs:75-226 (eval_mhf_single_period).
+// Weighted least squares of y on [1, x, cos h phi, sin h phi (h = 1..K)],
+// phi = 2 pi phase, for the models k = 0..K, scored by BIC; the trial's value
+// is the best BIC gain over the flat model k = 0.  FP64 throughout, in
+// registers (no LDS).
+//
+// Normal equations.  The reference sums every product of two design columns
+// over the points (mhf.rs:129-145).  Products of harmonics reduce to single
+// harmonics (cos a cos b = (cos(a-b) + cos(a+b))/2, and likewise), so the
+// point loop accumulates only
+//   C[m], S[m] = sum w cos(m phi), sum w sin(m phi)       m = 1..2K
+//   PC[h], PS[h] = sum (w x) cos(h phi), sum (w x) sin     h = 1..K
+//   YC[h], YS[h] = sum (w y) cos(h phi), sum (w y) sin
+// (8K sums instead of the (2K+2)(2K+3)/2 + 2K+2 entries), the harmonics by
+// the reference's Chebyshev recurrence from one sincos per point; the
+// offset/slope sums do not depend on the trial and come per curve.
+//
+// Models.  The leading (2k+2)-block of the normal matrix is model k's, and
+// Cholesky factors and forward substitutions are prefix-consistent: model k's
+// factor and z = L^-1 b are the leading parts of the full ones, computed by the
+// same operations.  So one factorization serves every model (the reference
+// factors each model's block anew with the same operations, mhf.rs:159-177),
+// and chi2_k = y'Wy - sum_{j < 2k+2} z_j^2, so that
+//   BIC_flat - BIC_k = sum_{2 <= j < 2k+2} z_j^2 - 2k ln n
+// without the residual pass of mhf.rs:181-189 (and without cancelling two
+// large chi2 values).  A pivot that is not positive fails its model and every
+// larger one (cholesky_solve's failure, fourier.rs:37-39); a NaN pivot, which
+// the reference carries into a NaN score clamped to 0, gives the same 0 here.
+// ---------------------------------------------------------------------------
+
+// Walk a curve's MHF records in order.  Records carry their own time (t*t
+// for the period derivative is formed in the kernel, exactly); chunks of C
+// records are prefetched with scalar loads one chunk ahead, as in scan_points.
+__device__ __forceinline__ MhfPoint load_record(const MhfPoint* p) {
+    return MhfPoint{uniform_load(&p->t), uniform_load(&p->w), uniform_load(&p->wx),
+                    uniform_load(&p->wy)};
+}
+
+template <bool HAS_PDT, int C, class Body>
+__device__ __forceinline__ void scan_records(const MhfPoint* __restrict__ pt, int64_t n,
+                                             const FoldTrial& f, Body body) {
+    struct Chunk {
+        MhfPoint r[C];
+    };
+    auto fetch = [&](int64_t base, Chunk& ch) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) ch.r[j] = load_record(pt + base + j);
+    };
+    auto run = [&](const Chunk& ch) {
+        double t[C], tt[C];
+        float ph[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            t[j] = ch.r[j].t;
+            tt[j] = HAS_PDT ? t[j] * t[j] : 0.0;
+        }
+        fold_phases<HAS_PDT, C>(t, tt, f, ph);
+#pragma unroll
+        for (int j = 0; j < C; ++j) body(ph[j], ch.r[j]);
+    };
+    int64_t i = 0;
+    if (n >= 2 * C) {
+        const int64_t last = n - C;
+        Chunk a, b;
+        fetch(0, a);
+        for (; i + 2 * C <= n; i += 2 * C) {
+            wait_scalar_loads();
+            fetch(i + C, b);
+            __builtin_amdgcn_sched_barrier(0);
+            run(a);
+            wait_scalar_loads();
+            const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
+            fetch(na, a);
+            __builtin_amdgcn_sched_barrier(0);
+            run(b);
+        }
+    }
+    for (; i < n; ++i) {
+        const MhfPoint r = load_record(pt + i);
+        body(fold_phase<HAS_PDT>(r.t, HAS_PDT ? r.t * r.t : 0.0, f), r);
+    }
+}
+
+template <bool HAS_PDT, int K>
+__device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, const LaneTrial lt) {
+    constexpr int M = 2 * K;   // highest harmonic summed
+    constexpr int NP = 2 + M;  // parameters of the largest model
+    const float period = a.feat ? g.periods[lt.curve] : g.periods[lt.trial / g.nd];
+    const float pdt = a.feat ? g.pdts[lt.curve] : g.pdts[lt.trial % g.nd];
+    const FoldTrial f = make_fold_trial(period, pdt);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+
+    // Index 0 of each array is unused (kept so indices read as harmonics).
+    double C[M + 1], S[M + 1], PC[K + 1], PS[K + 1], YC[K + 1], YS[K + 1];
+#pragma unroll
+    for (int m = 0; m <= M; ++m) C[m] = S[m] = 0.0;
+#pragma unroll
+    for (int h = 0; h <= K; ++h) PC[h] = PS[h] = YC[h] = YS[h] = 0.0;
+
+    if constexpr (K > 0) {
+        scan_records<HAS_PDT, 4>(a.pt + p0, n, f, [&](float ph, const MhfPoint& r) {
+            // phi = 2 pi phase (mhf.rs:100): sin/cos(pi * 2phase), 2phase exact.
+            double c[M + 1], s[M + 1];
+            sincospi(2.0 * static_cast<double>(ph), &s[1], &c[1]);
+            c[0] = 1.0;
+            s[0] = 0.0;
+            const double two_c1 = 2.0 * c[1];
+#pragma unroll
+            for (int m = 2; m <= M; ++m) {  // mhf.rs:118-128
+                c[m] = __builtin_fma(two_c1, c[m - 1], -c[m - 2]);
+                s[m] = __builtin_fma(two_c1, s[m - 1], -s[m - 2]);
+            }
+#pragma unroll
+            for (int m = 1; m <= M; ++m) {
+                C[m] = __builtin_fma(r.w, c[m], C[m]);
+                S[m] = __builtin_fma(r.w, s[m], S[m]);
+            }
+#pragma unroll
+            for (int h = 1; h <= K; ++h) {
+                PC[h] = __builtin_fma(r.wx, c[h], PC[h]);
+                PS[h] = __builtin_fma(r.wx, s[h], PS[h]);
+                YC[h] = __builtin_fma(r.wy, c[h], YC[h]);
+                YS[h] = __builtin_fma(r.wy, s[h], YS[h]);
+            }
+        });
+    }
+
+    const double* u = a.sums + lt.curve * kMhfSums;
+    const double sw = u[0], swx = u[1], swxx = u[2], swy = u[3], swxy = u[4], ln_n = u[5];
+    // Normal matrix entry (i >= j) and right-hand side; columns 0 = offset,
+    // 1 = slope, 2h = cos h phi, 2h+1 = sin h phi (mhf.rs:103-116).
+    auto A = [&](int i, int j) -> double {
+        if (i < 2) return i == 0 ? sw : (j == 0 ? swx : swxx);
+        const int hi = i >> 1;
+        const bool si = i & 1;
+        if (j == 0) return si ? S[hi] : C[hi];
+        if (j == 1) return si ? PS[hi] : PC[hi];
+        const int hj = j >> 1;
+        const bool sj = j & 1;
+        const int d = hi - hj;  // >= 0
+        const double cd = d == 0 ? sw : C[d];
+        const double sd = d == 0 ? 0.0 : S[d];
+        if (!si && !sj) return 0.5 * (cd + C[hi + hj]);
+        if (si && sj) return 0.5 * (cd - C[hi + hj]);
+        if (si) return 0.5 * (S[hi + hj] + sd);  // sin hi cos hj
+        return 0.5 * (S[hi + hj] - sd);          // cos hi sin hj
+    };
+    auto B = [&](int i) -> double {
+        if (i < 2) return i == 0 ? swy : swxy;
+        return (i & 1) ? YS[i >> 1] : YC[i >> 1];
+    };
+
+    // Cholesky, row by row as fourier.rs:30-47 (each entry's sum in ascending
+    // k), with the forward substitution of fourier.rs:49-53 per row.
+    double L[NP * (NP + 1) / 2];
+    double rd[NP], z[NP];
+    int bad = NP;  // first pivot that is not positive
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double v = A(i, j);
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                v = __builtin_fma(-L[i * (i + 1) / 2 + k], L[j * (j + 1) / 2 + k], v);
+            if (j < i) {
+                L[i * (i + 1) / 2 + j] = v * rd[j];
+            } else {
+                bad = (!(v > 0.0) && bad == NP) ? i : bad;
+                const double d = sqrt(v);
+                L[i * (i + 1) / 2 + i] = d;
+                rd[i] = 1.0 / d;
+            }
+        }
+        double v = B(i);
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = __builtin_fma(-L[i * (i + 1) / 2 + k], z[k], v);
+        z[i] = v * rd[i];
+    }
+
+    // mhf.rs:192-225.  Model k needs n > 2k + 2 points and a factor through
+    // pivot 2k+1; without model 0 (or for an invalid period) every score is 0.
+    const bool valid = period > 0.0f && isfinite(period) && n >= 3 && bad >= 2;
+    float delta[K + 1];
+    delta[0] = 0.0f;
+    double best = 0.0, q = 0.0;
+    int best_k = 0;
+#pragma unroll
+    for (int k = 1; k <= K; ++k) {
+        q = __builtin_fma(z[2 * k], z[2 * k], q);
+        q = __builtin_fma(z[2 * k + 1], z[2 * k + 1], q);
+        const bool ok = valid && bad >= 2 * k + 2 && n > 2 * k + 2;
+        double d = q - static_cast<double>(2 * k) * ln_n;
+        d = (ok && d > 0.0) ? d : 0.0;  // f64::max(0.0) maps NaN to 0
+        delta[k] = static_cast<float>(d);
+        if (d > best) {
+            best = d;
+            best_k = k;
+        }
+    }
+
+    if (!lt.active) return;
+    if (a.feat) {
+        float* o = a.feat + lt.curve * (K + 2);
+#pragma unroll
+        for (int k = 0; k <= K; ++k) o[k] = delta[k];
+        o[K + 1] = static_cast<float>(best_k);
+    } else {
+        g.out[lt.curve * g.T + lt.trial] = static_cast<float>(best);
+    }
+}
+
+template <bool HAS_PDT, int K>
+__global__ __launch_bounds__(kWave) void mhf_kernel(GridArgs g, MhfArgs a) {
+    mhf_block<HAS_PDT, K>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+}
+
+template <bool HAS_PDT, int K>
+__global__ __launch_bounds__(kWave) void mhf_persistent(GridArgs g, MhfArgs a, WorkQueue q) {
+    int64_t seen = 0;
+    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
+        for (int64_t it = cl.begin; it < cl.end; ++it)
+            mhf_block<HAS_PDT, K>(g, a, queue_trial(q, g, it));
+}
+
+// ---------------------------------------------------------------------------
 // BLS — rust/src/bls.rs:20-134.
 // LDS: [slot 0..nb][2][64] f32 (W, Y).  Binning adds bin k into slot k+1; the
 // prefix pass then turns slot k into prefix[k] in place (bls.rs:85-91).  The
@@ -1408,6 +1635,30 @@ hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hip
         else a.fast ? launch_grid_kernel(ls_kernel<false, true>, g, a, 0, s)
                     : launch_grid_kernel(ls_kernel<false, false>, g, a, 0, s);
     }
+    return hipGetLastError();
+}
+
+template <bool P, int K>
+void mhf_g2(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s) {
+    if (q) launch_persistent(mhf_persistent<P, K>, g, a, 0, *q, s);
+    else launch_grid_kernel(mhf_kernel<P, K>, g, a, 0, s);
+}
+
+template <bool P>
+void mhf_g(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s) {
+    switch (a.max_k) {
+        case 0: mhf_g2<P, 0>(g, a, q, s); break;
+        case 1: mhf_g2<P, 1>(g, a, q, s); break;
+        case 2: mhf_g2<P, 2>(g, a, q, s); break;
+        case 3: mhf_g2<P, 3>(g, a, q, s); break;
+        case 4: mhf_g2<P, 4>(g, a, q, s); break;
+        default: mhf_g2<P, kMhfMaxK>(g, a, q, s); break;
+    }
+}
+
+hipError_t launch_mhf(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s) {
+    if (a.max_k < 0 || a.max_k > kMhfMaxK) return hipErrorInvalidValue;
+    g.has_pdt ? mhf_g<true>(g, a, q, s) : mhf_g<false>(g, a, q, s);
     return hipGetLastError();
 }
 

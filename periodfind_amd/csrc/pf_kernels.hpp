@@ -89,6 +89,28 @@ struct MfArgs {
     float* feat;                // features: [n_curves][kMfFeatures]
 };
 
+// Multi-Harmonic Fourier (rust/src/mhf.rs).  Per point, in f64: the time,
+// the weight w = 1/(e*e) (0 where e <= 0, mhf.rs:257-266), and w*x, w*y with
+// x = f32(t - tmin) the slope column and y the magnitude (mhf.rs:105-106),
+// built once per curve set.  Per curve: the trial-independent sums of the
+// offset/slope block and ln(n) (kMhfSums doubles: sum w, sum w*x,
+// sum (w*x)*x, sum w*y, sum (w*x)*y, ln n).
+constexpr int kMhfMaxK = 5;
+constexpr int kMhfSums = 8;
+struct MhfPoint {
+    double t, w, wx, wy;
+};
+struct MhfArgs {
+    const MhfPoint* pt;     // [npts]
+    const double* sums;     // [n_curves][kMhfSums]
+    int max_k;              // harmonics 0..kMhfMaxK
+    // Per-harmonic mode: one trial per curve at (periods[c], pdts[c]);
+    // feat[c][max_k + 2] = dBIC for k = 0..max_k, then the best k.  Else null.
+    float* feat;
+};
+hipError_t launch_mhf_prep(const int64_t* off, const float* t, const float* y, const float* e,
+                           MhfPoint* pt, double* sums, int64_t n_curves, hipStream_t s);
+
 struct FpwArgs {
     const float* w;         // [npts] 1/(e*e)
     const float* wy;        // [npts] w*y
@@ -116,6 +138,7 @@ hipError_t launch_vn_lut(const int64_t* off, const int64_t* lut_off, float* lut,
 hipError_t launch_aov(const GridArgs& g, const AovArgs& a, size_t lds, int grp,
                       const WorkQueue* q, hipStream_t s);
 hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hipStream_t s);
+hipError_t launch_mhf(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s);
 hipError_t launch_fpw(const GridArgs& g, const FpwArgs& a, size_t lds, int grp,
                       const WorkQueue* q, hipStream_t s);
 hipError_t launch_bls(const GridArgs& g, const BlsArgs& a, size_t lds, int grp,

@@ -233,7 +233,82 @@ __global__ __launch_bounds__(kBlock) void vn_lut_kernel(const int64_t* __restric
     }
 }
 
+// Multi-Harmonic Fourier inputs of one curve per workgroup (rust/src/mhf.rs):
+// tmin as f32::min folds it (NaN skipped, mhf.rs:94), then per point the
+// record {t, w, w*x, w*y}: w = 1/(e*e) in f64, 0 unless e > 0 (mhf.rs:257-266,
+// e == NULL means ones), x = f32(t - tmin) (mhf.rs:106), y widened; and the
+// curve's offset/slope sums (the trial-independent entries of mhf.rs:129-145,
+// each product formed as there) and ln n (mhf.rs:269).  The sums are reduced
+// as a tree, not serially: they agree with the reference's to rounding.
+__global__ __launch_bounds__(kBlock) void mhf_prep_kernel(const int64_t* __restrict__ off,
+                                                          const float* __restrict__ t,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ e,
+                                                          MhfPoint* __restrict__ pt,
+                                                          double* __restrict__ sums) {
+    constexpr int kWaves = kBlock / 64;
+    constexpr int kS = 5;
+    __shared__ float w_min[kWaves];
+    __shared__ double w_sum[kS][kWaves];
+    const int64_t c = blockIdx.x;
+    const int64_t a = off[c], n = off[c + 1] - a;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float mn = INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) mn = fminf(mn, t[a + i]);
+#pragma unroll
+    for (int o2 = 32; o2 >= 1; o2 >>= 1) mn = fminf(mn, __shfl_xor(mn, o2));
+    if (lane == 0) w_min[wv] = mn;
+    __syncthreads();
+    float tmin = w_min[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) tmin = fminf(tmin, w_min[w]);
+
+    double acc[kS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+        const float ti = t[a + i];
+        const double ee = e ? static_cast<double>(e[a + i]) : 1.0;
+        const double w = ee > 0.0 ? 1.0 / (ee * ee) : 0.0;
+        const double x = static_cast<double>(ti - tmin);
+        const double yy = static_cast<double>(y[a + i]);
+        const double wx = w * x, wy = w * yy;
+        pt[a + i] = MhfPoint{static_cast<double>(ti), w, wx, wy};
+        acc[0] += w;
+        acc[1] += wx;
+        acc[2] += wx * x;
+        acc[3] += wy;
+        acc[4] += wx * yy;
+    }
+#pragma unroll
+    for (int k = 0; k < kS; ++k) {
+#pragma unroll
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) acc[k] += __shfl_xor(acc[k], o2);
+        if (lane == 0) w_sum[k][wv] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* o = sums + c * kMhfSums;
+#pragma unroll
+        for (int k = 0; k < kS; ++k) {
+            double s = w_sum[k][0];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) s += w_sum[k][w];
+            o[k] = s;
+        }
+        o[5] = log(static_cast<double>(n));
+        o[6] = 0.0;
+        o[7] = 0.0;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_mhf_prep(const int64_t* off, const float* t, const float* y, const float* e,
+                           MhfPoint* pt, double* sums, int64_t n_curves, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mhf_prep_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock), 0, s,
+                       off, t, y, e, pt, sums);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
     if (n <= 0 || n > 64) return hipErrorInvalidValue;

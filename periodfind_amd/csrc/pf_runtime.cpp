@@ -113,6 +113,8 @@ struct CurveSet {
     DevBuf vn_lut, vn_lut_off;  // VN emission tables, for extents product vn_ext
     int vn_ext = -1;
     std::vector<int64_t> vn_lut_off_h;  // host copy (source of the async upload)
+    DevBuf mhf_pt, mhf_sums;  // MHF point records and per-curve sums
+    bool mhf_ok = false;
     // Last periodogram.
     DevBuf out;
     int64_t out_T = 0;
@@ -132,7 +134,7 @@ struct CurveSet {
         graph_key.clear();
     }
     void invalidate() {
-        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = order_ok = false;
+        t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = mhf_ok = order_ok = false;
         vn_ext = -1;
         ce_nmag = -1;
         out_T = 0;
@@ -142,15 +144,15 @@ struct CurveSet {
     }
     // Drop what was built from the magnitudes (a new preparation mode).
     void invalidate_mags() {
-        aov_ok = fpw_ok = bls_ok = mf_ok = false;
+        aov_ok = fpw_ok = bls_ok = mf_ok = mhf_ok = false;
         ce_nmag = -1;
         drop_graph();
     }
     void release() {
         drop_graph();
         DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &out,
-                          &order, &counter};
+                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &mhf_pt,
+                          &mhf_sums, &out, &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
                 (void)hipFree(b->p);
@@ -212,6 +214,7 @@ struct Params {
     float qmin, qmax;
     int mag_prep;  // PF_MAG_*
     int bandwidth, margin;  // VN
+    int max_harmonics;      // MHF
 };
 
 // --- Launch tuning -----------------------------------------------------------
@@ -245,8 +248,8 @@ std::vector<int64_t> tune_edges(const pf_ctx* ctx, int algo) {
 }
 
 constexpr int kMaxTuneBuckets = 32;
-const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls", "mf", "vn"};
-constexpr int kNumAlgos = 7;
+const char* const kAlgoNames[] = {"ce", "aov", "ls", "fpw", "bls", "mf", "vn", "mhf"};
+constexpr int kNumAlgos = 8;
 
 int validate_tune(const pf_tune_entry* t, int64_t n) {
     if (n < 0 || (n > 0 && !t)) return fail(PF_E_ARG, "tuning table pointer/size invalid");
@@ -274,7 +277,7 @@ int validate_tune(const pf_tune_entry* t, int64_t n) {
 }
 
 // Text table: one entry per line, "<algo> <min_len> <group> <claim>",
-// algo in {ce, aov, ls, fpw, bls, mf, vn}; '#' starts a comment.
+// algo in {ce, aov, ls, fpw, bls, mf, vn, mhf}; '#' starts a comment.
 int parse_tune_file(const char* path, std::vector<pf_tune_entry>* out) {
     std::ifstream f(path);
     if (!f) return fail(PF_E_ARG, "cannot open tuning table %s", path);
@@ -360,6 +363,9 @@ int check_params(const Params& p, size_t lds_max) {
             if (pf::lds_ce(p.n_phase, p.n_mag) > lds_max)
                 return fail(PF_E_LIMIT, "VN histogram %dx%d exceeds LDS (%zu B per wave)",
                             p.n_phase, p.n_mag, lds_max);
+            break;
+        case PF_ALGO_MHF:
+            if (p.max_harmonics < 0) return fail(PF_E_ARG, "MHF needs max_harmonics >= 0");
             break;
         case PF_ALGO_MF:
             if (p.n_bins < 1) return fail(PF_E_ARG, "MF needs num_bins >= 1");
@@ -658,6 +664,11 @@ int ensure_mf_templates(pf_ctx* ctx, int nb) {
     return PF_OK;
 }
 
+// Harmonics of MHF's largest model: max_harmonics.min(5) (mhf.rs:238).
+int mhf_max_k(int max_harmonics) {
+    return max_harmonics < pf::kMhfMaxK ? max_harmonics : pf::kMhfMaxK;
+}
+
 // The magnitudes a call evaluates: as uploaded, or prepared on the device
 // (prepare() must have run for p).
 const float* mags(CurveSet& cs, const Params& p) {
@@ -683,7 +694,8 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
         cs.derived_mode = p.mag_prep;
     }
     const float* y = mags(cs, p);
-    if (!cs.t64_ok || (has_pdt && !cs.tt_ok)) {
+    // (MHF reads times from its own point records.)
+    if (p.algo != PF_ALGO_MHF && (!cs.t64_ok || (has_pdt && !cs.tt_ok))) {
         PF_TRY(ensure(cs.t64, n * sizeof(double)));
         if (has_pdt) PF_TRY(ensure(cs.tt, n * sizeof(double)));
         PF_HIP(pf::launch_widen_times(as<float>(cs.t), as<double>(cs.t64),
@@ -753,6 +765,15 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
                 PF_HIP(pf::launch_bls_prepare(off, y, as<float>(cs.w), as<float>(cs.bls_mean),
                                               as<float>(cs.bls_yw), cs.n_curves, cs.max_len, s));
                 cs.bls_ok = true;
+            }
+            break;
+        case PF_ALGO_MHF:
+            if (!cs.mhf_ok) {
+                PF_TRY(ensure(cs.mhf_pt, n * sizeof(pf::MhfPoint)));
+                PF_TRY(ensure(cs.mhf_sums, cs.n_curves * pf::kMhfSums * sizeof(double)));
+                PF_HIP(pf::launch_mhf_prep(off, as<float>(cs.t), y, e, as<pf::MhfPoint>(cs.mhf_pt),
+                                           as<double>(cs.mhf_sums), cs.n_curves, s));
+                cs.mhf_ok = true;
             }
             break;
         default:
@@ -850,6 +871,12 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
                          as<float>(ctx->mf_tmpl), p.n_bins, nullptr, nullptr};
             le = pf::launch_mf(g, a, pf::lds_mf(p.n_bins), group, q, s);
+            break;
+        }
+        case PF_ALGO_MHF: {
+            pf::MhfArgs a{as<pf::MhfPoint>(cs.mhf_pt), as<double>(cs.mhf_sums),
+                          mhf_max_k(p.max_harmonics), nullptr};
+            le = pf::launch_mhf(g, a, q, s);
             break;
         }
         case PF_ALGO_BLS: {
@@ -958,6 +985,7 @@ Params to_params(const pf_params* p) {
     q.mag_prep = p->mag_prep;
     q.bandwidth = p->bandwidth;
     q.margin = p->margin;
+    q.max_harmonics = p->max_harmonics;
     return q;
 }
 
@@ -1139,7 +1167,9 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
         reinterpret_cast<uint64_t>(cs.mf_wyy.p), reinterpret_cast<uint64_t>(ctx->mf_tmpl.p),
         reinterpret_cast<uint64_t>(cs.vn_lut.p), reinterpret_cast<uint64_t>(cs.vn_lut_off.p),
-        static_cast<uint64_t>(p.bandwidth), static_cast<uint64_t>(p.margin)};
+        static_cast<uint64_t>(p.bandwidth), static_cast<uint64_t>(p.margin),
+        reinterpret_cast<uint64_t>(cs.mhf_pt.p), reinterpret_cast<uint64_t>(cs.mhf_sums.p),
+        static_cast<uint64_t>(mhf_max_k(p.max_harmonics))};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
         key.push_back(static_cast<uint64_t>(b.r1));
@@ -1491,6 +1521,62 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
     PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const hipError_t le = pf::launch_mf(g, a, pf::lds_mf(num_bins), 1, nullptr, ctx->stream);
     if (le != hipSuccess) return fail(PF_E_HIP, "MF features launch failed: %s", hipGetErrorString(le));
+    PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    PF_HIP(hipMemcpyAsync(out, ctx->outbuf.p, ob, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+int pf_mhf_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                const float* y, const float* err, const float* periods, int64_t n_periods,
+                const float* period_dts, int64_t n_pdts, int max_harmonics, float* out) {
+    Params p{PF_ALGO_MHF, 0, 0, 1, 1, 0, 0.0f, 0.0f};
+    p.max_harmonics = max_harmonics;
+    return grid_host(ctx, p, n_curves, offsets, t, y, err, periods, n_periods, period_dts,
+                     n_pdts, out);
+}
+
+int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                 const float* y, const float* err, const float* periods,
+                 const float* period_dts, int max_harmonics, float* out) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    Params p{PF_ALGO_MHF, 0, 0, 1, 1, 0, 0.0f, 0.0f};
+    p.max_harmonics = max_harmonics;
+    PF_TRY(check_params(p, ctx->lds_max));
+    if (n_curves > 0 && (!periods || !out)) return fail(PF_E_ARG, "periods/out is NULL");
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
+    CurveSet& cs = ctx->scratch;
+    if (n_curves == 0) return finish(ctx);
+    // One trial per curve: curve c is folded at (periods[c], period_dts[c]).
+    std::vector<float> zeros;
+    if (!period_dts) {
+        zeros.assign(static_cast<size_t>(n_curves), 0.0f);
+        period_dts = zeros.data();
+    }
+    PF_TRY(upload_grid(ctx, periods, n_curves, period_dts, n_curves));
+    PF_TRY(prepare(ctx, cs, p, false));
+    const int K = mhf_max_k(max_harmonics);
+    const size_t ob = static_cast<size_t>(n_curves) * (K + 2) * sizeof(float);
+    PF_TRY(ensure(ctx->outbuf, ob));
+    pf::GridArgs g;
+    g.off = as<int64_t>(cs.off);
+    g.t64 = as<double>(cs.t64);
+    g.tt = as<double>(cs.t64);
+    g.periods = as<float>(ctx->periods);
+    g.pdts = as<float>(ctx->pdts);
+    g.nd = 1;
+    g.T = 1;
+    g.tblocks = 1;
+    g.n_blocks = static_cast<uint64_t>(n_curves);
+    g.out = nullptr;
+    g.has_pdt = ctx->has_pdt ? 1 : 0;
+    if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
+    pf::MhfArgs a{as<pf::MhfPoint>(cs.mhf_pt), as<double>(cs.mhf_sums), K,
+                  as<float>(ctx->outbuf)};
+    PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const hipError_t le = pf::launch_mhf(g, a, nullptr, ctx->stream);
+    if (le != hipSuccess) return fail(PF_E_HIP, "MHF per-k launch failed: %s", hipGetErrorString(le));
     PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     PF_HIP(hipMemcpyAsync(out, ctx->outbuf.p, ob, hipMemcpyDeviceToHost, ctx->stream));

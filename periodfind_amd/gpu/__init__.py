@@ -399,6 +399,74 @@ class MatchedFilter(_Algorithm):
                                                 int(self.num_bins))
 
 
+class MultiHarmonicFourier(_Algorithm):
+    """Multi-Harmonic Fourier periodogram (periodfind/mhf.pyx; rust/src/mhf.rs).
+
+    Per trial period, weighted least-squares fits of offset + slope + k Fourier
+    harmonics for k = 0..max_harmonics, compared by BIC; the score is the
+    best BIC gain over the flat model (higher = more periodic).  Computed in
+    FP64.  errs default to ones; magnitudes are used as given
+    (normalize/center are accepted and ignored, as in the reference)."""
+
+    _algo = "mhf"
+
+    def __init__(self, max_harmonics=5, device=0):
+        if not 1 <= max_harmonics <= 5:
+            raise ValueError(f"max_harmonics must be between 1 and 5, got {max_harmonics}")
+        self.max_harmonics = max_harmonics
+        self.device = device
+
+    def _params(self):
+        return dict(max_harmonics=int(self.max_harmonics))
+
+    def calc(self, times, mags, periods, period_dts, errs=None, output="stats", normalize=False,
+             center=False, n_stats=1, significance_type="stdmean", n_peaks=N_PEAKS_DEFAULT,
+             min_distance=MIN_DISTANCE_DEFAULT):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        errs = validate_errs(errs, times)
+        return self._run(times, mags, errs, None, periods, period_dts, output, n_stats,
+                         significance_type, n_peaks, min_distance)
+
+    def calc_per_k(self, times, mags, errs, periods, period_dts=None):
+        """BIC gain of every model at one period per curve
+        (periodfind/cpu/__init__.py:982-1049, rust/src/mhf.rs:299-334) ->
+        float32[n_curves, max_harmonics + 2]: the gain of k = 0..max_harmonics,
+        then the best k."""
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        ensure_float32(errs, "errs")
+        if len(errs) != len(times):
+            raise ValueError(
+                f"errs must have the same number of arrays as times, "
+                f"got {len(errs)} and {len(times)}")
+        for i, (e, t) in enumerate(zip(errs, times)):
+            if len(e) != len(t):
+                raise ValueError(
+                    f"errs[{i}] and times[{i}] have different lengths: {len(e)} vs {len(t)}")
+        periods = np.asarray(periods, dtype=np.float32)
+        if periods.ndim != 1 or len(periods) != len(times):
+            raise ValueError(
+                f"periods must be a 1D array with one entry per curve, "
+                f"got shape {periods.shape} for {len(times)} curves")
+        if period_dts is None:
+            period_dts = np.zeros(len(times), dtype=np.float32)
+        else:
+            period_dts = np.asarray(period_dts, dtype=np.float32)
+            if period_dts.ndim != 1 or len(period_dts) != len(times):
+                raise ValueError(
+                    f"period_dts must be a 1D array with one entry per curve, "
+                    f"got shape {period_dts.shape} for {len(times)} curves")
+        off, t = pack(times)
+        _, y = pack(mags)
+        _, e = pack(errs)
+        return context(self.device).mhf_per_k(off, t, y, e, np.ascontiguousarray(periods),
+                                               np.ascontiguousarray(period_dts),
+                                               int(self.max_harmonics))
+
+
 def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTANCE_DEFAULT,
                            use_max=True, device=0):
     """Peaks of host periodograms data[n_curves, n_periods, n_pdts] on the device
@@ -408,5 +476,5 @@ def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTA
 
 
 __all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares", "MatchedFilter",
-           "ViterbiNarrowband",
+           "ViterbiNarrowband", "MultiHarmonicFourier",
            "find_top_peaks_batched", "context"]
