@@ -1114,11 +1114,71 @@ __global__ __launch_bounds__(kWave) void mf_persistent(GridArgs g, MfArgs a, Wor
 // large chi2 values).  A pivot that is not positive fails its model and every
 // larger one (cholesky_solve's failure, fourier.rs:37-39); a NaN pivot, which
 // the reference carries into a NaN score clamped to 0, gives the same 0 here.
+//
+// sin/cos of phi per point: a table of (cos, sin)(2 pi j/256), j = 0..256, in
+// the wave's LDS (filled once per wave with the device sincospi), and the
+// angle-addition formula with Taylor polynomials for the remainder
+// |d| <= pi/256 (sin to d^7, cos to d^6: truncation below 1e-20 relative).
+// About 20 VALU operations and one LDS read per point instead of a full
+// double-precision sincos with its range reduction.
 // ---------------------------------------------------------------------------
+constexpr int kMhfTab = 257;
+constexpr size_t kMhfLds = kMhfTab * sizeof(double2);
+
+__device__ __forceinline__ void mhf_fill_table(double2* tab) {
+    for (int j = threadIdx.x; j < kMhfTab; j += kWave) {
+        double s, c;
+        sincospi(static_cast<double>(j) * (1.0 / 128.0), &s, &c);
+        tab[j] = double2{c, s};
+    }
+    __syncthreads();
+}
+
+// sin and cos of 2 pi ph for K phases ph in [0, 1], stage by stage: the K
+// table reads are issued together and the polynomials computed while they
+// are in flight.
+template <int K>
+__device__ __forceinline__ void mhf_sincos_n(const float (&ph)[K], const double2* tab,
+                                             double (&s)[K], double (&c)[K]) {
+    double d[K];
+    double2 t0[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double u = static_cast<double>(ph[k]) * 256.0;
+        const double j = __builtin_rint(u);
+        d[k] = (u - j) * (6.283185307179586476925286766559 / 256.0);
+        int ji = static_cast<int>(j);
+        ji = ji < 0 ? 0 : (ji > kMhfTab - 1 ? kMhfTab - 1 : ji);
+        t0[k] = tab[ji];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double sd[K], cd[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double d2 = d[k] * d[k];
+        double p = __builtin_fma(d2, -1.0 / 5040.0, 1.0 / 120.0);
+        p = __builtin_fma(d2, p, -1.0 / 6.0);
+        sd[k] = __builtin_fma(d[k], p * d2, d[k]);
+        double q = __builtin_fma(d2, -1.0 / 720.0, 1.0 / 24.0);
+        q = __builtin_fma(d2, q, -0.5);
+        cd[k] = __builtin_fma(d2, q, 1.0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        s[k] = __builtin_fma(t0[k].y, cd[k], t0[k].x * sd[k]);
+        c[k] = __builtin_fma(t0[k].x, cd[k], -(t0[k].y * sd[k]));
+    }
+}
 
 // Walk a curve's MHF records in order.  Records carry their own time (t*t
-// for the period derivative is formed in the kernel, exactly); chunks of C
-// records are prefetched with scalar loads one chunk ahead, as in scan_points.
+// for the period derivative is formed in the kernel, exactly).  Chunks of C
+// records are prefetched with scalar loads one chunk ahead, as in
+// scan_points, but in two steps per chunk: first the fold and the sin/cos
+// (whose table reads are LDS traffic, waited for on the same counter as
+// scalar loads, so they run while no scalar load is outstanding), then the
+// next chunk's prefetch, then the chunk's accumulation, which covers it.
+// `body(s1, c1, rec)` receives sin and cos of phi for one point.
 __device__ __forceinline__ MhfPoint load_record(const MhfPoint* p) {
     return MhfPoint{uniform_load(&p->t), uniform_load(&p->w), uniform_load(&p->wx),
                     uniform_load(&p->wy)};
@@ -1126,7 +1186,7 @@ __device__ __forceinline__ MhfPoint load_record(const MhfPoint* p) {
 
 template <bool HAS_PDT, int C, class Body>
 __device__ __forceinline__ void scan_records(const MhfPoint* __restrict__ pt, int64_t n,
-                                             const FoldTrial& f, Body body) {
+                                             const FoldTrial& f, const double2* tab, Body body) {
     struct Chunk {
         MhfPoint r[C];
     };
@@ -1134,7 +1194,7 @@ __device__ __forceinline__ void scan_records(const MhfPoint* __restrict__ pt, in
 #pragma unroll
         for (int j = 0; j < C; ++j) ch.r[j] = load_record(pt + base + j);
     };
-    auto run = [&](const Chunk& ch) {
+    auto trig = [&](const Chunk& ch, double (&s)[C], double (&c)[C]) {
         double t[C], tt[C];
         float ph[C];
 #pragma unroll
@@ -1143,54 +1203,81 @@ __device__ __forceinline__ void scan_records(const MhfPoint* __restrict__ pt, in
             tt[j] = HAS_PDT ? t[j] * t[j] : 0.0;
         }
         fold_phases<HAS_PDT, C>(t, tt, f, ph);
+        mhf_sincos_n<C>(ph, tab, s, c);
+    };
+    auto accumulate = [&](const Chunk& ch, const double (&s)[C], const double (&c)[C]) {
 #pragma unroll
-        for (int j = 0; j < C; ++j) body(ph[j], ch.r[j]);
+        for (int j = 0; j < C; ++j) body(s[j], c[j], ch.r[j]);
     };
     int64_t i = 0;
     if (n >= 2 * C) {
         const int64_t last = n - C;
         Chunk a, b;
+        double s[C], c[C];
         fetch(0, a);
+        wait_scalar_loads();
         for (; i + 2 * C <= n; i += 2 * C) {
-            wait_scalar_loads();
+            trig(a, s, c);
+            __builtin_amdgcn_sched_barrier(0);
             fetch(i + C, b);
             __builtin_amdgcn_sched_barrier(0);
-            run(a);
+            accumulate(a, s, c);
             wait_scalar_loads();
+            trig(b, s, c);
+            __builtin_amdgcn_sched_barrier(0);
             const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
             fetch(na, a);
             __builtin_amdgcn_sched_barrier(0);
-            run(b);
+            accumulate(b, s, c);
+            wait_scalar_loads();
         }
     }
     for (; i < n; ++i) {
         const MhfPoint r = load_record(pt + i);
-        body(fold_phase<HAS_PDT>(r.t, HAS_PDT ? r.t * r.t : 0.0, f), r);
+        float ph[1] = {fold_phase<HAS_PDT>(r.t, HAS_PDT ? r.t * r.t : 0.0, f)};
+        double s1[1], c1[1];
+        mhf_sincos_n<1>(ph, tab, s1, c1);
+        body(s1[0], c1[0], r);
     }
 }
 
+// The 8K harmonic sums of one trial (see above), harmonics 1-based.
+template <int K>
+struct MhfAcc {
+    static constexpr int M = 2 * K;
+    static constexpr int N = 8 * K;
+    double v[N > 0 ? N : 1];
+    __device__ __forceinline__ double& C(int m) { return v[m - 1]; }            // m = 1..2K
+    __device__ __forceinline__ double& S(int m) { return v[M + m - 1]; }
+    __device__ __forceinline__ double& PC(int h) { return v[2 * M + h - 1]; }   // h = 1..K
+    __device__ __forceinline__ double& PS(int h) { return v[2 * M + K + h - 1]; }
+    __device__ __forceinline__ double& YC(int h) { return v[2 * M + 2 * K + h - 1]; }
+    __device__ __forceinline__ double& YS(int h) { return v[2 * M + 3 * K + h - 1]; }
+};
+
+// The trial's period and period derivative (per-model mode: the curve's).
+struct MhfTrial {
+    float period, pdt;
+};
+
+__device__ __forceinline__ MhfTrial mhf_trial(const GridArgs& g, const MhfArgs& a,
+                                              const LaneTrial lt) {
+    return a.feat ? MhfTrial{g.periods[lt.curve], g.pdts[lt.curve]}
+                  : MhfTrial{g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]};
+}
+
+// Add the harmonic sums of points [0, n) of `pt` to acc, in point order.
 template <bool HAS_PDT, int K>
-__device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, const LaneTrial lt) {
-    constexpr int M = 2 * K;   // highest harmonic summed
-    constexpr int NP = 2 + M;  // parameters of the largest model
-    const float period = a.feat ? g.periods[lt.curve] : g.periods[lt.trial / g.nd];
-    const float pdt = a.feat ? g.pdts[lt.curve] : g.pdts[lt.trial % g.nd];
-    const FoldTrial f = make_fold_trial(period, pdt);
-    const int64_t p0 = g.off[lt.curve];
-    const int64_t n = g.off[lt.curve + 1] - p0;
-
-    // Index 0 of each array is unused (kept so indices read as harmonics).
-    double C[M + 1], S[M + 1], PC[K + 1], PS[K + 1], YC[K + 1], YS[K + 1];
-#pragma unroll
-    for (int m = 0; m <= M; ++m) C[m] = S[m] = 0.0;
-#pragma unroll
-    for (int h = 0; h <= K; ++h) PC[h] = PS[h] = YC[h] = YS[h] = 0.0;
-
+__device__ __forceinline__ void mhf_accumulate(const MhfPoint* pt, int64_t n, const MhfTrial tr,
+                                               const double2* tab, MhfAcc<K>& acc) {
+    constexpr int M = 2 * K;
     if constexpr (K > 0) {
-        scan_records<HAS_PDT, 4>(a.pt + p0, n, f, [&](float ph, const MhfPoint& r) {
-            // phi = 2 pi phase (mhf.rs:100): sin/cos(pi * 2phase), 2phase exact.
+        const FoldTrial f = make_fold_trial(tr.period, tr.pdt);
+        scan_records<HAS_PDT, 4>(pt, n, f, tab, [&](double s1, double c1, const MhfPoint& r) {
+            // phi = 2 pi phase (mhf.rs:100).
             double c[M + 1], s[M + 1];
-            sincospi(2.0 * static_cast<double>(ph), &s[1], &c[1]);
+            c[1] = c1;
+            s[1] = s1;
             c[0] = 1.0;
             s[0] = 0.0;
             const double two_c1 = 2.0 * c[1];
@@ -1201,19 +1288,27 @@ __device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, c
             }
 #pragma unroll
             for (int m = 1; m <= M; ++m) {
-                C[m] = __builtin_fma(r.w, c[m], C[m]);
-                S[m] = __builtin_fma(r.w, s[m], S[m]);
+                acc.C(m) = __builtin_fma(r.w, c[m], acc.C(m));
+                acc.S(m) = __builtin_fma(r.w, s[m], acc.S(m));
             }
 #pragma unroll
             for (int h = 1; h <= K; ++h) {
-                PC[h] = __builtin_fma(r.wx, c[h], PC[h]);
-                PS[h] = __builtin_fma(r.wx, s[h], PS[h]);
-                YC[h] = __builtin_fma(r.wy, c[h], YC[h]);
-                YS[h] = __builtin_fma(r.wy, s[h], YS[h]);
+                acc.PC(h) = __builtin_fma(r.wx, c[h], acc.PC(h));
+                acc.PS(h) = __builtin_fma(r.wx, s[h], acc.PS(h));
+                acc.YC(h) = __builtin_fma(r.wy, c[h], acc.YC(h));
+                acc.YS(h) = __builtin_fma(r.wy, s[h], acc.YS(h));
             }
         });
     }
+}
 
+// Normal equations from the sums, one Cholesky, every model's BIC gain
+// (mhf.rs:147-225); writes the lane's output.
+template <int K>
+__device__ __forceinline__ void mhf_finish(const GridArgs& g, const MhfArgs& a, const LaneTrial lt,
+                                           const MhfTrial tr, MhfAcc<K>& acc) {
+    constexpr int NP = 2 + 2 * K;  // parameters of the largest model
+    const int64_t n = g.off[lt.curve + 1] - g.off[lt.curve];
     const double* u = a.sums + lt.curve * kMhfSums;
     const double sw = u[0], swx = u[1], swxx = u[2], swy = u[3], swxy = u[4], ln_n = u[5];
     // Normal matrix entry (i >= j) and right-hand side; columns 0 = offset,
@@ -1222,21 +1317,21 @@ __device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, c
         if (i < 2) return i == 0 ? sw : (j == 0 ? swx : swxx);
         const int hi = i >> 1;
         const bool si = i & 1;
-        if (j == 0) return si ? S[hi] : C[hi];
-        if (j == 1) return si ? PS[hi] : PC[hi];
+        if (j == 0) return si ? acc.S(hi) : acc.C(hi);
+        if (j == 1) return si ? acc.PS(hi) : acc.PC(hi);
         const int hj = j >> 1;
         const bool sj = j & 1;
         const int d = hi - hj;  // >= 0
-        const double cd = d == 0 ? sw : C[d];
-        const double sd = d == 0 ? 0.0 : S[d];
-        if (!si && !sj) return 0.5 * (cd + C[hi + hj]);
-        if (si && sj) return 0.5 * (cd - C[hi + hj]);
-        if (si) return 0.5 * (S[hi + hj] + sd);  // sin hi cos hj
-        return 0.5 * (S[hi + hj] - sd);          // cos hi sin hj
+        const double cd = d == 0 ? sw : acc.C(d);
+        const double sd = d == 0 ? 0.0 : acc.S(d);
+        if (!si && !sj) return 0.5 * (cd + acc.C(hi + hj));
+        if (si && sj) return 0.5 * (cd - acc.C(hi + hj));
+        if (si) return 0.5 * (acc.S(hi + hj) + sd);  // sin hi cos hj
+        return 0.5 * (acc.S(hi + hj) - sd);          // cos hi sin hj
     };
     auto B = [&](int i) -> double {
         if (i < 2) return i == 0 ? swy : swxy;
-        return (i & 1) ? YS[i >> 1] : YC[i >> 1];
+        return (i & 1) ? acc.YS(i >> 1) : acc.YC(i >> 1);
     };
 
     // Cholesky, row by row as fourier.rs:30-47 (each entry's sum in ascending
@@ -1269,7 +1364,7 @@ __device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, c
 
     // mhf.rs:192-225.  Model k needs n > 2k + 2 points and a factor through
     // pivot 2k+1; without model 0 (or for an invalid period) every score is 0.
-    const bool valid = period > 0.0f && isfinite(period) && n >= 3 && bad >= 2;
+    const bool valid = tr.period > 0.0f && isfinite(tr.period) && n >= 3 && bad >= 2;
     float delta[K + 1];
     delta[0] = 0.0f;
     double best = 0.0, q = 0.0;
@@ -1300,16 +1395,93 @@ __device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, c
 }
 
 template <bool HAS_PDT, int K>
-__global__ __launch_bounds__(kWave) void mhf_kernel(GridArgs g, MhfArgs a) {
-    mhf_block<HAS_PDT, K>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+__device__ __forceinline__ void mhf_block(const GridArgs& g, const MhfArgs& a, const LaneTrial lt,
+                                          const double2* tab) {
+    const MhfTrial tr = mhf_trial(g, a, lt);
+    const int64_t p0 = g.off[lt.curve];
+    MhfAcc<K> acc;
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k) acc.v[k] = 0.0;
+    mhf_accumulate<HAS_PDT, K>(a.pt + p0, g.off[lt.curve + 1] - p0, tr, tab, acc);
+    mhf_finish<K>(g, a, lt, tr, acc);
 }
 
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, int K>
+__global__ __launch_bounds__(kWave) void mhf_kernel(GridArgs g, MhfArgs a) {
+    extern __shared__ double2 tab[];
+    if constexpr (K > 0) mhf_fill_table(tab);
+    mhf_block<HAS_PDT, K>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T), tab);
+}
+
+// Split launch, part 1: one workgroup per (curve, 64-trial block, point
+// slice); slice s of a curve of n points covers [s*len, min(n, (s+1)*len))
+// with len = ceil(n / split).  Writes the slice's sums to
+// part[((s * n_curves + curve) * 8K + k) * tblocks*64 + trial slot].
+// g.n_blocks counts (curve, trial block) pairs in both kernels; the split
+// kernel's grid is g.n_blocks * split workgroups.
+template <bool HAS_PDT, int K>
+__global__ __launch_bounds__(kWave) void mhf_split_kernel(GridArgs g, MhfArgs a) {
+    extern __shared__ double2 tab[];
+    mhf_fill_table(tab);
+    const uint64_t L = xcd_remap(blockIdx.x, g.n_blocks * static_cast<uint64_t>(a.split));
+    const int64_t sp = static_cast<int64_t>(L % static_cast<uint64_t>(a.split));
+    const int64_t pair = static_cast<int64_t>(L / static_cast<uint64_t>(a.split));
+    LaneTrial lt;
+    lt.curve = pair / g.tblocks;
+    const int64_t slot = (pair % g.tblocks) * kWave + threadIdx.x;  // raw trial index
+    lt.active = slot < g.T;
+    lt.trial = lt.active ? slot : g.T - 1;
+    const MhfTrial tr = mhf_trial(g, a, lt);
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    const int64_t len = (n + a.split - 1) / a.split;
+    const int64_t b = sp * len < n ? sp * len : n;
+    const int64_t e = b + len < n ? b + len : n;
+    MhfAcc<K> acc;
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k) acc.v[k] = 0.0;
+    mhf_accumulate<HAS_PDT, K>(a.pt + p0 + b, e - b, tr, tab, acc);
+    const int64_t ts = g.tblocks * kWave;
+    const int64_t n_curves = static_cast<int64_t>(g.n_blocks) / g.tblocks;
+    double* o = a.part + (sp * n_curves + lt.curve) * MhfAcc<K>::N * ts + slot;
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k) o[k * ts] = acc.v[k];
+}
+
+// Split launch, part 2: one workgroup per (curve, 64-trial block); adds the
+// slices' sums in slice order and finishes as mhf_block.
+template <int K>
+__global__ __launch_bounds__(kWave) void mhf_finish_kernel(GridArgs g, MhfArgs a) {
+    const uint64_t L = xcd_remap(blockIdx.x, g.n_blocks);
+    LaneTrial lt;
+    lt.curve = static_cast<int64_t>(L / static_cast<uint64_t>(g.tblocks));
+    const int64_t slot = static_cast<int64_t>(L % static_cast<uint64_t>(g.tblocks)) * kWave +
+                         threadIdx.x;
+    lt.active = slot < g.T;
+    lt.trial = lt.active ? slot : g.T - 1;
+    const int64_t ts = g.tblocks * kWave;
+    const int64_t n_curves = static_cast<int64_t>(g.n_blocks) / g.tblocks;
+    MhfAcc<K> acc;
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k) acc.v[k] = 0.0;
+    for (int sp = 0; sp < a.split; ++sp) {
+        const double* p = a.part + (sp * n_curves + lt.curve) * MhfAcc<K>::N * ts + slot;
+#pragma unroll
+        for (int k = 0; k < MhfAcc<K>::N; ++k) acc.v[k] += p[k * ts];
+    }
+    mhf_finish<K>(g, a, lt, mhf_trial(g, a, lt), acc);
+}
+
+// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
 template <bool HAS_PDT, int K>
 __global__ __launch_bounds__(kWave) void mhf_persistent(GridArgs g, MhfArgs a, WorkQueue q) {
+    extern __shared__ double2 tab[];
+    if constexpr (K > 0) mhf_fill_table(tab);
     int64_t seen = 0;
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it)
-            mhf_block<HAS_PDT, K>(g, a, queue_trial(q, g, it));
+            mhf_block<HAS_PDT, K>(g, a, queue_trial(q, g, it), tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -1640,8 +1812,16 @@ hipError_t launch_ls(const GridArgs& g, const LsArgs& a, const WorkQueue* q, hip
 
 template <bool P, int K>
 void mhf_g2(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s) {
-    if (q) launch_persistent(mhf_persistent<P, K>, g, a, 0, *q, s);
-    else launch_grid_kernel(mhf_kernel<P, K>, g, a, 0, s);
+    const size_t lds = K > 0 ? kMhfLds : 0;
+    if (q) {
+        launch_persistent(mhf_persistent<P, K>, g, a, lds, *q, s);
+    } else if (K > 0 && a.split > 1) {
+        const uint32_t nb = static_cast<uint32_t>(g.n_blocks * static_cast<uint64_t>(a.split));
+        hipLaunchKernelGGL((mhf_split_kernel<P, K>), dim3(nb), dim3(kWave), lds, s, g, a);
+        hipLaunchKernelGGL((mhf_finish_kernel<K>), grid_of(g), dim3(kWave), 0, s, g, a);
+    } else {
+        launch_grid_kernel(mhf_kernel<P, K>, g, a, lds, s);
+    }
 }
 
 template <bool P>
@@ -1658,6 +1838,8 @@ void mhf_g(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t 
 
 hipError_t launch_mhf(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, hipStream_t s) {
     if (a.max_k < 0 || a.max_k > kMhfMaxK) return hipErrorInvalidValue;
+    if (a.split > 1 && (q || !a.part || g.n_blocks * static_cast<uint64_t>(a.split) > 0xffffffffull))
+        return hipErrorInvalidValue;
     g.has_pdt ? mhf_g<true>(g, a, q, s) : mhf_g<false>(g, a, q, s);
     return hipGetLastError();
 }

@@ -107,7 +107,15 @@ struct MhfArgs {
     // Per-harmonic mode: one trial per curve at (periods[c], pdts[c]);
     // feat[c][max_k + 2] = dBIC for k = 0..max_k, then the best k.  Else null.
     float* feat;
+    // Point slices per (curve, trial block) of a plain launch (1: one kernel).
+    // With split > 1, partial sums go through part, of mhf_part_doubles().
+    int split;
+    double* part;
 };
+// Doubles of the partial-sum workspace of a split launch.
+inline size_t mhf_part_doubles(int split, int64_t n_curves, int64_t tblocks, int max_k) {
+    return static_cast<size_t>(split) * n_curves * (8 * max_k) * tblocks * 64;
+}
 hipError_t launch_mhf_prep(const int64_t* off, const float* t, const float* y, const float* e,
                            MhfPoint* pt, double* sums, int64_t n_curves, hipStream_t s);
 

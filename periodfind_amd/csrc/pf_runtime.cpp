@@ -115,6 +115,7 @@ struct CurveSet {
     std::vector<int64_t> vn_lut_off_h;  // host copy (source of the async upload)
     DevBuf mhf_pt, mhf_sums;  // MHF point records and per-curve sums
     bool mhf_ok = false;
+    DevBuf mhf_part;          // MHF partial sums of split launches
     // Last periodogram.
     DevBuf out;
     int64_t out_T = 0;
@@ -152,7 +153,7 @@ struct CurveSet {
         drop_graph();
         DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
                           &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &mhf_pt,
-                          &mhf_sums, &out, &order, &counter};
+                          &mhf_sums, &mhf_part, &out, &order, &counter};
         for (DevBuf* b : bufs)
             if (b->p) {
                 (void)hipFree(b->p);
@@ -172,6 +173,7 @@ struct pf_ctx {
     bool timed = false;
     float last_ms = 0.0f;
     size_t lds_max = 65536;
+    int n_cu = 256;  // compute units of the device
     int rmw_group = 2;  // LDS read-modify-write group size of the f32-binned kernels
     int ce_group = 0;   // CE: 0 = LDS u32 atomics, else read-modify-write group size
     // Per-(statistic, length bucket) launch choices; empty = the defaults above.
@@ -669,6 +671,39 @@ int mhf_max_k(int max_harmonics) {
     return max_harmonics < pf::kMhfMaxK ? max_harmonics : pf::kMhfMaxK;
 }
 
+// MHF kernel arguments for a plain launch over cs with `tblocks` trial
+// blocks per curve.  When the (curve, trial block) pairs are too few to give
+// every SIMD several equal work units, each pair's points are cut into
+// slices (at least 256 points each) until there are about 8 units per SIMD;
+// the slices' partial sums go through a workspace (kept under 1 GiB) and a
+// finishing kernel.  Results do not depend on the split but for the order in
+// which partial sums are added.
+int mhf_args(pf_ctx* ctx, CurveSet& cs, int max_k, int64_t tblocks, float* feat,
+             pf::MhfArgs* a) {
+    a->pt = as<pf::MhfPoint>(cs.mhf_pt);
+    a->sums = as<double>(cs.mhf_sums);
+    a->max_k = max_k;
+    a->feat = feat;
+    a->split = 1;
+    a->part = nullptr;
+    if (max_k == 0 || std::getenv("PF_MHF_NO_SPLIT")) return PF_OK;
+    const int64_t pairs = cs.n_curves * tblocks;
+    const int64_t target = 8 * 4 * static_cast<int64_t>(ctx->n_cu);
+    if (pairs <= 0 || 2 * pairs >= target) return PF_OK;
+    int64_t sp = (target + pairs - 1) / pairs;
+    sp = std::min<int64_t>(sp, cs.max_len / 256);
+    sp = std::min<int64_t>(sp, 64);
+    while (sp > 1 &&
+           pf::mhf_part_doubles(static_cast<int>(sp), cs.n_curves, tblocks, max_k) * 8 > (1ull << 30))
+        --sp;
+    if (sp <= 1) return PF_OK;
+    PF_TRY(ensure(cs.mhf_part,
+                  pf::mhf_part_doubles(static_cast<int>(sp), cs.n_curves, tblocks, max_k) * 8));
+    a->split = static_cast<int>(sp);
+    a->part = as<double>(cs.mhf_part);
+    return PF_OK;
+}
+
 // The magnitudes a call evaluates: as uploaded, or prepared on the device
 // (prepare() must have run for p).
 const float* mags(CurveSet& cs, const Params& p) {
@@ -808,6 +843,12 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
         return fail(PF_E_LIMIT, "grid of %llu workgroups exceeds one launch",
                     static_cast<unsigned long long>(g.n_blocks));
 
+    // MHF's launch shape (and any workspace) is settled before the timed region.
+    pf::MhfArgs mhf{as<pf::MhfPoint>(cs.mhf_pt), as<double>(cs.mhf_sums),
+                    mhf_max_k(p.max_harmonics), nullptr, 1, nullptr};
+    if (p.algo == PF_ALGO_MHF && !q)  // persistent (possibly captured): one kernel, no workspace
+        PF_TRY(mhf_args(ctx, cs, mhf_max_k(p.max_harmonics), g.tblocks, nullptr, &mhf));
+
     if (timed) PF_HIP(hipEventRecord(ctx->ev0, s));
     hipError_t le = hipSuccess;
     switch (p.algo) {
@@ -873,12 +914,9 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             le = pf::launch_mf(g, a, pf::lds_mf(p.n_bins), group, q, s);
             break;
         }
-        case PF_ALGO_MHF: {
-            pf::MhfArgs a{as<pf::MhfPoint>(cs.mhf_pt), as<double>(cs.mhf_sums),
-                          mhf_max_k(p.max_harmonics), nullptr};
-            le = pf::launch_mhf(g, a, q, s);
+        case PF_ALGO_MHF:
+            le = pf::launch_mhf(g, mhf, q, s);
             break;
-        }
         case PF_ALGO_BLS: {
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
@@ -1388,6 +1426,10 @@ int pf_ctx_create(pf_ctx** out, int device) {
         const int v = std::atoi(g);
         ctx->ce_group = (v >= 4) ? 4 : (v == 2 ? 2 : (v == 1 ? 1 : 0));
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        ctx->n_cu = cus;
     ctx->arch = device_arch(device);
     {
         const char* env = std::getenv("PF_TUNE_TABLE");
@@ -1572,8 +1614,8 @@ int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const fl
     g.out = nullptr;
     g.has_pdt = ctx->has_pdt ? 1 : 0;
     if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
-    pf::MhfArgs a{as<pf::MhfPoint>(cs.mhf_pt), as<double>(cs.mhf_sums), K,
-                  as<float>(ctx->outbuf)};
+    pf::MhfArgs a{};
+    PF_TRY(mhf_args(ctx, cs, K, 1, as<float>(ctx->outbuf), &a));
     PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const hipError_t le = pf::launch_mhf(g, a, nullptr, ctx->stream);
     if (le != hipSuccess) return fail(PF_E_HIP, "MHF per-k launch failed: %s", hipGetErrorString(le));

@@ -190,3 +190,19 @@ def test_negative_max_harmonics_is_an_error(ctx):
     t, m, e = _curves([50], 76)
     with pytest.raises(capi.PfError):
         ctx.grid("mhf", t, m, PERIODS[:4], np.zeros(1, np.float32), errs=e, max_harmonics=-1)
+
+
+def test_point_slices_agree_with_one_pass(ctx, monkeypatch):
+    """Plain launches cut the points of each (curve, trial block) into slices
+    when there are few of them; the slices' sums are added in slice order, so
+    only the order of additions differs from a single pass."""
+    t, m, e = _curves([20000, 3000, 700], 77)
+    P = np.linspace(0.5, 9.0, 100).astype(np.float32)
+    D = np.array([0.0, 1e-5], np.float32)
+    split = ctx.grid("mhf", t, m, P, D, errs=e, max_harmonics=5)
+    pk_split = ctx.mhf_per_k(t, m, e, P[:3], D[[0, 1, 0]], max_harmonics=5)
+    monkeypatch.setenv("PF_MHF_NO_SPLIT", "1")
+    whole = ctx.grid("mhf", t, m, P, D, errs=e, max_harmonics=5)
+    pk_whole = ctx.mhf_per_k(t, m, e, P[:3], D[[0, 1, 0]], max_harmonics=5)
+    np.testing.assert_allclose(split, whole, rtol=1e-7, atol=1e-6)
+    np.testing.assert_allclose(pk_split, pk_whole, rtol=1e-7, atol=1e-6)
