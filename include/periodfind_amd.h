@@ -189,6 +189,16 @@ int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const fl
                  const float* y, const float* err, const float* periods,
                  const float* period_dts, int max_harmonics, float* out);
 
+/* calc_fourier_batched (rust/src/lib.rs:331-366, fourier.rs:68-253;
+ * FourierDecomposition.calc, periodfind/cpu/__init__.py:1195-1247): each
+ * curve at its own period (periods[n_curves]) -> out[n_curves][14] = power,
+ * BIC, offset, slope, A1, B1, ..., A5, B5 of the BIC-chosen model among 0..5
+ * harmonics (NaN rows for fewer than 3 points, a period that is not finite
+ * and positive, or a singular flat model).  err == NULL means all ones.
+ * Computed in FP64. */
+int pf_fourier_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                        const float* y, const float* err, const float* periods, float* out);
+
 /* ------------------------------------------------------------------------ */
 /* Device top-N peaks.  Replaces the CPU-only fused entry points             */
 /* calc_{ce,aov,ls,fpw,bls}_peaks_batched (rust/src/lib.rs:559-754) and the  */

@@ -103,6 +103,10 @@ def lib():
         L.or_mhf_per_k.restype = _int
         L.or_mhf_per_k.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _f32p, _int,
                                    _f32p, _int]
+        L.or_fourier_batched.restype = _int
+        L.or_fourier_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _f32p, _int]
+        L.or_cholesky_solve.restype = _int
+        L.or_cholesky_solve.argtypes = [_f64p, _f64p, _int]
         L.or_vn_concentration.restype = _flt
         L.or_vn_concentration.argtypes = [_f32p, _int, _int, _int, _int]
         L.or_lsd_tau_power.restype = ctypes.c_double
@@ -280,6 +284,27 @@ def mhf_per_k(times, mags, errs, periods, period_dts=None, max_harmonics=5, thre
     if rc != 0:
         raise ValueError("max_harmonics must be >= 0")
     return out
+
+
+def fourier_features(times, mags, errs, periods, threads=0):
+    """calc_fourier_batched (rust/src/lib.rs:331-366, fourier.rs:68-253): one
+    period per curve -> float32[C, 14] = [power, BIC, offset, slope, A1, B1,
+    ..., A5, B5]; NaN rows for degenerate input."""
+    off, t = pack(times)
+    _, m = pack(mags)
+    _, e = pack(errs)
+    P = np.ascontiguousarray(periods, dtype=np.float32)
+    out = np.zeros((len(times), 14), np.float32)
+    lib().or_fourier_batched(len(times), off, t, m, e, P, out.reshape(-1), threads)
+    return out
+
+
+def cholesky_solve(a, b):
+    """fourier.rs:23-61 on one system -> (ok, x)."""
+    a = np.array(a, dtype=np.float64).reshape(-1).copy()
+    x = np.array(b, dtype=np.float64).copy()
+    ok = lib().or_cholesky_solve(a, x, x.size)
+    return bool(ok), x
 
 
 def bls_duration_range(n_bins, qmin, qmax):

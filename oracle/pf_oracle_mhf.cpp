@@ -1,5 +1,6 @@
 // CPU oracle: Multi-Harmonic Fourier (rust/src/mhf.rs, with cholesky_solve
-// of rust/src/fourier.rs:23-61).
+// of rust/src/fourier.rs:23-61) and the Fourier decomposition features
+// (rust/src/fourier.rs:68-230).
 //
 // TEST INFRASTRUCTURE ONLY, like pf_oracle.cpp: a restatement of the Rust
 // CPU backend for checking the HIP path; only tests/, __graft_entry__.smoke()
@@ -168,9 +169,101 @@ void prepare(const float* m, const float* e, int64_t n, std::vector<double>& y,
 
 int n_threads(int requested) { return requested > 0 ? requested : omp_get_max_threads(); }
 
+// fourier.rs:68-230 (calc_fourier_single): 14 features at one period.
+void fourier_single(const float* times, const float* mags, const float* errs, int64_t n,
+                    float period, float* out, std::vector<double>& xc) {
+    constexpr int kF = 14;
+    for (int j = 0; j < kF; ++j) out[j] = NAN;
+    if (n < 3 || period <= 0.0f || !std::isfinite(period)) return;
+    float tmin = INFINITY;
+    for (int64_t i = 0; i < n; ++i) tmin = std::fmin(tmin, times[i]);
+    std::vector<double> w(n), y(n);
+    xc.assign(static_cast<size_t>(kMaxParams) * n, 0.0);
+    auto X = [&](int c, int64_t d) -> double& { return xc[static_cast<size_t>(c) * n + d]; };
+    for (int64_t i = 0; i < n; ++i) {
+        const double e64 = static_cast<double>(errs[i]);
+        w[i] = e64 > 0.0 ? 1.0 / (e64 * e64) : 0.0;
+        y[i] = static_cast<double>(mags[i]);
+        X(0, i) = 1.0;
+        X(1, i) = static_cast<double>(times[i] - tmin);
+        const double phi = 2.0 * M_PI * static_cast<double>(times[i]) / static_cast<double>(period);
+        for (int h = 1; h <= kMaxK; ++h) {
+            X(2 * h, i) = std::cos(static_cast<double>(h) * phi);
+            X(2 * h + 1, i) = std::sin(static_cast<double>(h) * phi);
+        }
+    }
+    const double ln_n = std::log(static_cast<double>(n));
+    double chi2[kMaxK + 1];
+    double params[kMaxK + 1][kMaxParams] = {};
+    bool ok[kMaxK + 1] = {};
+    for (int k = 0; k <= kMaxK; ++k) {
+        const int npk = 2 + 2 * k;
+        double a[kMaxParams * kMaxParams], b[kMaxParams];
+        for (int cj = 0; cj < npk; ++cj) {
+            for (int ci = 0; ci <= cj; ++ci) {
+                double s = 0.0;
+                for (int64_t d = 0; d < n; ++d) s += w[d] * X(ci, d) * X(cj, d);
+                a[cj * npk + ci] = s;
+                a[ci * npk + cj] = s;
+            }
+            double s = 0.0;
+            for (int64_t d = 0; d < n; ++d) s += w[d] * X(cj, d) * y[d];
+            b[cj] = s;
+        }
+        if (!cholesky_solve(a, b, npk)) {
+            chi2[k] = INFINITY;
+            continue;
+        }
+        ok[k] = true;
+        for (int j = 0; j < npk; ++j) params[k][j] = b[j];
+        double c2 = 0.0;
+        for (int64_t d = 0; d < n; ++d) {
+            double pred = 0.0;
+            for (int j = 0; j < npk; ++j) pred += b[j] * X(j, d);
+            const double resid = y[d] - pred;
+            c2 += w[d] * resid * resid;
+        }
+        chi2[k] = c2;
+    }
+    if (!ok[0]) return;
+    int best = 0;
+    double best_bic = INFINITY;
+    for (int k = 0; k <= kMaxK; ++k) {
+        if (!ok[k]) continue;
+        const double bic = chi2[k] + ln_n * static_cast<double>(2 + 2 * k);
+        if (bic < best_bic) {
+            best_bic = bic;
+            best = k;
+        }
+    }
+    const double power = chi2[0] > 0.0 ? (chi2[0] - chi2[best]) / chi2[0] : 0.0;
+    out[0] = static_cast<float>(power);
+    out[1] = static_cast<float>(best_bic);
+    for (int j = 0; j < kMaxParams; ++j) out[2 + j] = static_cast<float>(params[best][j]);
+}
+
 }  // namespace
 
 extern "C" {
+
+// calc_fourier_batch (fourier.rs:236-253) over CSR curves: out[c][14].
+int or_fourier_batched(int64_t n_curves, const int64_t* off, const float* t, const float* m,
+                       const float* e, const float* periods, float* out, int threads) {
+#pragma omp parallel num_threads(n_threads(threads))
+    {
+        std::vector<double> xc;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t c = 0; c < n_curves; ++c) {
+            const int64_t a = off[c];
+            fourier_single(t + a, m + a, e + a, off[c + 1] - a, periods[c], out + c * 14, xc);
+        }
+    }
+    return 0;
+}
+
+// cholesky_solve (fourier.rs:23-61) on one system, for its known answers:
+// a[n*n] row-major, b[n] -> x; returns 1 on success.
+int or_cholesky_solve(double* a, double* b, int n) { return cholesky_solve(a, b, n) ? 1 : 0; }
 
 // calc_mhf (mhf.rs:234-291) over CSR curves: out[c][period][pdt] = best dBIC.
 int or_mhf_batched(int64_t n_curves, const int64_t* off, const float* t, const float* m,

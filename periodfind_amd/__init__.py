@@ -20,7 +20,7 @@ from periodfind_amd.results import Periodogram, Statistics
 __all__ = [
     "set_device", "get_device", "ConditionalEntropy", "AOV", "LombScargle", "FPW",
     "BoxLeastSquares", "MatchedFilter", "ViterbiNarrowband", "MultiHarmonicFourier",
-    "Statistics", "Periodogram",
+    "FourierDecomposition", "Statistics", "Periodogram",
 ]
 
 _default_device = None  # None: auto-detect
@@ -113,3 +113,8 @@ def ViterbiNarrowband(**kwargs):
 def MultiHarmonicFourier(**kwargs):
     """Multi-Harmonic Fourier periodogram on the resolved device."""
     return _make("MultiHarmonicFourier", kwargs)
+
+
+def FourierDecomposition(**kwargs):
+    """Fourier decomposition features at given periods on the resolved device."""
+    return _make("FourierDecomposition", kwargs)

@@ -78,6 +78,9 @@ cdef extern from "periodfind_amd.h":
     int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
                      const float* y, const float* err, const float* periods,
                      const float* period_dts, int max_harmonics, float* out) nogil
+    int pf_fourier_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets,
+                            const float* t, const float* y, const float* err,
+                            const float* periods, float* out) nogil
     int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves,
                  const int64_t* offsets, const float* t, const float* y, const float* err,
                  const float* periods, int64_t n_periods, const float* period_dts,
@@ -497,6 +500,25 @@ cdef class Context:
         with self.lock:
             with nogil:
                 rc = pf_mhf_per_k(self.ctx, C, o, tp, yp, ep, pp, dp, max_harmonics, op)
+        if rc != 0:
+            _raise(rc)
+        return out
+
+    def fourier_features(self, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
+                         cnp.ndarray periods):
+        """Fourier decomposition features, curve c at periods[c] -> float32[C, 14]."""
+        cdef int64_t C = off.shape[0] - 1
+        out = np.zeros((C, 14), dtype=np.float32)
+        cdef float* op = <float*> cnp.PyArray_DATA(out)
+        cdef const int64_t* o = <const int64_t*> cnp.PyArray_DATA(off)
+        cdef const float* tp = _fptr(t)
+        cdef const float* yp = _fptr(y)
+        cdef const float* ep = _fptr(err) if err is not None else NULL
+        cdef const float* pp = _fptr(periods)
+        cdef int rc
+        with self.lock:
+            with nogil:
+                rc = pf_fourier_features(self.ctx, C, o, tp, yp, ep, pp, op)
         if rc != 0:
             _raise(rc)
         return out

@@ -101,6 +101,8 @@ SIGNATURES = {
                            _i64, _int, _f32p]),
     "pf_mhf_per_k": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p,
                             _f32p_or_null, _int, _f32p]),
+    "pf_fourier_features": (_int, [_vp, _i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p,
+                                   _f32p]),
     "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
                         _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
     "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
@@ -299,6 +301,16 @@ class Context:
         out = np.zeros((len(times), max(0, min(max_harmonics, 5)) + 2), dtype=np.float32)
         check(lib().pf_mhf_per_k(self.h, len(times), off, t, y, e, P, D, max_harmonics,
                                  out.reshape(-1)))
+        return out
+
+    def fourier_features(self, times, mags, errs, periods):
+        """Fourier decomposition features at one period per curve -> float32[C, 14]."""
+        off, t = pack(times)
+        _, y = pack(mags)
+        e = pack(errs)[1] if errs is not None else None
+        P = np.ascontiguousarray(periods, dtype=np.float32)
+        out = np.zeros((len(times), 14), dtype=np.float32)
+        check(lib().pf_fourier_features(self.h, len(times), off, t, y, e, P, out.reshape(-1)))
         return out
 
     def peaks(self, algo, times, mags, periods, period_dts, errs=None, n_peaks=32,

@@ -1171,6 +1171,26 @@ __device__ __forceinline__ void mhf_sincos_n(const float (&ph)[K], const double2
     }
 }
 
+// sin and cos of 2 pi r for an f64 revolution count r in [0, 1] (the table
+// and polynomials of mhf_sincos_n).
+__device__ __forceinline__ void mhf_sincos_rev(double r, const double2* tab, double& s, double& c) {
+    const double u = r * 256.0;  // exact
+    const double j = __builtin_rint(u);
+    const double d = (u - j) * (6.283185307179586476925286766559 / 256.0);
+    int ji = static_cast<int>(j);
+    ji = ji < 0 ? 0 : (ji > kMhfTab - 1 ? kMhfTab - 1 : ji);
+    const double2 t0 = tab[ji];
+    const double d2 = d * d;
+    double p = __builtin_fma(d2, -1.0 / 5040.0, 1.0 / 120.0);
+    p = __builtin_fma(d2, p, -1.0 / 6.0);
+    const double sd = __builtin_fma(d, p * d2, d);
+    double q = __builtin_fma(d2, -1.0 / 720.0, 1.0 / 24.0);
+    q = __builtin_fma(d2, q, -0.5);
+    const double cd = __builtin_fma(d2, q, 1.0);
+    s = __builtin_fma(t0.y, cd, t0.x * sd);
+    c = __builtin_fma(t0.x, cd, -(t0.y * sd));
+}
+
 // Walk a curve's MHF records in order.  Records carry their own time (t*t
 // for the period derivative is formed in the kernel, exactly).  Chunks of C
 // records are prefetched with scalar loads one chunk ahead, as in
@@ -1482,6 +1502,164 @@ __global__ __launch_bounds__(kWave) void mhf_persistent(GridArgs g, MhfArgs a, W
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it)
             mhf_block<HAS_PDT, K>(g, a, queue_trial(q, g, it), tab);
+}
+
+// ---------------------------------------------------------------------------
+// Fourier decomposition features — rust/src/fourier.rs:68-230
+// (calc_fourier_single): at one period per curve, the models k = 0..5 of
+// MHF's design with phi = 2 pi t / period (not folded: the revolution count
+// t/period is formed as an exact f64 quotient and its fraction taken exactly),
+// the BIC choice among the models that factor, and 14 outputs [power, BIC,
+// offset, slope, A1, B1, ..., A5, B5] (NaN for fewer than 3 points, an
+// invalid period or a failed flat model).
+//
+// One wave per curve: lanes take points lane, lane+64, ... and accumulate the
+// 40 harmonic sums of the MHF kernel, which a butterfly then adds across the
+// wave (lane 0's order is the one written).  Unlike the reference, every
+// model comes from one Cholesky (see MHF), chi2_k = chi2_0 - sum_{2<=j<2k+2}
+// z_j^2 with chi2_0 from the curve's residual pass (clamped at 0 against
+// rounding), and the chosen model's coefficients by back substitution in its
+// leading block.  Every model is attempted whatever n is (as the reference
+// does): with fewer points than parameters its matrix is singular and whether
+// its Cholesky fails depends on rounding, in the reference too.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWave) void fourier_kernel(const int64_t* __restrict__ off,
+                                                        const MhfPoint* __restrict__ pt,
+                                                        const double* __restrict__ sums,
+                                                        const float* __restrict__ periods,
+                                                        float* __restrict__ out) {
+    constexpr int K = kMhfMaxK, M = 2 * K, NP = 2 + M;
+    extern __shared__ double2 tab[];
+    mhf_fill_table(tab);
+    const int64_t c = blockIdx.x;
+    const int64_t p0 = off[c];
+    const int64_t n = off[c + 1] - p0;
+    const float period = periods[c];
+    const double P = static_cast<double>(period);
+    const double inv_p = 1.0 / P;
+    MhfAcc<K> acc;
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k) acc.v[k] = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kWave) {
+        const MhfPoint r = pt[p0 + i];
+        // RN(t / P) by one correction step of the reciprocal (as fold_phase).
+        const double q0 = r.t * inv_p;
+        const double q = __builtin_fma(__builtin_fma(-q0, P, r.t), inv_p, q0);
+        double cs[M + 1], sn[M + 1];
+        mhf_sincos_rev(__builtin_amdgcn_fract(q), tab, sn[1], cs[1]);
+        cs[0] = 1.0;
+        sn[0] = 0.0;
+        const double two_c1 = 2.0 * cs[1];
+#pragma unroll
+        for (int m = 2; m <= M; ++m) {
+            cs[m] = __builtin_fma(two_c1, cs[m - 1], -cs[m - 2]);
+            sn[m] = __builtin_fma(two_c1, sn[m - 1], -sn[m - 2]);
+        }
+#pragma unroll
+        for (int m = 1; m <= M; ++m) {
+            acc.C(m) = __builtin_fma(r.w, cs[m], acc.C(m));
+            acc.S(m) = __builtin_fma(r.w, sn[m], acc.S(m));
+        }
+#pragma unroll
+        for (int h = 1; h <= K; ++h) {
+            acc.PC(h) = __builtin_fma(r.wx, cs[h], acc.PC(h));
+            acc.PS(h) = __builtin_fma(r.wx, sn[h], acc.PS(h));
+            acc.YC(h) = __builtin_fma(r.wy, cs[h], acc.YC(h));
+            acc.YS(h) = __builtin_fma(r.wy, sn[h], acc.YS(h));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MhfAcc<K>::N; ++k)
+#pragma unroll
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) acc.v[k] += __shfl_xor(acc.v[k], o2);
+
+    const double* u = sums + c * kMhfSums;
+    const double sw = u[0], swx = u[1], swxx = u[2], swy = u[3], swxy = u[4], ln_n = u[5];
+    const double chi2_0 = u[6];
+    auto A = [&](int i, int j) -> double {
+        if (i < 2) return i == 0 ? sw : (j == 0 ? swx : swxx);
+        const int hi = i >> 1;
+        const bool si = i & 1;
+        if (j == 0) return si ? acc.S(hi) : acc.C(hi);
+        if (j == 1) return si ? acc.PS(hi) : acc.PC(hi);
+        const int hj = j >> 1;
+        const bool sj = j & 1;
+        const int d = hi - hj;
+        const double cd = d == 0 ? sw : acc.C(d);
+        const double sd = d == 0 ? 0.0 : acc.S(d);
+        if (!si && !sj) return 0.5 * (cd + acc.C(hi + hj));
+        if (si && sj) return 0.5 * (cd - acc.C(hi + hj));
+        if (si) return 0.5 * (acc.S(hi + hj) + sd);
+        return 0.5 * (acc.S(hi + hj) - sd);
+    };
+    auto B = [&](int i) -> double {
+        if (i < 2) return i == 0 ? swy : swxy;
+        return (i & 1) ? acc.YS(i >> 1) : acc.YC(i >> 1);
+    };
+    double L[NP * (NP + 1) / 2];
+    double rd[NP], z[NP];
+    int bad = NP;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double v = A(i, j);
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                v = __builtin_fma(-L[i * (i + 1) / 2 + k], L[j * (j + 1) / 2 + k], v);
+            if (j < i) {
+                L[i * (i + 1) / 2 + j] = v * rd[j];
+            } else {
+                bad = (v <= 0.0 && bad == NP) ? i : bad;  // a NaN pivot carries on
+                const double d = sqrt(v);
+                L[i * (i + 1) / 2 + i] = d;
+                rd[i] = 1.0 / d;
+            }
+        }
+        double v = B(i);
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = __builtin_fma(-L[i * (i + 1) / 2 + k], z[k], v);
+        z[i] = v * rd[i];
+    }
+    if (threadIdx.x != 0) return;
+    float* o = out + c * kFourierFeatures;
+    if (n < 3 || !(period > 0.0f) || !isfinite(period) || bad < 2) {
+        for (int j = 0; j < kFourierFeatures; ++j) o[j] = __builtin_nanf("");
+        return;
+    }
+    // BIC choice (fourier.rs:184-196): the first minimum, from +inf, over the
+    // models that factor (a NaN BIC never wins).
+    int best = 0;
+    double best_bic = INFINITY, best_gain = 0.0, q = 0.0;
+    if (chi2_0 + ln_n * 2.0 < best_bic) best_bic = chi2_0 + ln_n * 2.0;
+#pragma unroll
+    for (int k = 1; k <= K; ++k) {
+        q = __builtin_fma(z[2 * k], z[2 * k], q);
+        q = __builtin_fma(z[2 * k + 1], z[2 * k + 1], q);
+        const double gain = q < chi2_0 ? q : chi2_0;  // chi2_0 - chi2_k, chi2_k >= 0
+        const double bic = (chi2_0 - gain) + ln_n * static_cast<double>(2 + 2 * k);
+        if (bad >= 2 * k + 2 && bic < best_bic) {
+            best_bic = bic;
+            best = k;
+            best_gain = gain;
+        }
+    }
+    // Back substitution L^T beta = z in the chosen model's leading block
+    // (fourier.rs:55-60).
+    const int np = 2 + 2 * best;
+    double beta[NP];
+#pragma unroll
+    for (int i = NP - 1; i >= 0; --i) {
+        double v = z[i];
+#pragma unroll
+        for (int k = i + 1; k < NP; ++k)
+            if (k < np) v = __builtin_fma(-L[k * (k + 1) / 2 + i], beta[k], v);
+        beta[i] = i < np ? v * rd[i] : 0.0;
+    }
+    o[0] = static_cast<float>(chi2_0 > 0.0 ? best_gain / chi2_0 : 0.0);
+    o[1] = static_cast<float>(best_bic);
+#pragma unroll
+    for (int j = 0; j < NP; ++j) o[2 + j] = static_cast<float>(beta[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1841,6 +2019,15 @@ hipError_t launch_mhf(const GridArgs& g, const MhfArgs& a, const WorkQueue* q, h
     if (a.split > 1 && (q || !a.part || g.n_blocks * static_cast<uint64_t>(a.split) > 0xffffffffull))
         return hipErrorInvalidValue;
     g.has_pdt ? mhf_g<true>(g, a, q, s) : mhf_g<false>(g, a, q, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_fourier(const int64_t* off, const MhfPoint* pt, const double* sums,
+                          const float* periods, float* out, int64_t n_curves, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    if (n_curves > 0xffffffffll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fourier_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kWave), kMhfLds,
+                       s, off, pt, sums, periods, out);
     return hipGetLastError();
 }
 

@@ -93,8 +93,8 @@ struct MfArgs {
 // the weight w = 1/(e*e) (0 where e <= 0, mhf.rs:257-266), and w*x, w*y with
 // x = f32(t - tmin) the slope column and y the magnitude (mhf.rs:105-106),
 // built once per curve set.  Per curve: the trial-independent sums of the
-// offset/slope block and ln(n) (kMhfSums doubles: sum w, sum w*x,
-// sum (w*x)*x, sum w*y, sum (w*x)*y, ln n).
+// offset/slope block, ln(n) and the offset/slope model's chi2 (kMhfSums
+// doubles: sum w, sum w*x, sum (w*x)*x, sum w*y, sum (w*x)*y, ln n, chi2_0).
 constexpr int kMhfMaxK = 5;
 constexpr int kMhfSums = 8;
 struct MhfPoint {
@@ -118,6 +118,11 @@ inline size_t mhf_part_doubles(int split, int64_t n_curves, int64_t tblocks, int
 }
 hipError_t launch_mhf_prep(const int64_t* off, const float* t, const float* y, const float* e,
                            MhfPoint* pt, double* sums, int64_t n_curves, hipStream_t s);
+// Fourier decomposition features (rust/src/fourier.rs:68-230) of every curve
+// at periods[c], from the MHF point records: out[c][kFourierFeatures].
+constexpr int kFourierFeatures = 14;
+hipError_t launch_fourier(const int64_t* off, const MhfPoint* pt, const double* sums,
+                          const float* periods, float* out, int64_t n_curves, hipStream_t s);
 
 struct FpwArgs {
     const float* w;         // [npts] 1/(e*e)

@@ -238,8 +238,11 @@ __global__ __launch_bounds__(kBlock) void vn_lut_kernel(const int64_t* __restric
 // record {t, w, w*x, w*y}: w = 1/(e*e) in f64, 0 unless e > 0 (mhf.rs:257-266,
 // e == NULL means ones), x = f32(t - tmin) (mhf.rs:106), y widened; and the
 // curve's offset/slope sums (the trial-independent entries of mhf.rs:129-145,
-// each product formed as there) and ln n (mhf.rs:269).  The sums are reduced
-// as a tree, not serially: they agree with the reference's to rounding.
+// each product formed as there), ln n (mhf.rs:269), and chi2 of the
+// offset + slope model by a residual pass (the flat model of
+// fourier.rs:159-172; NaN if that 2x2 system is not positive definite).  The
+// sums are reduced as a tree, not serially: they agree with the reference's
+// to rounding.
 __global__ __launch_bounds__(kBlock) void mhf_prep_kernel(const int64_t* __restrict__ off,
                                                           const float* __restrict__ t,
                                                           const float* __restrict__ y,
@@ -285,17 +288,51 @@ __global__ __launch_bounds__(kBlock) void mhf_prep_kernel(const int64_t* __restr
         if (lane == 0) w_sum[k][wv] = acc[k];
     }
     __syncthreads();
+    __shared__ double fit[2];
+    double tot[kS];
+#pragma unroll
+    for (int k = 0; k < kS; ++k) {
+        tot[k] = w_sum[k][0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) tot[k] += w_sum[k][w];
+    }
+    if (threadIdx.x == 0) {
+        // The 2x2 Cholesky of cholesky_solve (fourier.rs:30-60).
+        const double l00 = sqrt(tot[0]);
+        const double l10 = tot[1] / l00;
+        const double s11 = tot[2] - l10 * l10;
+        const double l11 = sqrt(s11);
+        const double z0 = tot[3] / l00;
+        const double z1 = (tot[4] - l10 * z0) / l11;
+        const double b1 = z1 / l11;
+        const double b0 = (z0 - l10 * b1) / l00;
+        const bool ok = tot[0] > 0.0 && s11 > 0.0;
+        fit[0] = ok ? b0 : __builtin_nan("");
+        fit[1] = ok ? b1 : __builtin_nan("");
+    }
+    __syncthreads();
+    const double b0 = fit[0], b1 = fit[1];
+    double c2 = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+        const MhfPoint r = pt[a + i];
+        const double x = static_cast<double>(t[a + i] - tmin);
+        const double res = static_cast<double>(y[a + i]) - (b0 + b1 * x);
+        c2 += r.w * res * res;
+    }
+#pragma unroll
+    for (int o2 = 32; o2 >= 1; o2 >>= 1) c2 += __shfl_xor(c2, o2);
+    __shared__ double w_c2[kWaves];
+    if (lane == 0) w_c2[wv] = c2;
+    __syncthreads();
     if (threadIdx.x == 0) {
         double* o = sums + c * kMhfSums;
 #pragma unroll
-        for (int k = 0; k < kS; ++k) {
-            double s = w_sum[k][0];
-#pragma unroll
-            for (int w = 1; w < kWaves; ++w) s += w_sum[k][w];
-            o[k] = s;
-        }
+        for (int k = 0; k < kS; ++k) o[k] = tot[k];
         o[5] = log(static_cast<double>(n));
-        o[6] = 0.0;
+        double s = w_c2[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += w_c2[w];
+        o[6] = s;
         o[7] = 0.0;
     }
 }

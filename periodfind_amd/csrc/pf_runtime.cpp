@@ -1625,6 +1625,34 @@ int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const fl
     return finish(ctx);
 }
 
+int pf_fourier_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
+                        const float* y, const float* err, const float* periods, float* out) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    if (n_curves > 0 && (!periods || !out)) return fail(PF_E_ARG, "periods/out is NULL");
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
+    CurveSet& cs = ctx->scratch;
+    if (n_curves == 0) return finish(ctx);
+    const float zero = 0.0f;
+    PF_TRY(upload_grid(ctx, periods, n_curves, &zero, 1));
+    Params p{PF_ALGO_MHF, 0, 0, 1, 1, 0, 0.0f, 0.0f};
+    p.max_harmonics = pf::kMhfMaxK;
+    PF_TRY(prepare(ctx, cs, p, false));  // the MHF point records and curve sums
+    const size_t ob = static_cast<size_t>(n_curves) * pf::kFourierFeatures * sizeof(float);
+    PF_TRY(ensure(ctx->outbuf, ob));
+    PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const hipError_t le =
+        pf::launch_fourier(as<int64_t>(cs.off), as<pf::MhfPoint>(cs.mhf_pt),
+                           as<double>(cs.mhf_sums), as<float>(ctx->periods),
+                           as<float>(ctx->outbuf), n_curves, ctx->stream);
+    if (le != hipSuccess)
+        return fail(PF_E_HIP, "Fourier features launch failed: %s", hipGetErrorString(le));
+    PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    PF_HIP(hipMemcpyAsync(out, ctx->outbuf.p, ob, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
 int pf_bls_grid(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const float* t,
                 const float* y, const float* err, const float* periods, int64_t n_periods,
                 const float* period_dts, int64_t n_pdts, int n_bins, float qmin, float qmax,

@@ -467,6 +467,40 @@ class MultiHarmonicFourier(_Algorithm):
                                                int(self.max_harmonics))
 
 
+class FourierDecomposition:
+    """Fourier decomposition features at given periods (the CPU class of
+    periodfind/cpu/__init__.py:1195-1247; rust/src/fourier.rs:68-253),
+    computed on the device in FP64.  Weighted least squares of offset + slope
+    + k harmonics, k = 0..5, with BIC model selection; 14 features per curve:
+    [power, BIC, offset, slope, A1, B1, A2, B2, A3, B3, A4, B4, A5, B5]."""
+
+    def __init__(self, device=0):
+        self.device = device
+
+    def calc(self, times, mags, errs, periods):
+        validate_inputs(times, mags)
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        ensure_float32(errs, "errs")
+        if len(errs) != len(times):
+            raise ValueError(
+                f"errs must have the same number of arrays as times, "
+                f"got {len(errs)} and {len(times)}")
+        for i, (e, t) in enumerate(zip(errs, times)):
+            if len(e) != len(t):
+                raise ValueError(
+                    f"errs[{i}] and times[{i}] have different lengths: {len(e)} vs {len(t)}")
+        periods = np.asarray(periods, dtype=np.float32)
+        if periods.ndim != 1 or len(periods) != len(times):
+            raise ValueError(
+                f"periods must be a 1D array with one entry per curve, "
+                f"got shape {periods.shape} for {len(times)} curves")
+        off, t = pack(times)
+        _, y = pack(mags)
+        _, e = pack(errs)
+        return context(self.device).fourier_features(off, t, y, e, np.ascontiguousarray(periods))
+
+
 def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTANCE_DEFAULT,
                            use_max=True, device=0):
     """Peaks of host periodograms data[n_curves, n_periods, n_pdts] on the device
@@ -476,5 +510,5 @@ def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTA
 
 
 __all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares", "MatchedFilter",
-           "ViterbiNarrowband", "MultiHarmonicFourier",
+           "ViterbiNarrowband", "MultiHarmonicFourier", "FourierDecomposition",
            "find_top_peaks_batched", "context"]
