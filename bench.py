@@ -52,7 +52,9 @@ FLOPS_INNER = {"ce": 10.0, "aov": 12.0, "ls": 17.0, "fpw": 11.0, "bls": 11.0,
                "vn": 10.0,
                # MHF(5), FP64: fold 6, harmonics 2..10 by recurrence (2 x 9 FMA), the
                # 8K = 40 harmonic sums (FMA); the sincos itself is not counted
-               "mhf": 6.0 + 2.0 * 18 + 2.0 * 40}
+               "mhf": 6.0 + 2.0 * 18 + 2.0 * 40,
+               # Fourier features: as MHF per point (revolution count 4 instead of the fold)
+               "fourier": 4.0 + 2.0 * 18 + 2.0 * 40}
 FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.0,
                # MF(20): 14 templates x 20 shifts x 20 bins (mul + add), 280 divisions,
                # ~12 per bin for the profile, R^2 and RMS terms
@@ -62,7 +64,10 @@ FLOPS_TRIAL = {"ce": 600.0, "aov": 125.0, "ls": 40.0, "fpw": 40.0, "bls": 12600.
                "vn": 20.0 * 19 * 20 * 6 + 400 + 400 + 3 * 400,
                # MHF(5): normal matrix assembly (78 entries), 12x12 Cholesky (~12^3/3 FMA)
                # with forward substitution, BIC of 6 models
-               "mhf": 2.0 * 78 + 2.0 * 576 + 2.0 * 78 + 30}
+               "mhf": 2.0 * 78 + 2.0 * 576 + 2.0 * 78 + 30,
+               # Fourier features (per curve): the wave reduction of 40 sums (6 x 40 adds),
+               # MHF's epilogue, the back substitution (~144)
+               "fourier": 6.0 * 40 + 2.0 * 78 + 2.0 * 576 + 2.0 * 78 + 30 + 144}
 # Fold-based statistics beside the headline five (measured by --workload siblings).
 SIBLINGS = ("mf", "vn", "mhf")
 ERR_ALGOS = ("fpw", "bls", "mf", "mhf")
@@ -369,6 +374,43 @@ def run_lsrefine(args):
                          "median_rel": float(np.median(rel)) if rel.size else None,
                          "status_mismatch": int((st[:n_or] != rst).sum()),
                          "grid_only_max_rel": float(grid_rel.max()) if rel.size else None}
+
+    # The consumer of the refined peaks: Fourier decomposition features
+    # (rust/src/fourier.rs:68-230) of every curve at its best refined period,
+    # on the raw magnitudes, FP64 on the device (pf_fourier_features).
+    print("bench: features at the refined period", file=sys.stderr, flush=True)
+    ctx2 = capi.Context(0)
+    Pbest = (1.0 / fbest).astype(np.float32)
+    for _ in range(max(1, args.warmup)):
+        ctx2.fourier_features(t, m, None, Pbest)
+    f_walls, f_ms = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        feat = ctx2.fourier_features(t, m, None, Pbest)
+        f_walls.append(time.perf_counter() - t0)
+        f_ms.append(ctx2.last_kernel_ms())
+    ctx2.close()
+    f_kt = float(np.median(f_ms)) * 1e-3
+    f_flops = C * N * FLOPS_INNER["fourier"] + C * FLOPS_TRIAL["fourier"]
+    threads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        threads = min(threads, int(omp))
+    ones = [np.ones(len(x), np.float32) for x in t]
+    t0 = time.perf_counter()
+    want = O.fourier_features(t, m, ones, Pbest, threads=threads)
+    f_cpu = time.perf_counter() - t0
+    diff = np.abs(feat[:, :2] - want[:, :2]) / np.maximum(np.abs(want[:, :2]), 1e-30)
+    out["features"] = {
+        "what": "14 Fourier features per curve at its best refined period, FP64",
+        "kernel_ms": f_kt * 1e3, "call_ms": float(np.median(f_walls)) * 1e3,
+        "curves_per_s": C / f_kt,
+        "roofline": {"bound": "valu-f64", "achieved": round(f_flops / f_kt / 1e12, 3),
+                     "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(f_flops / f_kt / 1e12 / PEAK_F64_TFLOPS, 4)},
+        "cpu_baseline": {"kind": "port", "cores": threads, "seconds": round(f_cpu, 3),
+                         "curves_per_s": C / f_cpu, "sample": f"all {C} curves"},
+        "max_rel_diff_power_bic_vs_oracle": float(np.nanmax(diff))}
 
     print("bench: autograd forward+backward", file=sys.stderr, flush=True)
     import torch
