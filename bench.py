@@ -705,8 +705,28 @@ def main():
             kms[k].append(r.step())
             walls[k].append(time.perf_counter() - tc)
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
     pts_per_step = sum(N_CURVES * n for (_, n) in sets)
+    serial = {"value": pts_per_step / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,
+              "submission": "one synchronous pf_curves_stats call per (statistic, size)"}
+
+    # The same 15 (statistic, size) evaluations submitted as one
+    # pf_curves_stats_many call per step: grid + device stats of every job on
+    # the context's side streams, one synchronisation.  Results are checked
+    # against the serial calls' below.
+    print(f"bench: timing {args.steps} steps, one call per step", file=sys.stderr, flush=True)
+    keys = list(sets)
+    sm = capi.StatsMany(ctx, [(sets[k].h, sets[k].prm, N_CURVES) for k in keys],
+                        sets[keys[0]].P, sets[keys[0]].D, n_stats=1)
+    for _ in range(max(1, args.warmup)):
+        sm.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sm.run()
+    wall = time.perf_counter() - t0
+    matches = all(np.array_equal(sm.summary[j], sets[k].summ) and
+                  np.array_equal(sm.idx[j], sets[k].idx) and
+                  np.array_equal(sm.val[j], sets[k].val) for j, k in enumerate(keys))
+    ms_per_step = wall * 1e3 / args.steps
     value = pts_per_step / (wall / args.steps)
 
     detail = {}
@@ -746,6 +766,10 @@ def main():
                      "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom),
                      "counters": measured_util(dom)},
         "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
+        "submission": "one pf_curves_stats_many call per step: 15 (statistic, size) jobs, grid "
+                      "+ device stats each, on 3 streams, one synchronisation",
+        "results_match_serial_calls": bool(matches),
+        "serial": serial,
         "kernels": detail,
     }
     for r in sets.values():

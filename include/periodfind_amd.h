@@ -323,6 +323,22 @@ int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const f
                     int64_t n_periods, const float* period_dts, int64_t n_pdts, int n_stats,
                     float* summary, int64_t* idx, float* val, int* fallback);
 
+/* Several resident (curve set, statistic) evaluations over one trial grid in
+ * one call: job j is pf_curves_stats(ctx, sets[j], &params[j], ...) with its
+ * outputs in summary[j], idx[j], val[j], fallback[j].  The jobs' grid and
+ * stats kernels run on a few default-priority streams of the context, so
+ * that short launches and the tails of long ones overlap, and the call
+ * synchronises once.  Jobs must use distinct curve sets.  kernel_ms[j] (may
+ * be NULL) is the time on job j's stream from its grid launch to the end of
+ * its stats, which overlaps other jobs.  No counterpart in the reference
+ * (each of its classes runs alone); for callers evaluating several
+ * statistics over the same curves. */
+int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
+                         const pf_params* params, const float* periods, int64_t n_periods,
+                         const float* period_dts, int64_t n_pdts, int n_stats,
+                         float* const* summary, int64_t* const* idx, float* const* val,
+                         int* const* fallback, float* kernel_ms);
+
 /* ------------------------------------------------------------------------ */
 /* Differentiable Lomb-Scargle (FP64).  No counterpart in the reference; the */
 /* power is the reference's (rust/src/ls.rs:15-79: unweighted, no floating   */

@@ -123,6 +123,10 @@ SIGNATURES = {
                         _f32p, _i64, _f32p, _i64, _int, _f32p, _i64p, _f32p, _i32p]),
     "pf_curves_stats": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
                                _int, _f32p, _i64p, _f32p, _i32p]),
+    "pf_curves_stats_many": (_int, [_vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(PfParams),
+                                    _f32p, _i64, _f32p, _i64, _int, ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    _f32p_or_null]),
     # Differentiable LS: device pointers (passed as integers) and a stream handle.
     "pf_ls_jet": (_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "pf_ls_backward": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
@@ -467,3 +471,31 @@ class CurveSet:
                                                min_distance, rtol, max_iter,
                                                *(a.reshape(-1) for a in out)))
         return out
+
+
+class StatsMany:
+    """pf_curves_stats_many over fixed jobs: (curve-set handle, PfParams,
+    n_curves) triples evaluated together on one trial grid; output arrays are
+    allocated once and refilled by each run()."""
+
+    def __init__(self, ctx, jobs, periods, period_dts, n_stats=1):
+        self.ctx = ctx
+        self.n = len(jobs)
+        self.P = np.ascontiguousarray(periods, dtype=np.float32)
+        self.D = np.ascontiguousarray(period_dts, dtype=np.float32)
+        self.n_stats = int(n_stats)
+        self.sets = (_vp * max(self.n, 1))(*[h for h, _, _ in jobs])
+        self.params = (PfParams * max(self.n, 1))(*[p for _, p, _ in jobs])
+        self.summary = [np.zeros((c, 4), np.float32) for _, _, c in jobs]
+        self.idx = [np.zeros((c, self.n_stats), np.int64) for _, _, c in jobs]
+        self.val = [np.zeros((c, self.n_stats), np.float32) for _, _, c in jobs]
+        self.fallback = [np.zeros(c, np.int32) for _, _, c in jobs]
+        ptrs = lambda arrs: (_vp * max(self.n, 1))(*[a.ctypes.data for a in arrs])
+        self._p = [ptrs(self.summary), ptrs(self.idx), ptrs(self.val), ptrs(self.fallback)]
+        self.kernel_ms = np.zeros(max(self.n, 1), np.float32)
+
+    def run(self):
+        check(lib().pf_curves_stats_many(self.ctx.h, self.n, self.sets, self.params, self.P,
+                                         self.P.size, self.D, self.D.size, self.n_stats,
+                                         *self._p, self.kernel_ms))
+        return self.kernel_ms[:self.n]

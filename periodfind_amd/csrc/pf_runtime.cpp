@@ -195,6 +195,12 @@ struct pf_ctx {
     void* in_stage[2] = {nullptr, nullptr};
     hipEvent_t in_ev[2] = {nullptr, nullptr};
     bool in_busy[2] = {false, false};
+    // Side streams of pf_curves_stats_many (default priority), created on
+    // first use, and a start/end event pair per job.
+    static constexpr int kSideStreams = 3;
+    hipStream_t side[kSideStreams] = {nullptr, nullptr, nullptr};
+    hipEvent_t join = nullptr;
+    std::vector<hipEvent_t> job_ev;
     // LS refinement: frequency grid (f64) and per-peak brackets.
     DevBuf ls_freq, ls_lo, ls_hi;
     // Matched Filter templates of mf_nb bins (rows of mf_template_stride).
@@ -822,10 +828,10 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
 // already be prepared).  `timed` brackets the launch with the context's
 // events (not allowed while the stream is being captured).
 int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd,
-                const pf::WorkQueue* q, bool timed, int group) {
+                const pf::WorkQueue* q, bool timed, int group, hipStream_t stream = nullptr) {
     const int64_t T = np * nd;
     const bool has_pdt = ctx->has_pdt;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = stream ? stream : ctx->stream;
 
     pf::GridArgs g;
     g.off = as<int64_t>(cs.off);
@@ -1468,6 +1474,11 @@ void pf_ctx_destroy(pf_ctx* ctx) {
         if (ctx->in_stage[i]) (void)hipHostFree(ctx->in_stage[i]);
         if (ctx->in_ev[i]) (void)hipEventDestroy(ctx->in_ev[i]);
     }
+    for (hipStream_t& st : ctx->side)
+        if (st) (void)hipStreamDestroy(st);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    for (hipEvent_t ev : ctx->job_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1794,6 +1805,124 @@ int pf_curves_stats(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const f
     PF_HIP(hipSetDevice(ctx->device));
     return stats_on(ctx, cv->cs, to_params(params), periods, n_periods, period_dts, n_pdts,
                     n_stats, summary, idx, val, fallback);
+}
+
+int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
+                         const pf_params* params, const float* periods, int64_t n_periods,
+                         const float* period_dts, int64_t n_pdts, int n_stats,
+                         float* const* summary, int64_t* const* idx, float* const* val,
+                         int* const* fallback, float* kernel_ms) {
+    if (!ctx) return fail(PF_E_ARG, "ctx is NULL");
+    if (n_jobs < 0) return fail(PF_E_ARG, "n_jobs < 0");
+    if (n_jobs == 0) return PF_OK;
+    if (!sets || !params || !summary || !idx || !val || !fallback)
+        return fail(PF_E_ARG, "NULL job array");
+    PF_TRY(check_grid(periods, n_periods, period_dts, n_pdts));
+    const int64_t T = n_periods * n_pdts;
+    if (T < 1 || T > pf::kStatsMaxT)
+        return fail(PF_E_LIMIT, "device stats need 1 <= n_periods*n_pdts <= %lld",
+                    static_cast<long long>(pf::kStatsMaxT));
+    if (n_stats < 1 || n_stats > T) return fail(PF_E_ARG, "n_stats must be in [1, T]");
+    if (pf::stats_lds_bytes(T) + pf::stats_lds_static_bytes() > ctx->lds_max)
+        return fail(PF_E_LIMIT, "stats exceed LDS");
+    for (int64_t j = 0; j < n_jobs; ++j) {
+        if (!sets[j]) return fail(PF_E_ARG, "job %lld: curve set is NULL", static_cast<long long>(j));
+        if (sets[j]->ctx != ctx)
+            return fail(PF_E_ARG, "job %lld: curve set belongs to another context",
+                        static_cast<long long>(j));
+        for (int64_t i = 0; i < j; ++i)
+            if (sets[i] == sets[j])
+                return fail(PF_E_ARG, "jobs %lld and %lld share a curve set",
+                            static_cast<long long>(i), static_cast<long long>(j));
+        PF_TRY(check_params(to_params(&params[j]), ctx->lds_max));
+        if (sets[j]->cs.n_curves > 0 &&
+            (!summary[j] || !idx[j] || !val[j] || !fallback[j]))
+            return fail(PF_E_ARG, "job %lld: NULL output", static_cast<long long>(j));
+    }
+    PF_HIP(hipSetDevice(ctx->device));
+    PF_TRY(upload_grid(ctx, periods, n_periods, period_dts, n_pdts));
+    // Streams and events, once per context.
+    for (hipStream_t& st : ctx->side)
+        if (!st) PF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (!ctx->join) PF_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+    while (static_cast<int64_t>(ctx->job_ev.size()) < 2 * n_jobs) {
+        hipEvent_t ev = nullptr;
+        PF_HIP(hipEventCreate(&ev));
+        ctx->job_ev.push_back(ev);
+    }
+    // Derived arrays and result buffers, on the context stream (cached
+    // after the first call).
+    OutPlan plan;
+    std::vector<size_t> o_idx(n_jobs), o_val(n_jobs), o_sum(n_jobs), o_flg(n_jobs);
+    for (int64_t j = 0; j < n_jobs; ++j) {
+        CurveSet& cs = sets[j]->cs;
+        if (cs.n_curves == 0) continue;
+        const Params p = to_params(&params[j]);
+        PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+        PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
+        cs.out_T = T;
+        const size_t C = static_cast<size_t>(cs.n_curves);
+        o_idx[j] = plan.add(idx[j], C * n_stats * sizeof(int64_t));
+        o_val[j] = plan.add(val[j], C * n_stats * sizeof(float));
+        o_sum[j] = plan.add(summary[j], C * 4 * sizeof(float));
+        o_flg[j] = plan.add(fallback[j], C * sizeof(int));
+    }
+    PF_TRY(ensure(ctx->outbuf, plan.total));
+    PF_HIP(hipEventRecord(ctx->join, ctx->stream));
+    const int K = pf_ctx::kSideStreams;
+    for (int k = 0; k < K; ++k) PF_HIP(hipStreamWaitEvent(ctx->side[k], ctx->join, 0));
+    // Streams: the context stream and the side streams (four hardware
+    // queues).  Jobs go longest first to the least-loaded stream, by an
+    // estimate of their cost (points x trials x the statistic's measured
+    // relative cost per point-trial); the estimate only orders the work.
+    hipStream_t streams[K + 1] = {ctx->stream};
+    for (int k = 0; k < K; ++k) streams[k + 1] = ctx->side[k];
+    auto weight = [](int algo) {
+        static const double w[kNumAlgos] = {1.1, 1.3, 1.0, 1.0, 2.1, 2.3, 5.2, 4.7};
+        return algo >= 0 && algo < kNumAlgos ? w[algo] : 1.0;
+    };
+    std::vector<double> cost(n_jobs);
+    std::vector<int64_t> order(n_jobs);
+    for (int64_t j = 0; j < n_jobs; ++j) {
+        order[j] = j;
+        cost[j] = static_cast<double>(sets[j]->cs.npts) * static_cast<double>(T) *
+                  weight(params[j].algo);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
+    double load[K + 1] = {};
+    for (const int64_t j : order) {
+        CurveSet& cs = sets[j]->cs;
+        int k = 0;
+        for (int i = 1; i <= K; ++i)
+            if (load[i] < load[k]) k = i;
+        load[k] += cost[j];
+        hipStream_t s = streams[k];
+        PF_HIP(hipEventRecord(ctx->job_ev[2 * j], s));
+        if (cs.n_curves > 0) {
+            const Params p = to_params(&params[j]);
+            PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, nullptr, false,
+                               tune_choice(ctx, p.algo, cs.rep_len).group, s));
+            const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
+            PF_HIP(pf::launch_stats(as<float>(cs.out), cs.n_curves, T, n_stats, use_max,
+                                    out_dev<float>(ctx, o_sum[j]), out_dev<int64_t>(ctx, o_idx[j]),
+                                    out_dev<float>(ctx, o_val[j]), out_dev<int>(ctx, o_flg[j]), s));
+        }
+        PF_HIP(hipEventRecord(ctx->job_ev[2 * j + 1], s));
+    }
+    for (int k = 0; k < K; ++k) {
+        PF_HIP(hipEventRecord(ctx->join, ctx->side[k]));
+        PF_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    }
+    ctx->timed = false;
+    PF_TRY(out_finish(ctx, plan));
+    if (kernel_ms)
+        for (int64_t j = 0; j < n_jobs; ++j) {
+            float ms = 0.0f;
+            PF_HIP(hipEventElapsedTime(&ms, ctx->job_ev[2 * j], ctx->job_ev[2 * j + 1]));
+            kernel_ms[j] = ms;
+        }
+    return PF_OK;
 }
 
 int pf_curves_peaks_ragged(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
