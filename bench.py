@@ -153,6 +153,29 @@ class Resident:
             self.h = ctypes.c_void_p()
 
 
+def run_many(args, ctx, sets, pts_per_step):
+    """The 15 (statistic, size) evaluations of a step submitted as one
+    pf_curves_stats_many call: grid + device stats of every job on the
+    context's streams, one synchronisation.  Returns (pts/s, ms/step, whether
+    every job's outputs equal those of the serial calls made before)."""
+    from periodfind_amd import capi
+
+    print(f"bench: timing {args.steps} steps, one call per step", file=sys.stderr, flush=True)
+    keys = list(sets)
+    sm = capi.StatsMany(ctx, [(sets[k].h, sets[k].prm, N_CURVES) for k in keys],
+                        sets[keys[0]].P, sets[keys[0]].D, n_stats=1)
+    for _ in range(max(1, args.warmup)):
+        sm.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sm.run()
+    wall = time.perf_counter() - t0
+    matches = all(np.array_equal(sm.summary[j], sets[k].summ) and
+                  np.array_equal(sm.idx[j], sets[k].idx) and
+                  np.array_equal(sm.val[j], sets[k].val) for j, k in enumerate(keys))
+    return pts_per_step / (wall / args.steps), wall * 1e3 / args.steps, bool(matches)
+
+
 def cpu_baseline(datasets, budget_s):
     """Oracle (C++/OpenMP restatement of the Rust path) on a bounded sample:
     the first `k` curves of each size, all five statistics."""
@@ -652,6 +675,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU-baseline budget (0 disables)")
     ap.add_argument("--no-drop-in", action="store_true")
+    ap.add_argument("--submission", choices=("many", "serial"), default="many",
+                    help="batch workload: many = one pf_curves_stats_many call per step "
+                         "(default); serial = one pf_curves_stats call per (statistic, size). "
+                         "Per-kernel times always come from the serial calls.")
     ap.add_argument("--workload", default="batch",
                     choices=["batch", "ragged", "lsrefine", "heavy", "siblings"],
                     help="batch: the headline 5-statistic workload; ragged: 10k ragged curves; "
@@ -713,23 +740,13 @@ def main():
     # pf_curves_stats_many call per step: grid + device stats of every job on
     # the context's side streams, one synchronisation.  Results are checked
     # against the serial calls' below.
-    print(f"bench: timing {args.steps} steps, one call per step", file=sys.stderr, flush=True)
-    keys = list(sets)
-    sm = capi.StatsMany(ctx, [(sets[k].h, sets[k].prm, N_CURVES) for k in keys],
-                        sets[keys[0]].P, sets[keys[0]].D, n_stats=1)
-    for _ in range(max(1, args.warmup)):
-        sm.run()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sm.run()
-    wall = time.perf_counter() - t0
-    matches = all(np.array_equal(sm.summary[j], sets[k].summ) and
-                  np.array_equal(sm.idx[j], sets[k].idx) and
-                  np.array_equal(sm.val[j], sets[k].val) for j, k in enumerate(keys))
-    ms_per_step = wall * 1e3 / args.steps
-    value = pts_per_step / (wall / args.steps)
+    if args.submission == "serial":
+        value, ms_per_step, matches = serial["value"], serial["ms_per_step"], True
+    else:
+        value, ms_per_step, matches = run_many(args, ctx, sets, pts_per_step)
 
     detail = {}
+
     best = None
     for (algo, n), v in kms.items():
         kt = float(np.mean(v)) * 1e-3
@@ -766,8 +783,9 @@ def main():
                      "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom),
                      "counters": measured_util(dom)},
         "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
-        "submission": "one pf_curves_stats_many call per step: 15 (statistic, size) jobs, grid "
-                      "+ device stats each, on 3 streams, one synchronisation",
+        "submission": ("one pf_curves_stats_many call per step: 15 (statistic, size) jobs, "
+                       "grid + device stats each, longest first over 4 streams, one "
+                       "synchronisation" if args.submission == "many" else serial["submission"]),
         "results_match_serial_calls": bool(matches),
         "serial": serial,
         "kernels": detail,
