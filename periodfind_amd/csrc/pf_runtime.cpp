@@ -1872,21 +1872,17 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
     const int K = pf_ctx::kSideStreams;
     for (int k = 0; k < K; ++k) PF_HIP(hipStreamWaitEvent(ctx->side[k], ctx->join, 0));
     // Streams: the context stream and the side streams (four hardware
-    // queues).  Jobs go longest first to the least-loaded stream, by an
-    // estimate of their cost (points x trials x the statistic's measured
-    // relative cost per point-trial); the estimate only orders the work.
+    // queues).  Jobs go longest first, by points x trials, to the
+    // least-loaded stream; jobs of equal size keep the caller's order.  (A
+    // per-statistic cost weight was measured 5% slower on the 15-job batch:
+    // it started LDS-heavy launches side by side.)
     hipStream_t streams[K + 1] = {ctx->stream};
     for (int k = 0; k < K; ++k) streams[k + 1] = ctx->side[k];
-    auto weight = [](int algo) {
-        static const double w[kNumAlgos] = {1.1, 1.3, 1.0, 1.0, 2.1, 2.3, 5.2, 4.7};
-        return algo >= 0 && algo < kNumAlgos ? w[algo] : 1.0;
-    };
     std::vector<double> cost(n_jobs);
     std::vector<int64_t> order(n_jobs);
     for (int64_t j = 0; j < n_jobs; ++j) {
         order[j] = j;
-        cost[j] = static_cast<double>(sets[j]->cs.npts) * static_cast<double>(T) *
-                  weight(params[j].algo);
+        cost[j] = static_cast<double>(sets[j]->cs.npts) * static_cast<double>(T);
     }
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
