@@ -1886,6 +1886,18 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
     }
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
+    // Until every stream has been joined, an early error return drains the
+    // side streams first, so no kernel of this call is still reading buffers
+    // that a later call may reuse.
+    struct Drain {
+        pf_ctx* ctx;
+        bool armed = true;
+        ~Drain() {
+            if (!armed) return;
+            for (hipStream_t st : ctx->side)
+                if (st) (void)hipStreamSynchronize(st);
+        }
+    } drain{ctx};
     double load[K + 1] = {};
     for (const int64_t j : order) {
         CurveSet& cs = sets[j]->cs;
@@ -1910,6 +1922,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         PF_HIP(hipEventRecord(ctx->join, ctx->side[k]));
         PF_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     }
+    drain.armed = false;  // the context stream now orders everything after this call
     ctx->timed = false;
     PF_TRY(out_finish(ctx, plan));
     if (kernel_ms)
