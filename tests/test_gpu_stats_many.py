@@ -68,3 +68,32 @@ def test_errors(ctx):
     capi.StatsMany(ctx, [], P, D).run()  # no jobs: nothing to do
     for h, _, _ in jobs:
         capi.lib().pf_curves_destroy(h)
+
+
+def test_ragged_lengths_period_derivatives_and_no_timing(ctx):
+    from bench import prep_mags, ERR_ALGOS
+
+    rng = np.random.default_rng(11)
+    lengths = [5, 64, 300, 1500, 2, 777]
+    t = [np.sort(rng.uniform(0, 500, n)).astype(np.float32) for n in lengths]
+    m = [synth._signal(rng, x.astype(np.float64), i % 3) for i, x in enumerate(t)]
+    e = [rng.uniform(0.02, 0.1, n).astype(np.float32) for n in lengths]
+    P = np.linspace(0.4, 9.0, 150).astype(np.float32)
+    D = np.array([0.0, 2e-5], np.float32)
+    jobs = []
+    for algo, p in JOBS:
+        off, tt = capi.pack(t)
+        _, y = capi.pack(prep_mags(algo, m))
+        ee = capi.pack(e)[1] if algo in ERR_ALGOS else None
+        h = ctypes.c_void_p()
+        capi.check(capi.lib().pf_curves_create(ctx.h, ctypes.byref(h), len(t), off, tt, y, ee))
+        jobs.append((h, capi.make_params(algo, **p), len(t)))
+    sm = capi.StatsMany(ctx, jobs, P, D, n_stats=2)
+    capi.check(capi.lib().pf_curves_stats_many(ctx.h, sm.n, sm.sets, sm.params, sm.P, sm.P.size,
+                                               sm.D, sm.D.size, 2, *sm._p, None))
+    for j, (h, prm, C) in enumerate(jobs):
+        want = _single(ctx, h, prm, C, P, D, 2)
+        for got, w in zip((sm.summary[j], sm.idx[j], sm.val[j], sm.fallback[j]), want):
+            np.testing.assert_array_equal(got, w, err_msg=f"job {j}")
+    for h, _, _ in jobs:
+        capi.lib().pf_curves_destroy(h)
