@@ -197,17 +197,18 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
 // barriers keep that order, so the fold covers the LDS read latency instead
 // of each group waiting on its own reads.  `prep(ph[K], rec[K], off[K],
 // val[A][K])` maps a group's phases and records to bin offsets and addends;
-// results are identical to updating one group after another.  The point
+// results are identical to updating one group after another.  CH points are
+// prefetched per chunk (a tuning knob: results do not depend on it).  The point
 // stream is read with vector loads here: they count on vmcnt, so the waits
 // on the LDS reads (lgkmcnt) do not also wait for the next chunk's prefetch,
 // as they would for scalar loads (which return out of order on lgkmcnt).
-template <bool HAS_PDT, int G, int A, class Load, class Prep>
+template <bool HAS_PDT, int G, int A, int CH = kChunk, class Load, class Prep>
 __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
                                                 const double* __restrict__ tt, int64_t n,
                                                 const FoldTrial& f, Load load, float* col,
                                                 Prep prep) {
     using Rec = decltype(load(int64_t(0)));
-    constexpr int C = HAS_PDT ? kChunk / 2 : kChunk;
+    constexpr int C = HAS_PDT ? CH / 2 : CH;
     constexpr int NG = C / G;
     static_assert(C % G == 0, "chunk must hold whole groups");
     struct Chunk {
@@ -1708,7 +1709,9 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
     constexpr uint32_t kSlotBytes = 2 * kPlaneBytes;
     float* col1 = col + 2 * kWave;  // slot 1
 
-    scan_points_rmw<HAS_PDT, G, 2>(
+    // Chunks of 16 points: measured 6% faster than 8 for BLS at 8192 points
+    // (slower for AOV and FPW, which keep the default).
+    scan_points_rmw<HAS_PDT, G, 2, 16>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; }, col1,
         [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
