@@ -165,10 +165,10 @@ def run_many(args, ctx, sets, pts_per_step):
     sm = capi.StatsMany(ctx, [(sets[k].h, sets[k].prm, N_CURVES) for k in keys],
                         sets[keys[0]].P, sets[keys[0]].D, n_stats=1)
     for _ in range(max(1, args.warmup)):
-        sm.run()
+        sm.run(timing=False)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sm.run()
+        sm.run(timing=False)
     wall = time.perf_counter() - t0
     matches = all(np.array_equal(sm.summary[j], sets[k].summ) and
                   np.array_equal(sm.idx[j], sets[k].idx) and

@@ -494,8 +494,10 @@ class StatsMany:
         self._p = [ptrs(self.summary), ptrs(self.idx), ptrs(self.val), ptrs(self.fallback)]
         self.kernel_ms = np.zeros(max(self.n, 1), np.float32)
 
-    def run(self):
+    def run(self, timing=True):
+        """One call; returns each job's stream time in ms (timing=False: no
+        per-job events, returns None)."""
         check(lib().pf_curves_stats_many(self.ctx.h, self.n, self.sets, self.params, self.P,
                                          self.P.size, self.D, self.D.size, self.n_stats,
-                                         *self._p, self.kernel_ms))
-        return self.kernel_ms[:self.n]
+                                         *self._p, self.kernel_ms if timing else None))
+        return self.kernel_ms[:self.n] if timing else None

@@ -1845,7 +1845,8 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
     for (hipStream_t& st : ctx->side)
         if (!st) PF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     if (!ctx->join) PF_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
-    while (static_cast<int64_t>(ctx->job_ev.size()) < 2 * n_jobs) {
+    const bool timing = kernel_ms != nullptr;  // per-job events only when asked for
+    while (timing && static_cast<int64_t>(ctx->job_ev.size()) < 2 * n_jobs) {
         hipEvent_t ev = nullptr;
         PF_HIP(hipEventCreate(&ev));
         ctx->job_ev.push_back(ev);
@@ -1906,7 +1907,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
             if (load[i] < load[k]) k = i;
         load[k] += cost[j];
         hipStream_t s = streams[k];
-        PF_HIP(hipEventRecord(ctx->job_ev[2 * j], s));
+        if (timing) PF_HIP(hipEventRecord(ctx->job_ev[2 * j], s));
         if (cs.n_curves > 0) {
             const Params p = to_params(&params[j]);
             PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, nullptr, false,
@@ -1916,7 +1917,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
                                     out_dev<float>(ctx, o_sum[j]), out_dev<int64_t>(ctx, o_idx[j]),
                                     out_dev<float>(ctx, o_val[j]), out_dev<int>(ctx, o_flg[j]), s));
         }
-        PF_HIP(hipEventRecord(ctx->job_ev[2 * j + 1], s));
+        if (timing) PF_HIP(hipEventRecord(ctx->job_ev[2 * j + 1], s));
     }
     for (int k = 0; k < K; ++k) {
         PF_HIP(hipEventRecord(ctx->join, ctx->side[k]));
