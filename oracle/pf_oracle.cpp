@@ -364,6 +364,9 @@ void or_bls_duration_range(int n_bins, float qmin, float qmax, int64_t* lo, int6
     const float nbf = static_cast<float>(n_bins);
     uint64_t a = f32_to_usize(std::floor(qmin * nbf));
     uint64_t b = f32_to_usize(std::ceil(qmax * nbf));
+    // A saturated a (qmin * n_bins >= 2^64) leaves Rust's nb_min..=nb_max
+    // empty; n_bins is above hi and keeps that range empty in int64.
+    a = std::min<uint64_t>(a, static_cast<uint64_t>(n_bins));
     *lo = static_cast<int64_t>(std::max<uint64_t>(a, 1));
     *hi = static_cast<int64_t>(std::min<uint64_t>(b, static_cast<uint64_t>(n_bins - 1)));
 }

@@ -1677,21 +1677,11 @@ __global__ __launch_bounds__(kWave) void fourier_kernel(const int64_t* __restric
 // loop bounds, so lanes never diverge on them.
 //
 // Exact max: the reference keeps max over fl(num/den).  Rather than one IEEE
-// division per candidate, a candidate is tested against the midpoint M
-// between the current best B and its f32 successor: fl(num/den) > B iff
-// num > M*den (or == with round-to-even going up).  num, den are f32 and M
-// has 25 significant bits, so M*den is exact in f64.  Only when a lane
-// improves is fl(num/den) computed by IEEE division.
+// division per candidate, a candidate is first tested with one f32 multiply
+// against the current best B, num >= fl(B*den), a test that every candidate
+// with fl(num/den) > B passes; only the few that pass have fl(num/den)
+// computed by IEEE division and compared with B.
 // ---------------------------------------------------------------------------
-// Successor of a finite f32 b >= 0 (best only grows from 0).
-__device__ __forceinline__ float succ_pos(float b) {
-    return __uint_as_float(__float_as_uint(b) + 1u);
-}
-
-__device__ __forceinline__ double bls_midpoint_up(float b) {
-    return 0.5 * (static_cast<double>(b) + static_cast<double>(succ_pos(b)));
-}
-
 template <bool HAS_PDT, int G>
 __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, const LaneTrial lt) {
     extern __shared__ float fl[];
@@ -1760,30 +1750,30 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
         // r > wt * 2^-150, exact in f64 (rlo); that last test only matters for a
         // window that would improve the best, so it sits on that path.
         const double rlo = static_cast<double>(wt) * 0x1p-150;
-        double mid = bls_midpoint_up(best);
         // Candidate windows [p, p+d) are examined in groups of four: their
         // sums (Rust's window sums, bls.rs:104-117) and the admission test
-        // num >= M*den are branch-free; only a group in which some lane
-        // admits a candidate takes the exact path.  Admission is a superset
-        // of fl(num/den) > best (it also admits ties and den == 0), and the
-        // exact path decides each admitted candidate, in duration order, by
-        // the IEEE quotient as Rust does.  A candidate not admitted under
-        // the group's starting best cannot beat a later, larger best.
+        // num >= fl(best*den) are branch-free; only a group in which some
+        // lane admits a candidate takes the exact path, which decides each
+        // admitted candidate, in duration order, by the IEEE quotient as
+        // Rust does.  Admission is a superset of fl(num/den) > best: that
+        // implies num/den > best (rounding is monotone and best is a float),
+        // so num > best*den exactly, and then fl(best*den) <= num because
+        // num is a float at or above best*den.  It also admits ties, den ==
+        // 0 and candidates within an ulp below best; den == inf with best ==
+        // 0 gives NaN and is not admitted, as no quotient num/inf exceeds 0.
+        // A candidate not admitted under the group's starting best cannot
+        // beat a later, larger best.
         // r in (0, wt) is min(r, wt - r) > 0: wt - r > 0 iff r < wt in f32.
         auto admit = [&](float r, float s, float& num, float& den) -> bool {
             const float t = wt - r;
             num = s * s;
             den = r * t;
-            return (fminf(r, t) > 0.0f) &
-                   (static_cast<double>(num) >= mid * static_cast<double>(den));
+            return (fminf(r, t) > 0.0f) & (num >= best * den);
         };
         auto improve = [&](bool ok, float r, float num, float den) {
             if (ok && static_cast<double>(r) > rlo) {
                 const float v = num / den;
-                if (v > best) {
-                    best = v;
-                    mid = bls_midpoint_up(best);
-                }
+                if (v > best) best = v;
             }
         };
         constexpr int kG = 4;  // candidates per group

@@ -404,6 +404,8 @@ void bls_duration_range(int nb, float qmin, float qmax, int* lo, int* hi) {
     uint64_t b = sat(std::ceil(qmax * static_cast<float>(nb)));
     if (a < 1) a = 1;
     if (b > static_cast<uint64_t>(nb - 1)) b = static_cast<uint64_t>(nb - 1);
+    // Any a above b searches nothing; nb keeps it above b and within int.
+    if (a > static_cast<uint64_t>(nb)) a = static_cast<uint64_t>(nb);
     *lo = static_cast<int>(a);
     *hi = static_cast<int>(b);
 }

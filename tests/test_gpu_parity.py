@@ -159,6 +159,10 @@ def test_grid_negative_times(ctx, algo):
     ("fpw", dict(n_bins=37)),
     ("bls", dict(n_bins=20, qmin=0.05, qmax=0.3)),
     ("bls", dict(n_bins=100, qmin=0.0, qmax=1.0)),
+    ("bls", dict(n_bins=2, qmin=0.01, qmax=1.0)),
+    # no duration to search: every statistic is 0
+    ("bls", dict(n_bins=30, qmin=1e30, qmax=0.5)),
+    ("bls", dict(n_bins=40, qmin=0.6, qmax=0.3)),
 ])
 def test_grid_params(ctx, algo, params):
     t, m, e = _curves([200, 999, 40], seed=14)

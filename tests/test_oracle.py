@@ -245,6 +245,12 @@ def test_bls_perfect_box():
 def test_bls_duration_range_and_zero_weight():
     assert O.bls_duration_range(50, 0.01, 0.5) == (1, 25)
     assert O.bls_duration_range(10, 0.0, 1.0) == (1, 9)
+    # Rust's saturating casts: a huge or infinite qmin and a NaN or negative
+    # qmax leave no duration to search (nb_min > nb_max).
+    for qmin, qmax in [(1e30, 0.5), (np.inf, 0.5), (0.01, np.nan), (0.01, -3.0), (0.6, 0.3)]:
+        lo, hi = O.bls_duration_range(50, qmin, qmax)
+        assert 1 <= lo and hi <= 49 and lo > hi
+    assert O.bls_duration_range(50, np.nan, 0.5) == (1, 25)
     t, y, _ = _curve(50, 7)
     z = O.bls_grid([t], [y], [np.full_like(y, np.inf)], PERIODS, D0)
     assert np.all(z == 0.0)
