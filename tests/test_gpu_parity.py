@@ -174,6 +174,36 @@ def test_grid_params(ctx, algo, params):
     _check(algo, got, ref)
 
 
+def _special_curves():
+    """Curves with the inputs whose results the reference defines by plain f32
+    arithmetic: a zero error (weight inf), negative errors, infinite errors
+    (weight 0), a NaN error, every error infinite (total weight 0), a NaN and
+    an infinite magnitude."""
+    t, m, e = _curves([300] * 8, seed=17)
+    e[0][10] = 0.0
+    e[1][::7] = -e[1][::7]
+    e[2][::3] = np.inf
+    e[3][5] = np.nan
+    e[4][:] = np.inf
+    m[5][40] = np.nan
+    m[6][41] = np.inf
+    m[7][::50] = -np.inf
+    return t, m, e
+
+
+@pytest.mark.parametrize("algo", ["ce", "aov", "ls", "fpw", "bls"])
+def test_grid_special_values(ctx, algo):
+    t, m, e = _special_curves()
+    if algo == "ls":  # its tolerance check needs finite values: finite mags only
+        t, m, e = t[:5], m[:5], e[:5]
+    m = _prep(algo, m)
+    P = np.linspace(0.5, 10.0, 120).astype(np.float32)
+    D = np.array([0.0, 1e-5], np.float32)
+    ref = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+    got = _gpu(ctx, algo, t, m, e, P, D, **PARAMS[algo])
+    _check(algo, got, ref)
+
+
 def test_grid_config2_sample(ctx):
     """The benchmark workload's shape (8192 points, 1000 periods) on 6 curves."""
     t, m, e, P, D = synth.config2(8192, n_curves=6)
