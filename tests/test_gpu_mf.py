@@ -63,6 +63,28 @@ def test_degenerate_curves(ctx):
     assert np.all(got[1] == 0.0) and np.all(got[2] == 0.0)
 
 
+def test_special_values(ctx):
+    """Zero, negative, infinite and NaN errors and NaN/inf magnitudes: the
+    reference's plain f32 arithmetic on them, bit for bit (grid and
+    features)."""
+    t, m, e = _curves([300] * 8, 34)
+    e[0][10] = 0.0
+    e[1][::7] = -e[1][::7]
+    e[2][::3] = np.inf
+    e[3][5] = np.nan
+    e[4][:] = np.inf
+    m[5][40] = np.nan
+    m[6][41] = np.inf
+    m[7][::50] = -np.inf
+    D = np.array([0.0, 1e-5], np.float32)
+    got = ctx.grid("mf", t, m, PERIODS[:96], D, errs=e, n_bins=20)
+    want = O.mf_grid(t, m, e, PERIODS[:96], D, 20)
+    np.testing.assert_array_equal(got, want)
+    periods = np.linspace(0.7, 9.0, len(t)).astype(np.float32)
+    np.testing.assert_array_equal(ctx.mf_features(t, m, e, periods, 20),
+                                  O.mf_features(t, m, e, periods, 20))
+
+
 def test_features_match_oracle(ctx):
     t, m, e = _curves(LENGTHS + [5000], 33)
     periods = np.random.default_rng(3).uniform(0.3, 10.0, len(t)).astype(np.float32)

@@ -70,6 +70,24 @@ def test_period_dts_and_long_curves(ctx):
     np.testing.assert_array_equal(got, want)
 
 
+def test_special_magnitudes(ctx):
+    """NaN and infinite magnitudes through VN's default normalization: the
+    reference's arithmetic on them, bit for bit."""
+    rng = np.random.default_rng(44)
+    t = [np.sort(rng.uniform(0.0, 1000.0, 300)).astype(np.float32) for _ in range(4)]
+    m = [synth._signal(rng, x.astype(np.float64), i % 3) for i, x in enumerate(t)]
+    m[0][40] = np.nan
+    m[1][41] = np.inf
+    m[2][::50] = -np.inf
+    with np.errstate(invalid="ignore", divide="ignore"):
+        m = O.prepare_magnitudes(m, False, True)
+    cfg = CONFIGS[0]
+    D = np.array([0.0, 1e-5], np.float32)
+    got = ctx.grid("vn", t, m, PERIODS[:64], D, **cfg)
+    want = O.vn_grid(t, m, PERIODS[:64], D, dlog=True, **cfg)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_against_reference_log(ctx):
     cfg = CONFIGS[0]
     t, m = _curves([300, 1000, 3000, 8192, 500, 2000], 43)
