@@ -191,18 +191,22 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
 }
 
 // scan_points for the statistics that update lane-private LDS bins by
-// read-modify-write (AOV, FPW, BLS), software-pipelined: the reads of group
+// read-modify-write (AOV, FPW, MF, BLS), software-pipelined: the reads of group
 // g are issued, then group g+1 is folded and binned (independent VALU work,
 // mostly f64), then group g is finished (added, written).  The scheduling
 // barriers keep that order, so the fold covers the LDS read latency instead
 // of each group waiting on its own reads.  `prep(ph[K], rec[K], off[K],
 // val[A][K])` maps a group's phases and records to bin offsets and addends;
 // results are identical to updating one group after another.  CH points are
-// prefetched per chunk (a tuning knob: results do not depend on it).  The point
-// stream is read with vector loads here: they count on vmcnt, so the waits
-// on the LDS reads (lgkmcnt) do not also wait for the next chunk's prefetch,
-// as they would for scalar loads (which return out of order on lgkmcnt).
-template <bool HAS_PDT, int G, int A, int CH = kChunk, class Load, class Prep>
+// prefetched per chunk (a tuning knob: results do not depend on it).  SCALAR
+// selects how the times are read (the caller's `load` reads the records the
+// same way): scalar loads deliver a point once per wave, into SGPRs, but
+// count on lgkmcnt like the LDS reads, so a wait on an LDS read also waits
+// for the next chunk's prefetch; vector loads count on vmcnt instead, at the
+// cost of one copy of the point per lane.  Measured at 8192 points: scalar
+// is 3-5% faster for AOV and FPW, vector 28% faster for BLS (16-point
+// chunks, one LDS wait per point).
+template <bool HAS_PDT, int G, int A, int CH = kChunk, bool SCALAR = true, class Load, class Prep>
 __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
                                                 const double* __restrict__ tt, int64_t n,
                                                 const FoldTrial& f, Load load, float* col,
@@ -223,8 +227,8 @@ __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
     auto fetch = [&](int64_t base, Chunk& ch) {
 #pragma unroll
         for (int j = 0; j < C; ++j) {
-            ch.t[j] = t64[base + j];
-            ch.t2[j] = HAS_PDT ? tt[base + j] : 0.0;
+            ch.t[j] = SCALAR ? uniform_load(t64 + base + j) : t64[base + j];
+            ch.t2[j] = HAS_PDT ? (SCALAR ? uniform_load(tt + base + j) : tt[base + j]) : 0.0;
             ch.r[j] = load(base + j);
         }
     };
@@ -685,7 +689,8 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
 
     if constexpr (EXT1) {
         scan_points_rmw<HAS_PDT, G, 3>(
-            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{y[i], yy[i]}; }, col,
+            g.t64 + p0, g.tt + p0, n, f,
+            [&](int64_t i) { return F2{uniform_load(y + i), uniform_load(yy + i)}; }, col,
             [&](const auto& ph, const auto& v, auto& off, auto& val) {
                 constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
@@ -849,7 +854,8 @@ __device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, c
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
 
     scan_points_rmw<HAS_PDT, G, 2>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], wy[i]}; }, col,
+        g.t64 + p0, g.tt + p0, n, f,
+        [&](int64_t i) { return F2{uniform_load(w + i), uniform_load(wy + i)}; }, col,
         [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
@@ -937,7 +943,9 @@ __device__ __forceinline__ void mf_block(const GridArgs& g, const MfArgs& a, con
     constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
 
     scan_points_rmw<HAS_PDT, G, 3>(
-        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F3{w[i], wy[i], wyy[i]}; }, col,
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) {
+            return F3{uniform_load(w + i), uniform_load(wy + i), uniform_load(wyy + i)};
+        }, col,
         [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
@@ -1701,7 +1709,7 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
 
     // Chunks of 16 points: measured 6% faster than 8 for BLS at 8192 points
     // (slower for AOV and FPW, which keep the default).
-    scan_points_rmw<HAS_PDT, G, 2, 16>(
+    scan_points_rmw<HAS_PDT, G, 2, 16, false>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; }, col1,
         [&](const auto& ph, const auto& v, auto& off, auto& val) {
             constexpr int K = group_size_v<decltype(ph)>;
