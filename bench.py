@@ -691,8 +691,9 @@ def main():
                     help="lsrefine: curves whose refinement is checked against the oracle")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(curve sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. Across GPUs "
+              "the path shards by curves (one process and context per GPU, contiguous curve "
+              "ranges balanced by points x trials, no collective).", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration > 0:
