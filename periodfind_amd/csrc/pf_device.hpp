@@ -49,13 +49,22 @@ __device__ __forceinline__ FoldTrial make_fold_trial(float period, float period_
 // for ratio < 0 (its clamp below 1.0 rounds to the same f32 1.0f that the
 // reference returns).  t and tt are wave-uniform (scalar operands); the f64
 // ops run at full rate on CDNA4.
-template <bool HAS_PDT>
+//
+// EXACT = false drops the correction step: q = RN(t_corr*RN(1/p)), within
+// about 1.5 ulp of the IEEE quotient (1e-11 of a cycle at 4e4 cycles), so
+// the f32 phase differs from the reference's by at most one f32 ulp, and
+// only at the rare ratios next to an f32 rounding boundary.  Only the
+// statistics whose parity bar is a tolerance take it (LS with the hardware
+// sin/cos).
+template <bool HAS_PDT, bool EXACT = true>
 __device__ __forceinline__ float fold_phase(double t, double tt, const FoldTrial& f) {
     double tc = t;
     if (HAS_PDT) tc = t - f.k * tt;
-    const double q0 = tc * f.inv_p;
-    const double r = __builtin_fma(-q0, f.p, tc);
-    const double q = __builtin_fma(r, f.inv_p, q0);
+    double q = tc * f.inv_p;
+    if constexpr (EXACT) {
+        const double r = __builtin_fma(-q, f.p, tc);
+        q = __builtin_fma(r, f.inv_p, q);
+    }
     return static_cast<float>(__builtin_amdgcn_fract(q));
 }
 
@@ -75,7 +84,7 @@ __device__ __forceinline__ T uniform_load(const T* p) {
 // f64 operations; interleaving K independent chains lets one wave issue
 // back to back instead of waiting on each result (the compiler otherwise
 // emits the chains one after another).  Same operations, same results.
-template <bool HAS_PDT, int K>
+template <bool HAS_PDT, int K, bool EXACT = true>
 __device__ __forceinline__ void fold_phases(const double (&t)[K], const double (&tt)[K],
                                             const FoldTrial& f, float (&ph)[K]) {
     double tc[K], q0[K], r[K];
@@ -84,12 +93,14 @@ __device__ __forceinline__ void fold_phases(const double (&t)[K], const double (
 #pragma unroll
     for (int j = 0; j < K; ++j) q0[j] = tc[j] * f.inv_p;
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (EXACT) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) r[j] = __builtin_fma(-q0[j], f.p, tc[j]);
-    __builtin_amdgcn_sched_barrier(0);
+        for (int j = 0; j < K; ++j) r[j] = __builtin_fma(-q0[j], f.p, tc[j]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int j = 0; j < K; ++j) q0[j] = __builtin_fma(r[j], f.inv_p, q0[j]);
-    __builtin_amdgcn_sched_barrier(0);
+        for (int j = 0; j < K; ++j) q0[j] = __builtin_fma(r[j], f.inv_p, q0[j]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
 #pragma unroll
     for (int j = 0; j < K; ++j) q0[j] = __builtin_amdgcn_fract(q0[j]);
     __builtin_amdgcn_sched_barrier(0);

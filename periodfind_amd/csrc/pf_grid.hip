@@ -125,7 +125,7 @@ constexpr int group_size_v = static_cast<int>(std::extent<std::remove_reference_
 // clamped to the last full chunk, so it never reads past the curve.
 // `body(ph[G], rec[G])` receives the folded phases of G consecutive points
 // (G inside full chunks, 1 for the remainder).
-template <bool HAS_PDT, int G, class Load, class Body>
+template <bool HAS_PDT, int G, bool EXACT = true, class Load, class Body>
 __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
                                             const double* __restrict__ tt, int64_t n,
                                             const FoldTrial& f, Load load, Body body) {
@@ -147,7 +147,7 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
     };
     auto run = [&](const Chunk& ch) {
         float phc[C];
-        fold_phases<HAS_PDT, C>(ch.t, ch.t2, f, phc);
+        fold_phases<HAS_PDT, C, EXACT>(ch.t, ch.t2, f, phc);
 #pragma unroll
         for (int gi = 0; gi < C; gi += G) {
             float ph[G];
@@ -184,7 +184,7 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
         }
     }
     for (; i < n; ++i) {
-        float ph[1] = {fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
+        float ph[1] = {fold_phase<HAS_PDT, EXACT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
         Rec rc[1] = {load(i)};
         body(ph, rc);
     }
@@ -781,7 +781,9 @@ __device__ __forceinline__ void ls_block(const GridArgs& g, const LsArgs& a, con
     const float two_pi = 2.0f * 3.14159274101257324f;  // 2 * (f32)pi, ls.rs:9
 
     float mc = 0.0f, ms = 0.0f, cc = 0.0f, cs = 0.0f;
-    scan_points<HAS_PDT, 4>(
+    // The hardware sin/cos path takes the uncorrected fold (its bar is a
+    // tolerance); the precise path keeps the exact one.
+    scan_points<HAS_PDT, 4, !FAST>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(y + i); },
         [&](const auto& ph, const auto& yv) {
             constexpr int K = group_size_v<decltype(ph)>;
