@@ -7,7 +7,12 @@
  *      rb = RN(1/b), q0 = ph*rb equals trunc(IEEE ph/b) for every f32 phase
  *      within +-W ulps of each bin edge k*b (the only places where the two
  *      truncations can differ) and on random phases elsewhere.
- * Prints "fold_bad=<n> bin_bad=<n>"; both must be 0.
+ *  (3) the uncorrected fold of the LS hardware sin/cos path: q0 = t*RN(1/p)
+ *      alone is within 2 ulp of t/p, and its f32 phase is within one f32 ulp
+ *      of [0.5, 1) (2^-24) of the exact phase on the circle (a phase of 1.0
+ *      and one of 0.0 are the same point).
+ * Prints "fold_bad=<n> bin_bad=<n> fast_bad=<n>" (all must be 0) and the
+ * largest distances of (3).
  */
 #include <math.h>
 #include <stdint.h>
@@ -31,9 +36,13 @@ static int bin_ok(float ph, float b, float rb) {
     return (uint32_t)truncf(q) == (uint32_t)truncf(qe);
 }
 
+static int64_t dbits(double d) { int64_t u; memcpy(&u, &d, 8); return u; }
+
 int main(int argc, char** argv) {
     long n_fold = argc > 1 ? atol(argv[1]) : 2000000;
-    long fold_bad = 0, bin_bad = 0;
+    long fold_bad = 0, bin_bad = 0, fast_bad = 0;
+    int64_t fast_max_ulp = 0;
+    double fast_max_phase = 0.0;
     for (long i = 0; i < n_fold; ++i) {
         float t = (float)(urand() * 3000.0);
         float p = (float)(0.05 * pow(2000.0, urand()));
@@ -45,7 +54,17 @@ int main(int argc, char** argv) {
         double r = 1.0 / (double)p;
         double q0 = tc * r;
         double q = fma(fma(-q0, (double)p, tc), r, q0);
-        if (q != tc / (double)p) fold_bad++;
+        const double qe = tc / (double)p;
+        if (q != qe) fold_bad++;
+        int64_t du = dbits(q0) - dbits(qe);  /* same sign: bit distance = ulps */
+        if (du < 0) du = -du;
+        if (du > fast_max_ulp) fast_max_ulp = du;
+        const float ph_e = (float)(qe - floor(qe));
+        const float ph_f = (float)(q0 - floor(q0));
+        double dp = fabs((double)ph_f - (double)ph_e);
+        if (1.0 - dp < dp) dp = 1.0 - dp;
+        if (dp > fast_max_phase) fast_max_phase = dp;
+        if (du > 2 || dp > 0x1p-24) fast_bad++;
     }
     const int ns[] = {1, 2, 3, 5, 7, 9, 10, 11, 12, 13, 15, 16, 17, 20, 25, 30, 33, 37,
                       50, 64, 100, 128, 255};
@@ -71,6 +90,7 @@ int main(int argc, char** argv) {
         if (!bin_ok(1.0f, b, rb)) bin_bad++;
         if (!bin_ok(0.0f, b, rb)) bin_bad++;
     }
-    printf("fold_bad=%ld bin_bad=%ld\n", fold_bad, bin_bad);
-    return (fold_bad || bin_bad) ? 1 : 0;
+    printf("fold_bad=%ld bin_bad=%ld fast_bad=%ld fast_max_ulp=%lld fast_max_phase=%.3g\n",
+           fold_bad, bin_bad, fast_bad, (long long)fast_max_ulp, fast_max_phase);
+    return (fold_bad || bin_bad || fast_bad) ? 1 : 0;
 }

@@ -278,11 +278,12 @@ def test_golden_fixtures():
 
 
 def test_device_division_identities():
-    """The exact fold and the division-free bin used by the kernels."""
+    """The exact fold and the division-free bin used by the kernels, and the
+    error bound of the uncorrected fold that LS's hardware path takes."""
     src = os.path.join(HERE, "exactness", "check_exact.c")
     with tempfile.TemporaryDirectory() as d:
         exe = os.path.join(d, "check_exact")
         subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, src, "-lm"], check=True)
         r = subprocess.run([exe, "2000000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "fold_bad=0 bin_bad=0" in r.stdout
+    assert "fold_bad=0 bin_bad=0 fast_bad=0" in r.stdout
