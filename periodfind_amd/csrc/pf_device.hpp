@@ -51,7 +51,7 @@ __device__ __forceinline__ FoldTrial make_fold_trial(float period, float period_
 // ops run at full rate on CDNA4.
 //
 // EXACT = false drops the correction step: q = RN(t_corr*RN(1/p)), within
-// about 1.5 ulp of the IEEE quotient (1e-11 of a cycle at 4e4 cycles), so
+// 2 ulp of the IEEE quotient (1e-11 of a cycle at 4e4 cycles), so
 // the f32 phase differs from the reference's by at most one f32 ulp, and
 // only at the rare ratios next to an f32 rounding boundary.  Only the
 // statistics whose parity bar is a tolerance take it (LS with the hardware
