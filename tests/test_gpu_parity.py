@@ -174,6 +174,24 @@ def test_grid_params(ctx, algo, params):
     _check(algo, got, ref)
 
 
+def test_ls_precise_path(ctx, monkeypatch):
+    """PF_LS_PRECISE=1 takes sincosf(fl(2 pi phase)), the exact fold and the
+    reference's separate multiply and add; the default takes the hardware
+    revolution sin/cos, the uncorrected fold and fused sums.  Both meet the
+    LS bar against the oracle, and the switch does change the computation."""
+    t, m, e = _curves([300, 2049, 40, 5000], seed=19)
+    m = _prep("ls", m)
+    P = np.linspace(0.5, 10.0, 200).astype(np.float32)
+    D = np.array([0.0, 1e-5], np.float32)
+    ref = _oracle("ls", t, m, e, P, D)
+    fast = _gpu(ctx, "ls", t, m, e, P, D)
+    monkeypatch.setenv("PF_LS_PRECISE", "1")
+    precise = _gpu(ctx, "ls", t, m, e, P, D)
+    _check("ls", fast, ref)
+    _check("ls", precise, ref)
+    assert not np.array_equal(fast, precise)
+
+
 def _special_curves():
     """Curves with the inputs whose results the reference defines by plain f32
     arithmetic: a zero error (weight inf), negative errors, infinite errors
