@@ -51,9 +51,11 @@ __device__ __forceinline__ FoldTrial make_fold_trial(float period, float period_
 // ops run at full rate on CDNA4.
 //
 // EXACT = false drops the correction step: q = RN(t_corr*RN(1/p)), within
-// 2 ulp of the IEEE quotient (1e-11 of a cycle at 4e4 cycles), so
-// the f32 phase differs from the reference's by at most one f32 ulp, and
-// only at the rare ratios next to an f32 rounding boundary.  Only the
+// 2 ulp of the IEEE quotient (5e-10 of a cycle at 1.4e6 cycles, i.e. MJD-scale
+// times folded at 0.05 d), so the f32 phase differs from the reference's by at
+// most an absolute 2^-24 of a cycle, and only at the rare ratios next to an
+// f32 rounding boundary.  (2^-24 is one f32 ulp of a phase in [0.5, 1); a
+// smaller phase can move by several of its own, smaller, ulps.)  Only the
 // statistics whose parity bar is a tolerance take it (LS with the hardware
 // sin/cos).
 template <bool HAS_PDT, bool EXACT = true>

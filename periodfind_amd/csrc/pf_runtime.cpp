@@ -203,10 +203,15 @@ struct pf_ctx {
     std::vector<hipEvent_t> job_ev;
     // LS refinement: frequency grid (f64) and per-peak brackets.
     DevBuf ls_freq, ls_lo, ls_hi;
-    // Matched Filter templates of mf_nb bins (rows of mf_template_stride).
-    DevBuf mf_tmpl;
-    int mf_nb = -1;
-    std::vector<float> mf_tmpl_h;
+    // Matched Filter templates, one table per bin count (rows of
+    // mf_template_stride(nb)); a call with several MF jobs of different bin
+    // counts reads each job's own table.
+    struct MfTable {
+        int nb;
+        DevBuf buf;
+        std::vector<float> h;  // host copy (source of the async upload)
+    };
+    std::vector<MfTable> mf_tables;
 };
 
 struct pf_curves {
@@ -422,9 +427,12 @@ int upload_grid(pf_ctx* ctx, const float* periods, int64_t np, const float* pdts
                 std::memcmp(ctx->periods_h.data(), periods, np * sizeof(float)) == 0 &&
                 std::memcmp(ctx->pdts_h.data(), pdts, nd * sizeof(float)) == 0;
     if (same) return PF_OK;
-    ctx->has_pdt = false;
+    // Forget the cached grid first: a failed grow below frees the old buffers.
+    ctx->periods_h.clear();
+    ctx->pdts_h.clear();
+    bool has_pdt = false;
     for (int64_t i = 0; i < nd; ++i)
-        if (pdts[i] != 0.0f) ctx->has_pdt = true;
+        if (pdts[i] != 0.0f) has_pdt = true;
     PF_TRY(ensure(ctx->periods, np * sizeof(float)));
     PF_TRY(ensure(ctx->pdts, nd * sizeof(float)));
     PF_HIP(hipMemcpyAsync(ctx->periods.p, periods, np * sizeof(float), hipMemcpyHostToDevice,
@@ -434,6 +442,7 @@ int upload_grid(pf_ctx* ctx, const float* periods, int64_t np, const float* pdts
     PF_HIP(hipStreamSynchronize(ctx->stream));
     ctx->periods_h.assign(periods, periods + np);
     ctx->pdts_h.assign(pdts, pdts + nd);
+    ctx->has_pdt = has_pdt;
     return PF_OK;
 }
 
@@ -588,9 +597,16 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
     if (npts > 0 && (!t || !y)) return fail(PF_E_ARG, "t and y must not be NULL");
     set_layout(cs, n_curves, off, npts, mx, e != nullptr);
     const size_t nb = static_cast<size_t>(npts) * sizeof(float);
-    PF_TRY(ensure(cs.off, (n_curves + 1) * sizeof(int64_t)));
-    PF_TRY(ensure(cs.t, nb));
-    PF_TRY(ensure(cs.y, nb));
+    const int64_t zero = 0;
+    // A failed grow leaves an empty, consistent set.
+    auto grow = [&](DevBuf& b, size_t bytes) {
+        const int rc = ensure(b, bytes);
+        if (rc != PF_OK) set_layout(cs, 0, &zero, 0, 0, false);
+        return rc;
+    };
+    PF_TRY(grow(cs.off, (n_curves + 1) * sizeof(int64_t)));
+    PF_TRY(grow(cs.t, nb));
+    PF_TRY(grow(cs.y, nb));
     PF_HIP(hipMemcpyAsync(cs.off.p, off, (n_curves + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                           ctx->stream));
     if (nb) {
@@ -598,7 +614,7 @@ int upload_curves(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const int64_t* of
         PF_HIP(hipMemcpyAsync(cs.y.p, y, nb, hipMemcpyHostToDevice, ctx->stream));
     }
     if (e) {
-        PF_TRY(ensure(cs.e, nb));
+        PF_TRY(grow(cs.e, nb));
         if (nb) PF_HIP(hipMemcpyAsync(cs.e.p, e, nb, hipMemcpyHostToDevice, ctx->stream));
     }
     return PF_OK;
@@ -659,18 +675,31 @@ std::vector<std::vector<float>> mf_shapes(int nb) {
     return out;
 }
 
+// Device table of the templates for nb bins, built on first use and kept
+// (entries are only added once their upload has been issued).
+const float* mf_templates(const pf_ctx* ctx, int nb) {
+    for (const pf_ctx::MfTable& m : ctx->mf_tables)
+        if (m.nb == nb) return static_cast<const float*>(m.buf.p);
+    return nullptr;
+}
+
 int ensure_mf_templates(pf_ctx* ctx, int nb) {
-    if (ctx->mf_nb == nb) return PF_OK;
+    if (mf_templates(ctx, nb)) return PF_OK;
     const std::vector<std::vector<float>> sh = mf_shapes(nb);
     const int stride = pf::mf_template_stride(nb);
-    ctx->mf_tmpl_h.assign(static_cast<size_t>(pf::kMfShapes) * stride, 0.0f);
+    pf_ctx::MfTable m;
+    m.nb = nb;
+    m.h.assign(static_cast<size_t>(pf::kMfShapes) * stride, 0.0f);
     for (int s = 0; s < pf::kMfShapes; ++s)
-        for (int j = 0; j < stride; ++j) ctx->mf_tmpl_h[s * stride + j] = sh[s][j % nb];
-    PF_TRY(ensure(ctx->mf_tmpl, ctx->mf_tmpl_h.size() * sizeof(float)));
-    PF_HIP(hipMemcpyAsync(ctx->mf_tmpl.p, ctx->mf_tmpl_h.data(),
-                          ctx->mf_tmpl_h.size() * sizeof(float), hipMemcpyHostToDevice,
-                          ctx->stream));
-    ctx->mf_nb = nb;
+        for (int j = 0; j < stride; ++j) m.h[s * stride + j] = sh[s][j % nb];
+    PF_TRY(ensure(m.buf, m.h.size() * sizeof(float)));
+    const hipError_t e = hipMemcpyAsync(m.buf.p, m.h.data(), m.h.size() * sizeof(float),
+                                        hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(m.buf.p);
+        return fail(PF_E_HIP, "MF template upload failed: %s", hipGetErrorString(e));
+    }
+    ctx->mf_tables.push_back(std::move(m));
     return PF_OK;
 }
 
@@ -712,6 +741,11 @@ int mhf_args(pf_ctx* ctx, CurveSet& cs, int max_k, int64_t tblocks, float* feat,
     return PF_OK;
 }
 
+// LS variant of a launch: 1 = hardware revolution sin/cos with the
+// uncorrected fold (default), 0 = sincosf of fl(2 pi phase) with the exact
+// fold (PF_LS_PRECISE set).  Read at every launch; captured graphs key on it.
+int ls_fast_mode() { return std::getenv("PF_LS_PRECISE") ? 0 : 1; }
+
 // The magnitudes a call evaluates: as uploaded, or prepared on the device
 // (prepare() must have run for p).
 const float* mags(CurveSet& cs, const Params& p) {
@@ -726,7 +760,10 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
     const int64_t* off = as<int64_t>(cs.off);
     // Magnitudes in the requested preparation; what was derived from another
     // preparation is rebuilt.
+    // Each cached array's validity is cleared before its buffer may be
+    // regrown: a failed grow frees the old buffer.
     if (p.mag_prep != PF_MAG_AS_IS && cs.yp_mode != p.mag_prep) {
+        cs.yp_mode = -1;
         PF_TRY(ensure(cs.yp, n * sizeof(float)));
         PF_HIP(pf::launch_prep_mags(off, as<float>(cs.y), as<float>(cs.yp), cs.n_curves,
                                     p.mag_prep, s));
@@ -739,17 +776,19 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
     const float* y = mags(cs, p);
     // (MHF reads times from its own point records.)
     if (p.algo != PF_ALGO_MHF && (!cs.t64_ok || (has_pdt && !cs.tt_ok))) {
+        cs.t64_ok = cs.tt_ok = false;
         PF_TRY(ensure(cs.t64, n * sizeof(double)));
         if (has_pdt) PF_TRY(ensure(cs.tt, n * sizeof(double)));
         PF_HIP(pf::launch_widen_times(as<float>(cs.t), as<double>(cs.t64),
                                       has_pdt ? as<double>(cs.tt) : nullptr, n, s));
         cs.t64_ok = true;
-        cs.tt_ok = cs.tt_ok || has_pdt;
+        cs.tt_ok = has_pdt;
     }
     switch (p.algo) {
         case PF_ALGO_VN:
             if (cs.vn_ext != p.phase_ext * p.mag_ext) {
                 const int ext = p.phase_ext * p.mag_ext;
+                cs.vn_ext = -1;
                 std::vector<int64_t>& lo = cs.vn_lut_off_h;
                 lo.assign(static_cast<size_t>(cs.n_curves), 0);
                 int64_t total = 0;
@@ -769,6 +808,7 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
             [[fallthrough]];
         case PF_ALGO_CE:
             if (cs.ce_nmag != p.n_mag) {
+                cs.ce_nmag = -1;
                 PF_TRY(ensure(cs.ce_mrow, n * sizeof(uint32_t)));
                 PF_HIP(pf::launch_ce_mag_rows(y, as<uint32_t>(cs.ce_mrow), n, p.n_mag, s));
                 cs.ce_nmag = p.n_mag;
@@ -889,7 +929,7 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             break;
         }
         case PF_ALGO_LS: {
-            pf::LsArgs a{mags(cs, p), std::getenv("PF_LS_PRECISE") ? 0 : 1};
+            pf::LsArgs a{mags(cs, p), ls_fast_mode()};
             le = pf::launch_ls(g, a, q, s);
             break;
         }
@@ -918,7 +958,7 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
         }
         case PF_ALGO_MF: {
             pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
-                         as<float>(ctx->mf_tmpl), p.n_bins, nullptr, nullptr};
+                         mf_templates(ctx, p.n_bins), p.n_bins, nullptr, nullptr};
             le = pf::launch_mf(g, a, pf::lds_mf(p.n_bins), group, q, s);
             break;
         }
@@ -948,6 +988,7 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     ctx->timed = false;
     if (cs.n_curves == 0 || T == 0) return PF_OK;
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
     return launch_grid(ctx, cs, p, np, nd, nullptr, true,
@@ -1153,6 +1194,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     }
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
     OutPlan plan;
@@ -1211,11 +1253,13 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
         reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
-        reinterpret_cast<uint64_t>(cs.mf_wyy.p), reinterpret_cast<uint64_t>(ctx->mf_tmpl.p),
+        reinterpret_cast<uint64_t>(cs.mf_wyy.p),
+        reinterpret_cast<uint64_t>(p.algo == PF_ALGO_MF ? mf_templates(ctx, p.n_bins) : nullptr),
         reinterpret_cast<uint64_t>(cs.vn_lut.p), reinterpret_cast<uint64_t>(cs.vn_lut_off.p),
         static_cast<uint64_t>(p.bandwidth), static_cast<uint64_t>(p.margin),
         reinterpret_cast<uint64_t>(cs.mhf_pt.p), reinterpret_cast<uint64_t>(cs.mhf_sums.p),
-        static_cast<uint64_t>(mhf_max_k(p.max_harmonics))};
+        static_cast<uint64_t>(mhf_max_k(p.max_harmonics)),
+        static_cast<uint64_t>(p.algo == PF_ALGO_LS ? ls_fast_mode() : 0)};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
         key.push_back(static_cast<uint64_t>(b.r1));
@@ -1468,9 +1512,11 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->outbuf, &ctx->ls_freq, &ctx->ls_lo,
-                      &ctx->ls_hi, &ctx->mf_tmpl};
+                      &ctx->ls_hi};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (pf_ctx::MfTable& m : ctx->mf_tables)
+        if (m.buf.p) (void)hipFree(m.buf.p);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     for (int i = 0; i < 2; ++i) {
         if (ctx->in_stage[i]) (void)hipHostFree(ctx->in_stage[i]);
@@ -1571,7 +1617,7 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
     g.has_pdt = 0;
     if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
     pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
-                 as<float>(ctx->mf_tmpl), num_bins, as<float>(ctx->periods),
+                 mf_templates(ctx, num_bins), num_bins, as<float>(ctx->periods),
                  as<float>(ctx->outbuf)};
     PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const hipError_t le = pf::launch_mf(g, a, pf::lds_mf(num_bins), 1, nullptr, ctx->stream);
@@ -1862,6 +1908,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         if (cs.n_curves == 0) continue;
         const Params p = to_params(&params[j]);
         PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+        cs.out_T = 0;
         PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
         cs.out_T = T;
         const size_t C = static_cast<size_t>(cs.n_curves);

@@ -1,16 +1,20 @@
 /* Checks the two division identities the HIP kernels rely on (CPU, IEEE):
  *
  *  (1) f64 fold: q = fma(fma(-q0, p, t), r, q0) with r = RN(1/p), q0 = t*r
- *      equals the IEEE quotient t/p (random t in [0, 3000], p log-uniform
- *      in [0.05, 100], and t_corr = t - (k*t)*t == t - k*(t*t)).
+ *      equals the IEEE quotient t/p (random t in [0, 3000] for half of the
+ *      cases and at MJD scale, [5e4, 7e4], for the other half; p
+ *      log-uniform in [0.05, 100]; up to 1.4e6 cycles), and
+ *      t_corr = t - (k*t)*t == t - k*(t*t).
  *  (2) exact bin: trunc(fma(fma(-q0, b, ph), rb, q0)) with b = fl(1/n),
  *      rb = RN(1/b), q0 = ph*rb equals trunc(IEEE ph/b) for every f32 phase
  *      within +-W ulps of each bin edge k*b (the only places where the two
  *      truncations can differ) and on random phases elsewhere.
  *  (3) the uncorrected fold of the LS hardware sin/cos path: q0 = t*RN(1/p)
- *      alone is within 2 ulp of t/p, and its f32 phase is within one f32 ulp
- *      of [0.5, 1) (2^-24) of the exact phase on the circle (a phase of 1.0
- *      and one of 0.0 are the same point).
+ *      alone is within 2 ulp of t/p, and its f32 phase is within an absolute
+ *      2^-24 of a cycle of the exact phase on the circle (a phase of 1.0 and
+ *      one of 0.0 are the same point).  2^-24 is one f32 ulp of a phase in
+ *      [0.5, 1); a smaller phase can differ by several of its own, smaller,
+ *      ulps within the same absolute bound.
  * Prints "fold_bad=<n> bin_bad=<n> fast_bad=<n>" (all must be 0) and the
  * largest distances of (3).
  */
@@ -44,7 +48,7 @@ int main(int argc, char** argv) {
     int64_t fast_max_ulp = 0;
     double fast_max_phase = 0.0;
     for (long i = 0; i < n_fold; ++i) {
-        float t = (float)(urand() * 3000.0);
+        float t = (i % 4 < 2) ? (float)(urand() * 3000.0) : (float)(5e4 + urand() * 2e4);
         float p = (float)(0.05 * pow(2000.0, urand()));
         float pdt = (i % 2) ? 0.0f : (float)((urand() - 0.5) * 2e-4);
         float k = (pdt / p) / 2.0f;

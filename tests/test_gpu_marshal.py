@@ -110,23 +110,23 @@ def test_mag_prep_switches_on_one_set(ctx):
     cs.close()
 
 
-@pytest.mark.parametrize("algo", ["ce", "ls", "aov", "bls"])
+@pytest.mark.parametrize("algo", ["ce", "ls", "aov", "fpw", "bls"])
 def test_drop_in_calc_matches_host_prepared_grid(ctx, algo):
     # The drop-in classes load lists and prepare on the device; the result
     # equals the CSR grid over numpy-prepared magnitudes.
     import periodfind_amd.gpu as G
 
     t, m, e = _curves([50, 400, 1, 3000, 8193], 13)
-    cls = {"ce": G.ConditionalEntropy, "ls": G.LombScargle, "aov": G.AOV,
+    cls = {"ce": G.ConditionalEntropy, "ls": G.LombScargle, "aov": G.AOV, "fpw": G.FPW,
            "bls": G.BoxLeastSquares}[algo]
-    kw = {"errs": e} if algo == "bls" else {}
+    kw = {"errs": e} if algo in ("fpw", "bls") else {}
     if algo == "aov":
         kw["center"] = True  # exercise a non-default preparation
     got = cls(**PARAMS[algo]).calc(t, m, PERIODS, PDTS, output="periodogram", **kw)
     center = algo == "ls" or algo == "aov"
     normalize = algo == "ce"
     host = O.prepare_magnitudes(m, center, normalize)
-    cs = capi.CurveSet(ctx, t, host, e if algo == "bls" else None)
+    cs = capi.CurveSet(ctx, t, host, e if algo in ("fpw", "bls") else None)
     want = cs.grid(algo, PERIODS, PDTS, **PARAMS[algo])
     cs.close()
     for i, p in enumerate(got):

@@ -97,3 +97,30 @@ def test_ragged_lengths_period_derivatives_and_no_timing(ctx):
             np.testing.assert_array_equal(got, w, err_msg=f"job {j}")
     for h, _, _ in jobs:
         capi.lib().pf_curves_destroy(h)
+
+
+def test_mixed_matched_filter_bin_counts(ctx):
+    """Matched Filter jobs of different bin counts in one call each read their
+    own template table (a table per bin count is kept on the context)."""
+    t, m, e, P, D = synth.config2(600, 6)
+    P = P[:200].copy()
+    jobs = []
+    for nb in (20, 40, 20, 7):
+        off, tt = capi.pack(t)
+        _, y = capi.pack(m)
+        h = ctypes.c_void_p()
+        capi.check(capi.lib().pf_curves_create(ctx.h, ctypes.byref(h), len(t), off, tt, y,
+                                               capi.pack(e)[1]))
+        jobs.append((h, capi.make_params("mf", n_bins=nb), len(t)))
+    sm = capi.StatsMany(ctx, jobs, P, D, n_stats=2)
+    sm.run(timing=False)
+    got = [[a.copy() for a in (sm.summary[j], sm.idx[j], sm.val[j])] for j in range(len(jobs))]
+    for j, (h, prm, C) in enumerate(jobs):
+        want = _single(ctx, h, prm, C, P, D, 2)
+        for g, w in zip(got[j], want):
+            np.testing.assert_array_equal(g, w, err_msg=f"job {j} (n_bins {prm.n_bins})")
+    # Jobs 0 and 2 hold the same curves and bin count; 1 and 3 differ from them.
+    np.testing.assert_array_equal(got[0][2], got[2][2])
+    assert not np.array_equal(got[0][2], got[1][2])
+    for h, _, _ in jobs:
+        capi.lib().pf_curves_destroy(h)
