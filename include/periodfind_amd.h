@@ -246,6 +246,27 @@ int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64
              int64_t n_periods, const float* period_dts, int64_t n_pdts, int64_t n_peaks,
              int64_t min_distance, int64_t* idx, float* val);
 
+/* pf_peaks without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and results
+ * as pf_peaks for CE, AOV, LS, FPW and BLS (other statistics: PF_E_ARG).
+ *   - offsets, periods, period_dts: host arrays, as in pf_peaks;
+ *   - t, y, err (err may be NULL: ones): device float32[offsets[n_curves]];
+ *   - idx, val: device [n_curves][n_peaks] outputs;
+ *   - work is queued on `stream` (a hipStream_t; NULL = the default stream)
+ *     and NOT waited for; the small host arrays are copied before return.
+ * The derived per-point arrays and the periodogram use one 64 MiB device
+ * array per GPU, created when the library's code is loaded on that GPU and
+ * held until the process ends.  A call needing more (about 24 B per point
+ * plus 4 B per curve x trial, plus the grid) fails with PF_E_LIMIT before
+ * any work is queued.  Calls on one GPU share that array: calls that overlap
+ * in time on one GPU (other streams, threads or contexts) race. */
+int pf_peaks_noalloc(pf_ctx* ctx, void* stream, const pf_params* params, int64_t n_curves,
+                     const int64_t* offsets, const float* t, const float* y, const float* err,
+                     const float* periods, int64_t n_periods, const float* period_dts,
+                     int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                     float* val);
+
 /* Standalone peak finder over a host periodogram data[n_curves][T]
  * (find_top_peaks_batched, rust/src/lib.rs:509-555). */
 int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,

@@ -28,7 +28,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["pf_grid.hip", "pf_prep.hip", "pf_peaks.hip", "pf_stats.hip", "pf_lsdiff.hip",
-           "pf_runtime.cpp"]
+           "pf_noalloc.hip", "pf_runtime.cpp"]
 
 
 def _compile(src):

@@ -106,6 +106,9 @@ SIGNATURES = {
     "pf_peaks": (_int, [_vp, ctypes.POINTER(PfParams), _i64, _i64p, _f32p, _f32p,
                         _f32p_or_null, _f32p, _i64, _f32p, _i64, _i64, _i64, _i64p, _f32p]),
     "pf_find_top_peaks": (_int, [_vp, _f32p, _i64, _i64, _i64, _i64, _int, _i64p, _f32p]),
+    # Allocation-free peaks: host offsets and grid, device t/y/err and outputs.
+    "pf_peaks_noalloc": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _i64, _i64p, _vp, _vp, _vp,
+                                _f32p, _i64, _f32p, _i64, _i64, _i64, _vp, _vp]),
     "pf_curves_create": (_int, [_vp, ctypes.POINTER(_vp), _i64, _i64p, _f32p, _f32p,
                                 _f32p_or_null]),
     "pf_curves_destroy": (None, [_vp]),

@@ -200,6 +200,13 @@ hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_to
                         float* summary, int64_t* top_idx, float* top_val, int* flags,
                         hipStream_t s);
 
+// Allocation-free entry point (pf_noalloc.hip): the bytes of its per-GPU
+// device array, and that array's address on `device`, which must be the
+// current device (resolved with hipGetSymbolAddress on the first call per
+// device, then cached; thread-safe).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // Peak selection (pf_peaks.hip).
 hipError_t launch_peaks(const float* data, int64_t n_curves, int64_t T, int64_t n_peaks,
                         int64_t min_distance, int use_max, int64_t* idx, float* val,

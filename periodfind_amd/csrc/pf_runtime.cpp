@@ -54,12 +54,15 @@ int fail(int code, const char* fmt, ...) {
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;  // a region of memory the buffer does not own: never regrown
 };
 
 // Grow-only device buffer.  Returns PF_OK or an error code.
 int ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return PF_OK;
+    if (b.fixed)
+        return fail(PF_E_LIMIT, "a fixed %zu-byte region cannot hold %zu bytes", b.cap, bytes);
     if (b.p) {
         (void)hipFree(b.p);
         b.p = nullptr;
@@ -149,13 +152,20 @@ struct CurveSet {
         ce_nmag = -1;
         drop_graph();
     }
+    std::vector<DevBuf*> buffers() {
+        return {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
+                &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &mhf_pt,
+                &mhf_sums, &mhf_part, &out, &order, &counter};
+    }
+    // A set over memory it does not own: no buffer may grow (a grow fails
+    // instead of allocating).
+    void fix_all() {
+        for (DevBuf* b : buffers()) b->fixed = true;
+    }
     void release() {
         drop_graph();
-        DevBuf* bufs[] = {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                          &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &mhf_pt,
-                          &mhf_sums, &mhf_part, &out, &order, &counter};
-        for (DevBuf* b : bufs)
-            if (b->p) {
+        for (DevBuf* b : buffers())
+            if (b->p && !b->fixed) {
                 (void)hipFree(b->p);
                 b->p = nullptr;
                 b->cap = 0;
@@ -212,6 +222,10 @@ struct pf_ctx {
         std::vector<float> h;  // host copy (source of the async upload)
     };
     std::vector<MfTable> mf_tables;
+    // The device array of pf_peaks_noalloc on this device (resolved at
+    // creation) and the status of that lookup.
+    void* noalloc = nullptr;
+    hipError_t noalloc_rc = hipErrorNotInitialized;
 };
 
 struct pf_curves {
@@ -753,8 +767,7 @@ const float* mags(CurveSet& cs, const Params& p) {
 }
 
 // Build (or reuse) the derived arrays params.algo needs.
-int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
-    hipStream_t s = ctx->stream;
+int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt, hipStream_t s) {
     const int64_t n = cs.npts;
     const float* e = cs.has_err ? as<float>(cs.e) : nullptr;
     const int64_t* off = as<int64_t>(cs.off);
@@ -865,22 +878,34 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt) {
     return PF_OK;
 }
 
-// Launch the grid kernel of params.algo over cs into cs.out ([n_curves][T]).
-// With q, the persistent kernel pulls work from *q (derived arrays must
-// already be prepared).  `timed` brackets the launch with the context's
-// events (not allowed while the stream is being captured).
+// The trial grid a launch reads: device periods[np], period_dts[nd].
+struct TrialGrid {
+    const float* periods;
+    const float* pdts;
+    bool has_pdt;  // any period_dt != 0
+};
+
+TrialGrid ctx_grid(const pf_ctx* ctx) {
+    return TrialGrid{static_cast<const float*>(ctx->periods.p),
+                     static_cast<const float*>(ctx->pdts.p), ctx->has_pdt};
+}
+
+// Launch the grid kernel of params.algo over cs into cs.out ([n_curves][T])
+// on stream s.  With q, the persistent kernel pulls work from *q (derived
+// arrays must already be prepared).  `timed` brackets the launch with the
+// context's events (not allowed while the stream is being captured).
 int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd,
-                const pf::WorkQueue* q, bool timed, int group, hipStream_t stream = nullptr) {
+                const TrialGrid& tg, const pf::WorkQueue* q, bool timed, int group,
+                hipStream_t s) {
     const int64_t T = np * nd;
-    const bool has_pdt = ctx->has_pdt;
-    hipStream_t s = stream ? stream : ctx->stream;
+    const bool has_pdt = tg.has_pdt;
 
     pf::GridArgs g;
     g.off = as<int64_t>(cs.off);
     g.t64 = as<double>(cs.t64);
     g.tt = has_pdt ? as<double>(cs.tt) : as<double>(cs.t64);
-    g.periods = as<float>(ctx->periods);
-    g.pdts = as<float>(ctx->pdts);
+    g.periods = tg.periods;
+    g.pdts = tg.pdts;
     g.nd = nd;
     g.T = T;
     g.tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
@@ -987,12 +1012,12 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     const int64_t T = np * nd;
     ctx->timed = false;
     if (cs.n_curves == 0 || T == 0) return PF_OK;
-    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
-    return launch_grid(ctx, cs, p, np, nd, nullptr, true,
-                       tune_choice(ctx, p.algo, cs.rep_len).group);
+    return launch_grid(ctx, cs, p, np, nd, ctx_grid(ctx), nullptr, true,
+                       tune_choice(ctx, p.algo, cs.rep_len).group, ctx->stream);
 }
 
 int finish(pf_ctx* ctx) {
@@ -1193,7 +1218,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         return PF_OK;
     }
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
-    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+    PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
@@ -1285,7 +1310,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
                 q.bucket_claim[b] = buckets[b].ch.claim;
                 q.bucket_end[b] = buckets[b].r1;
             }
-            rc = launch_grid(ctx, cs, p, np, nd, &q, false, buckets[0].ch.group);
+            rc = launch_grid(ctx, cs, p, np, nd, ctx_grid(ctx), &q, false, buckets[0].ch.group,
+                             s);
         }
         if (rc == PF_OK &&
             pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
@@ -1347,6 +1373,24 @@ int check_dev(pf_ctx* ctx, const void* ptr, const char* name) {
 
 int check_dev_opt(pf_ctx* ctx, const void* ptr, const char* name) {
     return ptr ? check_dev(ctx, ptr, name) : PF_OK;
+}
+
+// A device array of `bytes` bytes at ptr must lie inside one HIP allocation
+// of ctx's device (a caller's sub-allocation is checked against the whole
+// allocation it sits in).
+int check_dev_span(pf_ctx* ctx, const void* ptr, size_t bytes, const char* name) {
+    PF_TRY(check_dev(ctx, ptr, name));
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(ptr)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PF_E_ARG, "%s: no HIP allocation holds this address", name);
+    }
+    const uintptr_t lo = reinterpret_cast<uintptr_t>(base);
+    const uintptr_t at = reinterpret_cast<uintptr_t>(ptr);
+    if (at < lo || at - lo > size || bytes > size - (at - lo))
+        return fail(PF_E_ARG, "%s: %zu bytes run past the end of its allocation", name, bytes);
+    return PF_OK;
 }
 
 // Makes ctx's device current for the scope of a device-pointer call and
@@ -1497,6 +1541,10 @@ int pf_ctx_create(pf_ctx** out, int device) {
             ctx->tune_source = path;
         }
     }
+    // The allocation-free entry point's device array on this device (the
+    // device is current); a failed lookup only disables that entry point.
+    ctx->noalloc_rc = pf::noalloc_scratch(device, &ctx->noalloc);
+    if (ctx->noalloc_rc != hipSuccess) (void)hipGetLastError();
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         pf_ctx_destroy(ctx);
@@ -1600,7 +1648,7 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
     // 0), and the kernel folds curve c at periods[c].
     const float zero = 0.0f;
     PF_TRY(upload_grid(ctx, periods, n_curves, &zero, 1));
-    PF_TRY(prepare(ctx, cs, p, false));
+    PF_TRY(prepare(ctx, cs, p, false, ctx->stream));
     const size_t ob = static_cast<size_t>(n_curves) * pf::kMfFeatures * sizeof(float);
     PF_TRY(ensure(ctx->outbuf, ob));
     pf::GridArgs g;
@@ -1656,7 +1704,7 @@ int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const fl
         period_dts = zeros.data();
     }
     PF_TRY(upload_grid(ctx, periods, n_curves, period_dts, n_curves));
-    PF_TRY(prepare(ctx, cs, p, false));
+    PF_TRY(prepare(ctx, cs, p, false, ctx->stream));
     const int K = mhf_max_k(max_harmonics);
     const size_t ob = static_cast<size_t>(n_curves) * (K + 2) * sizeof(float);
     PF_TRY(ensure(ctx->outbuf, ob));
@@ -1696,7 +1744,7 @@ int pf_fourier_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, c
     PF_TRY(upload_grid(ctx, periods, n_curves, &zero, 1));
     Params p{PF_ALGO_MHF, 0, 0, 1, 1, 0, 0.0f, 0.0f};
     p.max_harmonics = pf::kMhfMaxK;
-    PF_TRY(prepare(ctx, cs, p, false));  // the MHF point records and curve sums
+    PF_TRY(prepare(ctx, cs, p, false, ctx->stream));  // the MHF point records and curve sums
     const size_t ob = static_cast<size_t>(n_curves) * pf::kFourierFeatures * sizeof(float);
     PF_TRY(ensure(ctx->outbuf, ob));
     PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -1734,6 +1782,152 @@ int pf_peaks(pf_ctx* ctx, const pf_params* params, int64_t n_curves, const int64
     PF_TRY(upload_curves(ctx, ctx->scratch, n_curves, offsets, t, y, err));
     return peaks_on(ctx, ctx->scratch, p, periods, n_periods, period_dts, n_pdts, n_peaks,
                     min_distance, idx, val);
+}
+
+// pf_peaks without any device allocation: the derived per-point arrays and
+// the periodogram live in the per-GPU array of pf_noalloc.hip, laid out per
+// call; inputs and outputs are the caller's device memory.  Every region is
+// written in full by the kernel that first reads it later in the same call,
+// so nothing has to start at zero.  Calls that overlap on one GPU share the
+// array and race.
+int pf_peaks_noalloc(pf_ctx* ctx, void* stream, const pf_params* params, int64_t n_curves,
+                     const int64_t* offsets, const float* t, const float* y, const float* err,
+                     const float* periods, int64_t n_periods, const float* period_dts,
+                     int64_t n_pdts, int64_t n_peaks, int64_t min_distance, int64_t* idx,
+                     float* val) {
+    if (!ctx || !params) return fail(PF_E_ARG, "ctx/params is NULL");
+    const Params p = to_params(params);
+    PF_TRY(check_params(p, ctx->lds_max));
+    if (p.algo != PF_ALGO_CE && p.algo != PF_ALGO_AOV && p.algo != PF_ALGO_LS &&
+        p.algo != PF_ALGO_FPW && p.algo != PF_ALGO_BLS)
+        return fail(PF_E_ARG, "pf_peaks_noalloc evaluates CE, AOV, LS, FPW and BLS only");
+    PF_TRY(check_grid(periods, n_periods, period_dts, n_pdts));
+    if (n_peaks < 0 || n_peaks > 64) return fail(PF_E_LIMIT, "n_peaks must be in [0, 64]");
+    if (min_distance < 0) return fail(PF_E_ARG, "min_distance < 0");
+    int64_t npts = 0, mx = 0;
+    PF_TRY(validate_offsets(n_curves, offsets, &npts, &mx));
+    if (n_curves == 0 || n_peaks == 0) return PF_OK;
+
+    // Scratch layout (256-byte aligned regions); sizes are checked against
+    // the array before any product can overflow.
+    constexpr uint64_t kMax = pf::kNoAllocScratchBytes;
+    const uint64_t C = static_cast<uint64_t>(n_curves), N = static_cast<uint64_t>(npts);
+    const uint64_t T = static_cast<uint64_t>(n_periods) * static_cast<uint64_t>(n_pdts);
+    if (C > kMax / 8 || N > kMax / 8 || static_cast<uint64_t>(n_periods) > kMax / 4 ||
+        static_cast<uint64_t>(n_pdts) > kMax / 4 || (T > 0 && C > kMax / 4 / T))
+        return fail(PF_E_LIMIT, "pf_peaks_noalloc: the call exceeds the %llu-byte scratch array",
+                    static_cast<unsigned long long>(kMax));
+    bool has_pdt = false;
+    for (int64_t i = 0; i < n_pdts; ++i)
+        if (period_dts[i] != 0.0f) has_pdt = true;
+    const bool prep = p.mag_prep != PF_MAG_AS_IS;
+    uint64_t total = 0;
+    auto region = [&](uint64_t bytes) {
+        const uint64_t at = total;
+        total += (std::max<uint64_t>(bytes, 16) + 255) & ~uint64_t(255);
+        return at;
+    };
+    const uint64_t o_off = region((C + 1) * 8), o_per = region(n_periods * 4ull),
+                   o_pdt = region(n_pdts * 4ull), o_t64 = region(N * 8),
+                   o_tt = has_pdt ? region(N * 8) : 0, o_yp = prep ? region(N * 4) : 0;
+    uint64_t o_a = 0, o_b = 0, o_c = 0;  // the statistic's own arrays
+    switch (p.algo) {
+        case PF_ALGO_CE: o_a = region(N * 4); break;                      // magnitude rows
+        case PF_ALGO_AOV: o_a = region(N * 4); o_c = region(C * 4); break;  // y*y, mean
+        case PF_ALGO_FPW: o_a = region(N * 4); o_b = region(N * 4); break;  // w, w*y
+        case PF_ALGO_BLS:                                                  // w, yw, mean
+            o_a = region(N * 4);
+            o_b = region(N * 4);
+            o_c = region(C * 4);
+            break;
+        default: break;
+    }
+    const uint64_t o_out = region(C * T * 4);
+    if (total > kMax)
+        return fail(PF_E_LIMIT,
+                    "pf_peaks_noalloc: this call needs %llu bytes of scratch; the array holds %llu",
+                    static_cast<unsigned long long>(total), static_cast<unsigned long long>(kMax));
+
+    DeviceGuard dg(ctx->device);
+    if (ctx->noalloc_rc != hipSuccess || !ctx->noalloc)
+        return fail(PF_E_HIP, "the allocation-free scratch array is unavailable: %s",
+                    hipGetErrorString(ctx->noalloc_rc));
+    PF_TRY(check_dev_span(ctx, idx, C * n_peaks * sizeof(int64_t), "idx"));
+    PF_TRY(check_dev_span(ctx, val, C * n_peaks * sizeof(float), "val"));
+    if (N > 0) {
+        PF_TRY(check_dev_span(ctx, t, N * 4, "t"));
+        PF_TRY(check_dev_span(ctx, y, N * 4, "y"));
+        if (err) PF_TRY(check_dev_span(ctx, err, N * 4, "err"));
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(ctx->noalloc);
+
+    // A curve set over the scratch regions and the caller's arrays; every
+    // buffer is fixed, so nothing below can allocate.
+    CurveSet cs;
+    cs.fix_all();
+    auto fixed = [](void* ptr, uint64_t bytes) {
+        DevBuf b;
+        b.p = ptr;
+        b.cap = std::max<uint64_t>(bytes, 16);
+        b.fixed = true;
+        return b;
+    };
+    cs.n_curves = n_curves;
+    cs.npts = npts;
+    cs.max_len = mx;
+    cs.has_err = err != nullptr;
+    {
+        double s1 = 0.0, s2 = 0.0;
+        for (int64_t c = 0; c < n_curves; ++c) {
+            const double len = static_cast<double>(offsets[c + 1] - offsets[c]);
+            s1 += len;
+            s2 += len * len;
+        }
+        cs.rep_len = s1 > 0.0 ? static_cast<int64_t>(s2 / s1) : 0;
+    }
+    cs.off = fixed(base + o_off, (C + 1) * 8);
+    cs.t = fixed(const_cast<float*>(t), N * 4);
+    cs.y = fixed(const_cast<float*>(y), N * 4);
+    if (err) cs.e = fixed(const_cast<float*>(err), N * 4);
+    cs.t64 = fixed(base + o_t64, N * 8);
+    if (has_pdt) cs.tt = fixed(base + o_tt, N * 8);
+    if (prep) cs.yp = fixed(base + o_yp, N * 4);
+    switch (p.algo) {
+        case PF_ALGO_CE: cs.ce_mrow = fixed(base + o_a, N * 4); break;
+        case PF_ALGO_AOV:
+            cs.aov_yy = fixed(base + o_a, N * 4);
+            cs.aov_avg = fixed(base + o_c, C * 4);
+            break;
+        case PF_ALGO_FPW:
+            cs.w = fixed(base + o_a, N * 4);
+            cs.fpw_wy = fixed(base + o_b, N * 4);
+            break;
+        case PF_ALGO_BLS:
+            cs.w = fixed(base + o_a, N * 4);
+            cs.bls_yw = fixed(base + o_b, N * 4);
+            cs.bls_mean = fixed(base + o_c, C * 4);
+            break;
+        default: break;
+    }
+    cs.out = fixed(base + o_out, C * T * 4);
+    cs.out_T = static_cast<int64_t>(T);
+
+    // Host arrays in, then the same kernels as pf_peaks.
+    PF_HIP(hipMemcpyAsync(cs.off.p, offsets, (C + 1) * 8, hipMemcpyHostToDevice, s));
+    if (T > 0) {
+        PF_HIP(hipMemcpyAsync(base + o_per, periods, n_periods * 4ull, hipMemcpyHostToDevice, s));
+        PF_HIP(hipMemcpyAsync(base + o_pdt, period_dts, n_pdts * 4ull, hipMemcpyHostToDevice, s));
+        const TrialGrid tg{reinterpret_cast<const float*>(base + o_per),
+                           reinterpret_cast<const float*>(base + o_pdt), has_pdt};
+        PF_TRY(prepare(ctx, cs, p, has_pdt, s));
+        PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, tg, nullptr, false,
+                           tune_choice(ctx, p.algo, cs.rep_len).group, s));
+    }
+    const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
+    PF_HIP(pf::launch_peaks(as<float>(cs.out), n_curves, static_cast<int64_t>(T), n_peaks,
+                            min_distance, use_max, idx, val, s));
+    return PF_OK;
 }
 
 int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t T,
@@ -1907,7 +2101,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         CurveSet& cs = sets[j]->cs;
         if (cs.n_curves == 0) continue;
         const Params p = to_params(&params[j]);
-        PF_TRY(prepare(ctx, cs, p, ctx->has_pdt));
+        PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
         cs.out_T = 0;
         PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
         cs.out_T = T;
@@ -1959,7 +2153,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         if (timing) PF_HIP(hipEventRecord(ctx->job_ev[2 * j], s));
         if (cs.n_curves > 0) {
             const Params p = to_params(&params[j]);
-            PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, nullptr, false,
+            PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, ctx_grid(ctx), nullptr, false,
                                tune_choice(ctx, p.algo, cs.rep_len).group, s));
             const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
             PF_HIP(pf::launch_stats(as<float>(cs.out), cs.n_curves, T, n_stats, use_max,
