@@ -96,7 +96,7 @@ def lib():
         L.or_mf_features.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _f32p, _int, _f32p, _int]
         L.or_vn_batched.restype = _int
         L.or_vn_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p, _i64, _f32p, _i64, _int,
-                                    _int, _int, _int, _int, _int, _f32p, _int, _int]
+                                    _int, _int, _int, _int, _int, _f32p, _int]
         L.or_mhf_batched.restype = _int
         L.or_mhf_batched.argtypes = [_i64, _i64p, _f32p, _f32p, _f32p_or_null, _f32p, _i64,
                                      _f32p, _i64, _int, _f32p, _int]
@@ -233,15 +233,13 @@ def mf_circular_pearson_max(profile, template):
 
 
 def vn_grid(times, mags, periods, period_dts, n_phase=20, n_mag=20, phase_ext=1, mag_ext=1,
-            bandwidth=2, margin=1, threads=0, dlog=False):
+            bandwidth=2, margin=1, threads=0):
     """calc_vn_batched (rust/src/lib.rs, vn.rs:156-219) on magnitudes as given
-    (the CPU class normalizes them first, cpu/__init__.py:1093-1150).  dlog:
-    emission logs as the double log rounded to f32 instead of libm logf."""
+    (the CPU class normalizes them first, cpu/__init__.py:1093-1150)."""
     off, t, P, D, out = _grid_args(times, periods, period_dts)
     _, m = pack(mags)
     rc = lib().or_vn_batched(len(times), off, t, m, P, P.size, D, D.size, n_phase, n_mag,
-                             phase_ext, mag_ext, bandwidth, margin, out.reshape(-1), threads,
-                             int(bool(dlog)))
+                             phase_ext, mag_ext, bandwidth, margin, out.reshape(-1), threads)
     if rc != 0:
         raise ValueError("invalid Viterbi Narrowband parameters")
     return out

@@ -5,10 +5,7 @@
 // and bench.py's cpu_baseline leg may load it.
 //
 // Operation by operation as vn.rs, built with -ffp-contract=off; ln is the
-// C library's logf, as Rust std calls it on Linux.  With `dlog` set, the
-// emission log is instead the double log rounded to f32 (what the HIP kernel
-// computes), so the two can be compared with the log taken out of the
-// question.
+// C library's logf, as Rust std calls it on Linux.
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -37,12 +34,12 @@ inline float fold_time(float t, float period, float pdt_corr) {
 // vn.rs:17-150 (circular_viterbi_concentration); hist[phi * n_mag + m].
 float viterbi_concentration(const float* hist, int np_, int nm, int bw, int margin,
                             std::vector<float>& emit, std::vector<int>& back,
-                            std::vector<int>& best_path, bool dlog) {
+                            std::vector<int>& best_path) {
     const float eps = 1e-10f;
     const float neg_inf = -INFINITY;
     for (int i = 0; i < np_ * nm; ++i) {
         const float x = hist[i] + eps;
-        emit[i] = dlog ? static_cast<float>(std::log(static_cast<double>(x))) : logf(x);
+        emit[i] = logf(x);
     }
     float best_total = neg_inf;
     std::vector<float> v_prev(nm), v_curr(nm);
@@ -104,7 +101,7 @@ float viterbi_concentration(const float* hist, int np_, int nm, int bw, int marg
 float vn_trial(const float* t, const float* m, int64_t n, float period, float pdt, int np_,
                int nm, int pext, int mext, int bw, int margin, std::vector<uint32_t>& hist,
                std::vector<float>& hf, std::vector<float>& emit, std::vector<int>& back,
-               std::vector<int>& path, bool dlog) {
+               std::vector<int>& path) {
     const float pbs = 1.0f / static_cast<float>(np_);
     const float mbs = 1.0f / static_cast<float>(nm);
     const float norm = static_cast<float>(static_cast<uint64_t>(n) * pext * mext);
@@ -122,7 +119,7 @@ float vn_trial(const float* t, const float* m, int64_t n, float period, float pd
             }
     }
     for (int i = 0; i < np_ * nm; ++i) hf[i] = static_cast<float>(hist[i]) / norm;
-    return viterbi_concentration(hf.data(), np_, nm, bw, margin, emit, back, path, dlog);
+    return viterbi_concentration(hf.data(), np_, nm, bw, margin, emit, back, path);
 }
 
 int n_threads(int requested) { return requested > 0 ? requested : omp_get_max_threads(); }
@@ -135,7 +132,7 @@ extern "C" {
 int or_vn_batched(int64_t n_curves, const int64_t* off, const float* t, const float* m,
                   const float* periods, int64_t np_, const float* pdts, int64_t nd, int n_phase,
                   int n_mag, int phase_ext, int mag_ext, int bandwidth, int margin, float* out,
-                  int threads, int dlog) {
+                  int threads) {
     if (n_phase < 1 || n_mag < 1 || phase_ext < 1 || mag_ext < 1 || bandwidth < 0 || margin < 0)
         return -1;
     const int64_t T = np_ * nd;
@@ -151,8 +148,7 @@ int or_vn_batched(int64_t n_curves, const int64_t* off, const float* t, const fl
             const int64_t c = g / T, tr = g % T;
             const int64_t a = off[c], n = off[c + 1] - off[c];
             out[g] = vn_trial(t + a, m + a, n, periods[tr / nd], pdts[tr % nd], n_phase, n_mag,
-                              phase_ext, mag_ext, bandwidth, margin, hist, hf, emit, back, path,
-                              dlog != 0);
+                              phase_ext, mag_ext, bandwidth, margin, hist, hf, emit, back, path);
         }
     }
     return 0;
@@ -163,7 +159,7 @@ float or_vn_concentration(const float* hist, int n_phase, int n_mag, int bandwid
     const int cells = n_phase * n_mag;
     std::vector<float> emit(cells);
     std::vector<int> back(cells), path(n_phase);
-    return viterbi_concentration(hist, n_phase, n_mag, bandwidth, margin, emit, back, path, false);
+    return viterbi_concentration(hist, n_phase, n_mag, bandwidth, margin, emit, back, path);
 }
 
 }  // extern "C"

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pf_logf.hpp"
+
 namespace pf {
 
 constexpr int kWave = 64;

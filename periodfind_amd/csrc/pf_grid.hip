@@ -442,7 +442,7 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
         for (int i = 0; i < a.n_mag; ++i) {
             const float pij = static_cast<float>(count(j, i)) / norm;
             if (pij != 0.0f) {
-                const float term = pij * logf(pj / pij);
+                const float term = pij * logf_glibc(pj / pij);
                 ce += term;
             }
         }

@@ -216,9 +216,8 @@ __global__ __launch_bounds__(kBlock) void prep_mags_kernel(const int64_t* __rest
 
 // Viterbi Narrowband emission table (rust/src/vn.rs:28-33, 208-212): for
 // curve c of n points and every count k in [0, n*ext], ln(fl(fl(k / norm) +
-// 1e-10)) with norm = f32(n*ext).  The f32 log is the double log rounded to
-// f32 (correctly rounded but for inputs within a few double ulps of an f32
-// rounding boundary).  One workgroup per curve.
+// 1e-10)) with norm = f32(n*ext), the log being the C library's logf as the
+// reference calls it (logf_glibc).  One workgroup per curve.
 __global__ __launch_bounds__(kBlock) void vn_lut_kernel(const int64_t* __restrict__ off,
                                                         const int64_t* __restrict__ lut_off,
                                                         float* __restrict__ lut, int ext) {
@@ -229,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void vn_lut_kernel(const int64_t* __restric
     for (int64_t k = threadIdx.x; k <= kmax; k += kBlock) {
         const float h = static_cast<float>(static_cast<uint32_t>(k)) / norm;
         const float x = h + 1e-10f;
-        o[k] = static_cast<float>(log(static_cast<double>(x)));
+        o[k] = logf_glibc(x);
     }
 }
 

@@ -26,11 +26,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _ce_bar(got, ref):
-    """CE: exact histograms; values within rtol 2e-6 of the oracle's and the
-    same best index per curve."""
-    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-7)
-    for c in range(got.shape[0]):
-        assert np.argmin(got[c]) == np.argmin(ref[c]), c
+    """CE: bit-identical to the oracle (exact histograms, the reference's f32
+    epilogue and the C library's logf algorithm)."""
+    np.testing.assert_array_equal(got, ref)
 
 
 def test_ce_256_points_periodogram_full():
@@ -127,6 +125,5 @@ def test_ragged_ce_peaks_against_oracle(ragged, tuned_ctx):
     ref = O.ce_grid(t, mm, P, D, **prm)
     _ce_bar(grid, ref)
     oi, ov = O.find_top_peaks_batched(ref, 32, 1, use_max=False)
-    assert np.array_equal(gi[:, 0], oi[:, 0])
-    same = float(np.mean(gi == oi))
-    print(f"ragged CE: peak ranks identical to the oracle's: {same:.4f}", flush=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gv, ov)

@@ -287,3 +287,17 @@ def test_device_division_identities():
         r = subprocess.run([exe, "2000000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "fold_bad=0 bin_bad=0 fast_bad=0" in r.stdout
+
+
+def test_device_logf_is_the_c_library_logf():
+    """The logf the CE and VN kernels use (periodfind_amd/csrc/pf_logf.hpp,
+    compiled here for the host) equals the C library's logf, which the
+    reference's f32::ln calls, on every positive float and the special values."""
+    src = os.path.join(HERE, "exactness", "check_logf_glibc.cpp")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "check_logf_glibc")
+        subprocess.run(["g++", "-O2", "-fopenmp", "-ffp-contract=off", "-o", exe, src, "-lm"],
+                       check=True)
+        r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0 special_bad=0" in r.stdout
