@@ -1,21 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark: batched periodogram throughput on one MI355X.
 
-Workload (one "step"): the five statistics CE(10,10), AOV(15), LS, FPW(10)
-and BLS(50, 0.01, 0.5), each over 100 curves at each of 1024, 4096 and 8192
-points (mixed sinusoid / box-eclipse / noise curves, times uniform over 1000
-days), 1000 trial periods linspace(0.5, 10), period_dts [0], output 'stats'.
-Inputs are resident in HBM before the timed region (resident curve sets);
-a step evaluates the grid and the per-curve statistics on the device and
-copies the statistics back.
+The metric has three parts: pts/sec @1000 periods per statistic; the ragged
+batching speed-up over per-call launches; the refined LS period error.  The
+default run measures all three and prints ONE JSON line:
 
-Metric: pts/sec @1000 periods = (curves x points, summed over the 5
-statistics and 3 sizes) / wall seconds per step.  Also reported: point-period
-evaluations per second, the per-kernel roofline fraction (HIP events on the
-kernel's stream), the drop-in calc() rate from host arrays, and the CPU
-oracle (C++/OpenMP restatement of the Rust path) timed on this host.
-
-Prints ONE JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
+- `value` (batch workload, one "step"): the five statistics CE(10,10),
+  AOV(15), LS, FPW(10) and BLS(50, 0.01, 0.5), each over 100 curves at each of
+  1024, 4096 and 8192 points (mixed sinusoid / box-eclipse / noise curves,
+  times uniform over 1000 days), 1000 trial periods linspace(0.5, 10),
+  period_dts [0], output 'stats'.  Inputs are resident in HBM before the
+  timed region; a step evaluates the 15 grids and per-curve statistics on the
+  device and copies the statistics back.  value = (curves x points summed
+  over the 15 jobs) / wall seconds per step, i.e. the pts/sec of all five
+  statistics together; `per_algo` gives each statistic's own rate.
+- `dropin`: the reference's own protocol (benchmarks/throughput_bench.py:
+  100 sinusoids, 1000 periods, calc(output='stats') from host arrays, median
+  of 3 after 1 warmup) at 256, 1024, 4096 and 8192 points, next to the
+  published 1x P100 and 28-core CPU figures.
+- `ragged`: 10,000 curves of log-uniform length in [64, 16384] (gappy
+  cadence), 20,000 frequencies x 3 period_dts, CE and BLS peaks: the
+  persistent graph-captured launch against one C-ABI call per curve and
+  against one plain launch.
+- `ls_refine`: 1,000 sinusoids x 2,048 points, 10,000 frequencies, top-8
+  peaks refined in FP64; error against the long-double oracle.
+Also: the dominant batch kernel's roofline fraction (HIP events on the
+kernel's stream) and the CPU oracle (C++/OpenMP restatement of the Rust path)
+timed on this host.  --workload selects one part alone (or the heavy and
+sibling workloads).  Only one GPU is used: --gpus > 1 exits with a message.
 """
 
 import argparse
@@ -75,8 +87,32 @@ PEAK_VALU_TFLOPS = 157.3  # MI355X FP32 vector, published
 PEAK_F64_TFLOPS = 78.6    # MI355X FP64 vector, published
 # Arithmetic of each statistic's hot loop, and the peak it is held against.
 PEAK_OF = {"mhf": PEAK_F64_TFLOPS}
-# Published 1x P100 pts/sec at 8192 points (benchmarks/throughput_results_1gpu.csv:49-81).
-P100_8192 = {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510}
+METRIC = "pts/sec @1000 periods per algo; ragged vs per-call speedup; LS refine err"
+# Published pts/sec of the reference (benchmarks/throughput_results_1gpu.csv):
+# 1x Tesla P100 (rows 44-81) and Rust/Rayon on a 28-core Skylake Xeon (rows 4-41).
+PUBLISHED_P100 = {
+    256: {"ce": 1118040, "aov": 1104843, "ls": 1241167, "fpw": 1104035, "bls": 1015487},
+    1024: {"ce": 3835229, "aov": 3060418, "ls": 4499492, "fpw": 2660383, "bls": 3158174},
+    4096: {"ce": 9816875, "aov": 3194430, "ls": 13246978, "fpw": 3523416, "bls": 6177516},
+    8192: {"ce": 13663053, "aov": 3690899, "ls": 19784185, "fpw": 5601693, "bls": 5488510},
+}
+PUBLISHED_CPU28 = {
+    256: {"ce": 139518, "aov": 184434, "ls": 145505, "fpw": 245282, "bls": 120590},
+    1024: {"ce": 175612, "aov": 210792, "ls": 180641, "fpw": 290322, "bls": 227534},
+    4096: {"ce": 185223, "aov": 217435, "ls": 193907, "fpw": 306540, "bls": 293268},
+    8192: {"ce": 186445, "aov": 218622, "ls": 198511, "fpw": 309305, "bls": 307071},
+}
+DROPIN_SIZES = (256, 1024, 4096, 8192)
+
+
+def host_threads():
+    """CPU threads the oracle may use here: the process's affinity, capped by
+    OMP_NUM_THREADS when set."""
+    threads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        threads = min(threads, int(omp))
+    return threads
 
 
 def measured_traffic(kernel):
@@ -181,10 +217,7 @@ def cpu_baseline(datasets, budget_s):
     the first `k` curves of each size, all five statistics."""
     from oracle import oracle as O
 
-    threads = len(os.sched_getaffinity(0))
-    omp = os.environ.get("OMP_NUM_THREADS")
-    if omp and omp.isdigit():
-        threads = min(threads, int(omp))
+    threads = host_threads()
     # Calibrate: one small run, then size the sample to ~budget_s.
     t, m, e, P, D = datasets[SIZES[0]]
     t0 = time.perf_counter()
@@ -222,33 +255,53 @@ def cpu_baseline(datasets, budget_s):
     }
 
 
-def drop_in_rate(datasets, reps=2):
-    """Reference-style wall time: calc(..., output='stats') from host arrays."""
+def dropin_leg(reps=3, warmup=1):
+    """The reference's throughput protocol (benchmarks/throughput_bench.py:
+    79-102, 129-130) on this path: 100 of the reference's sinusoids
+    (synth.reference_bench_curves) at each size, 1000 periods, one
+    calc(output='stats') call from host arrays per repeat, median of `reps`
+    after `warmup` calls; pts/sec = curves x points / median wall seconds.
+    The wall time includes the host marshalling, the transfers, the device
+    grid and statistics and the Statistics objects, as the reference's does."""
     import periodfind_amd.gpu as G
+    from periodfind_amd import synth
 
     cls = {"ce": G.ConditionalEntropy, "aov": G.AOV, "ls": G.LombScargle, "fpw": G.FPW,
            "bls": G.BoxLeastSquares}
-    res = {}
-    for algo in ALGOS:
-        obj = cls[algo](**PARAMS[algo] if algo != "aov" else {"n_phase": 15})
-        t, m, e, P, D = datasets[8192]
-        kw = {"errs": e} if algo in ("fpw", "bls") else {}
-        obj.calc(t, m, P, D, output="stats", **kw)
-        best = 1e9
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            obj.calc(t, m, P, D, output="stats", **kw)
-            best = min(best, time.perf_counter() - t0)
-        res[algo] = len(t) * 8192 / best
-    return res
+    P = np.linspace(0.5, 10.0, 1000, dtype=np.float32)
+    D = np.zeros(1, np.float32)
+    rates = {a: {} for a in ALGOS}
+    for n in DROPIN_SIZES:
+        t, m, e = synth.reference_bench_curves(N_CURVES, n)
+        for algo in ALGOS:
+            obj = cls[algo](**PARAMS[algo])
+            kw = {"errs": e} if algo in ("fpw", "bls") else {}
+            for _ in range(warmup):
+                obj.calc(t, m, P, D, output="stats", **kw)
+            el = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                obj.calc(t, m, P, D, output="stats", **kw)
+                el.append(time.perf_counter() - t0)
+            rates[algo][n] = N_CURVES * n / float(np.median(el))
+    return {
+        "what": "calc(output='stats') from host arrays, 100 sinusoids (reference benchmark "
+                "curves), 1000 periods, median of 3 after 1 warmup",
+        "pts_per_s": {a: {str(n): round(v) for n, v in r.items()} for a, r in rates.items()},
+        "vs_p100_published": {a: {str(n): round(v / PUBLISHED_P100[n][a], 1)
+                                  for n, v in r.items()} for a, r in rates.items()},
+        "vs_cpu28_published": {a: {str(n): round(v / PUBLISHED_CPU28[n][a], 1)
+                                   for n, v in r.items()} for a, r in rates.items()},
+    }
 
 
-def run_ragged(args):
+def ragged_leg(args, reps=3):
     """Ragged workload: 10,000 curves, lengths log-uniform in [64, 16384], gappy
     cadence, 20,000 frequencies x 3 period_dts, CE(10,10) and BLS(50), peaks
-    (n_peaks=32).  Compares the ragged persistent + hipGraph path with the plain
-    per-call path (one C-ABI call per curve, as the reference launches per
-    curve) and with the single plain grid launch."""
+    (n_peaks=32).  Compares the ragged persistent + hipGraph path (median of
+    `reps` replays) with the plain per-call path (one C-ABI call per curve, as
+    the reference launches per curve) and with the single plain grid launch,
+    and reports whether all three give the same peaks."""
     from periodfind_amd import capi, synth
 
     print("bench: building ragged workload", file=sys.stderr, flush=True)
@@ -268,24 +321,29 @@ def run_ragged(args):
         cs = capi.CurveSet(ctx, t, mags, errs)
         print(f"bench: {algo} ragged", file=sys.stderr, flush=True)
         gi, gv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)  # capture
-        best = 1e9
-        for _ in range(max(1, args.steps)):
+        walls, gms = [], []
+        for _ in range(max(1, reps)):
             t0 = time.perf_counter()
             gi, gv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
-            best = min(best, time.perf_counter() - t0)
-        graph_ms = ctx.last_kernel_ms()
+            walls.append(time.perf_counter() - t0)
+            gms.append(ctx.last_kernel_ms())
+        best = float(np.median(walls))
+        graph_ms = float(np.median(gms))
         tuned = [e for e in ctx.get_tuning() if e[0] == algo]
         saved = ctx.get_tuning()
         ctx.set_tuning([])  # built-in defaults, for comparison
         cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
         t0 = time.perf_counter()
-        cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
+        ui, uv = cs.peaks(algo, P, D, 32, 1, ragged=True, **prm)
         default_s = time.perf_counter() - t0
         ctx.set_tuning(saved)
         print(f"bench: {algo} plain single launch", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        pi, pv = cs.peaks(algo, P, D, 32, 1, ragged=False, **prm)
-        plain_s = time.perf_counter() - t0
+        pw = []
+        for _ in range(max(1, reps)):
+            t0 = time.perf_counter()
+            pi, pv = cs.peaks(algo, P, D, 32, 1, ragged=False, **prm)
+            pw.append(time.perf_counter() - t0)
+        plain_s = float(np.median(pw))
         cs.close()
         print(f"bench: {algo} per-call", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
@@ -306,6 +364,7 @@ def run_ragged(args):
             "pts_per_s": npts / best, "evals_per_s": evals / best,
             "identical_to_per_call": same / len(t),
             "identical_plain": bool(np.array_equal(pi, gi) and np.array_equal(pv, gv)),
+            "identical_untuned": bool(np.array_equal(ui, gi) and np.array_equal(uv, gv)),
         }
     ctx.close()
     if args.cpu_seconds > 0:
@@ -315,10 +374,7 @@ def run_ragged(args):
         lens = np.array([len(x) for x in t])
         order = np.argsort(lens)
         pick = order[np.linspace(0, len(t) - 1, 40).astype(int)]
-        threads = len(os.sched_getaffinity(0))
-        omp = os.environ.get("OMP_NUM_THREADS")
-        if omp and omp.isdigit():
-            threads = min(threads, int(omp))
+        threads = host_threads()
         cpu = {}
         for algo in ("ce", "bls"):
             mags = prep_mags(algo, [m[i] for i in pick])
@@ -336,11 +392,16 @@ def run_ragged(args):
                                **cpu}
     sp = [v["speedup_vs_per_call"] for v in out["algos"].values()]
     out["value"] = float(np.exp(np.mean(np.log(sp))))
-    out["steps"] = args.steps
-    print(json.dumps(out), flush=True)
+    out["value_definition"] = "geometric mean over CE and BLS of per-call time / ragged time"
+    out["reps"] = reps
+    return out
 
 
-def run_lsrefine(args):
+def run_ragged(args):
+    print(json.dumps(ragged_leg(args)), flush=True)
+
+
+def lsrefine_leg(args, steps=None, extras=True):
     """Differentiable-LS workload: 1000 sinusoids (SNR U[3,30]) x 2048 points,
     10,000-frequency grid linspace(0.1, 2); f32 LS grid + device top-8 peaks +
     FP64 Newton refinement (pf_curves_ls_peaks_refined) on resident curves.
@@ -355,10 +416,11 @@ def run_lsrefine(args):
     ctx = capi.Context(0)
     cs = capi.CurveSet(ctx, t, y)
     K = 8
+    steps = args.steps if steps is None else steps
     for _ in range(max(1, args.warmup)):
         cs.ls_peaks_refined(F, n_peaks=K)
     walls, grid_ms = [], []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t0 = time.perf_counter()
         idx, val, fr, pw, st = cs.ls_peaks_refined(F, n_peaks=K)
         walls.append(time.perf_counter() - t0)
@@ -370,7 +432,7 @@ def run_lsrefine(args):
     fbest = fr[np.arange(C), best]
     recovered = float(np.mean(np.abs(1.0 / fbest - true_p) / true_p < 1e-3))
     out = {"metric": "LS refine err", "unit": "max relative period error vs FP64 oracle",
-           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": False, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "data": "synthetic", "dtype": "f32 grid, f64 refinement",
            "config": {"workload": "differentiable LS: 1000 sinusoids (SNR U[3,30]) x 2048 "
                                   "points, 10000 freqs linspace(0.1,2), top-8 peaks refined",
@@ -396,7 +458,11 @@ def run_lsrefine(args):
                          "max_rel": float(rel.max()) if rel.size else None,
                          "median_rel": float(np.median(rel)) if rel.size else None,
                          "status_mismatch": int((st[:n_or] != rst).sum()),
-                         "grid_only_max_rel": float(grid_rel.max()) if rel.size else None}
+                         "grid_only_max_rel": float(grid_rel.max()) if rel.size else None,
+                         "oracle_s": round(or_s, 2)}
+    if not extras:
+        ctx.close()
+        return out
 
     # The consumer of the refined peaks: Fourier decomposition features
     # (rust/src/fourier.rs:68-230) of every curve at its best refined period,
@@ -415,10 +481,7 @@ def run_lsrefine(args):
     ctx2.close()
     f_kt = float(np.median(f_ms)) * 1e-3
     f_flops = C * N * FLOPS_INNER["fourier"] + C * FLOPS_TRIAL["fourier"]
-    threads = len(os.sched_getaffinity(0))
-    omp = os.environ.get("OMP_NUM_THREADS")
-    if omp and omp.isdigit():
-        threads = min(threads, int(omp))
+    threads = host_threads()
     ones = [np.ones(len(x), np.float32) for x in t]
     t0 = time.perf_counter()
     want = O.fourier_features(t, m, ones, Pbest, threads=threads)
@@ -458,10 +521,7 @@ def run_lsrefine(args):
 
     if args.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
-        threads = len(os.sched_getaffinity(0))
-        omp = os.environ.get("OMP_NUM_THREADS")
-        if omp and omp.isdigit():
-            threads = min(threads, int(omp))
+        threads = host_threads()
         Pf = (1.0 / F).astype(np.float32)
 
         def cpu_leg(k):
@@ -479,7 +539,11 @@ def run_lsrefine(args):
                                "sample": f"{k} curves: f32 LS grid (OpenMP), peaks, long-double "
                                          f"refinement (1 thread) ({dt:.1f} s)"}
         out["gpu_over_cpu"] = round(out["curves_per_s"] / out["cpu_baseline"]["value"], 1)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_lsrefine(args):
+    print(json.dumps(lsrefine_leg(args)), flush=True)
 
 
 def run_heavy(args):
@@ -534,10 +598,7 @@ def run_heavy(args):
         from oracle import oracle as O
 
         print("bench: CPU baseline", file=sys.stderr, flush=True)
-        threads = len(os.sched_getaffinity(0))
-        omp = os.environ.get("OMP_NUM_THREADS")
-        if omp and omp.isdigit():
-            threads = min(threads, int(omp))
+        threads = host_threads()
         cpu = {}
         for algo in ("ce", "bls"):
             mags = prep_mags(algo, m[:1])
@@ -628,10 +689,7 @@ def run_siblings(args):
            "kernels": detail}
     if args.cpu_seconds > 0:
         print("bench: siblings CPU baseline", file=sys.stderr, flush=True)
-        threads = len(os.sched_getaffinity(0))
-        omp = os.environ.get("OMP_NUM_THREADS")
-        if omp and omp.isdigit():
-            threads = min(threads, int(omp))
+        threads = host_threads()
         t, m, e, P, D = datasets[8192]
         cpu = {}
         for algo in SIBLINGS:
@@ -674,7 +732,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU-baseline budget (0 disables)")
-    ap.add_argument("--no-drop-in", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true",
+                    help="skip the drop-in calc() leg (reference protocol, host arrays)")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged leg")
+    ap.add_argument("--no-ls-refine", action="store_true", help="skip the LS refinement leg")
     ap.add_argument("--submission", choices=("many", "serial"), default="many",
                     help="batch workload: many = one pf_curves_stats_many call per step "
                          "(default); serial = one pf_curves_stats call per (statistic, size). "
@@ -736,6 +797,11 @@ def main():
     pts_per_step = sum(N_CURVES * n for (_, n) in sets)
     serial = {"value": pts_per_step / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,
               "submission": "one synchronous pf_curves_stats call per (statistic, size)"}
+    # Each statistic's own rate: its 3 jobs' points over their serial call times.
+    per_algo = {}
+    for algo in ALGOS:
+        secs = sum(float(np.median(walls[(algo, n)])) for n in SIZES)
+        per_algo[algo] = N_CURVES * sum(SIZES) / secs
 
     # The same 15 (statistic, size) evaluations submitted as one
     # pf_curves_stats_many call per step: grid + device stats of every job on
@@ -764,8 +830,11 @@ def main():
     dom, kt, ach = best
 
     result = {
-        "metric": "pts/sec @1000 periods per algo",
+        "metric": METRIC,
         "value": value,
+        "value_definition": "pts/sec @1000 periods of the whole batch step: curves x points "
+                            "summed over 5 statistics x {1024,4096,8192} points, resident "
+                            "inputs, divided by the step's wall time (per statistic: per_algo)",
         "unit": "pts/sec",
         "n_gpus": 1,
         "steps": args.steps,
@@ -783,7 +852,10 @@ def main():
                      "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(ach / PEAK_VALU_TFLOPS, 4), "traffic": measured_traffic(dom),
                      "counters": measured_util(dom)},
-        "evals_per_s": pts_per_step * 1000 / (wall / args.steps),
+        "evals_per_s": value * 1000,
+        "per_algo": {"pts_per_s": {a: round(v) for a, v in per_algo.items()},
+                     "what": "each statistic alone: 100 curves x (1024+4096+8192) points over "
+                             "the sum of its three resident grid + stats calls"},
         "submission": ("one pf_curves_stats_many call per step: 15 (statistic, size) jobs, "
                        "grid + device stats each, longest first over 4 streams, one "
                        "synchronisation" if args.submission == "many" else serial["submission"]),
@@ -797,10 +869,7 @@ def main():
 
     if not args.no_drop_in:
         print("bench: drop-in calc() from host arrays", file=sys.stderr, flush=True)
-        di = drop_in_rate(datasets)
-        result["drop_in_pts_per_s_8192"] = {a: round(v) for a, v in di.items()}
-        result["drop_in_vs_p100_published_8192"] = {a: round(di[a] / P100_8192[a], 1)
-                                                    for a in ALGOS}
+        result["dropin"] = dropin_leg()
     if args.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(datasets, args.cpu_seconds)
@@ -811,6 +880,17 @@ def main():
                 for a in ALGOS}
         result["gpu_over_cpu_8192"] = {
             a: round(gpu8[a] / result["cpu_baseline"]["pts_per_s_8192"][a], 1) for a in ALGOS}
+    if not args.no_ragged:
+        rg = ragged_leg(args)
+        result["ragged"] = {k: rg[k] for k in ("value", "value_definition", "unit", "config",
+                                              "algos", "cpu_baseline") if k in rg}
+    if not args.no_ls_refine:
+        ls = lsrefine_leg(args, steps=min(args.steps, 5), extras=False)
+        result["ls_refine"] = {k: ls[k] for k in ("value", "unit", "config", "pipeline_ms",
+                                                 "grid_kernel_ms", "curves_per_s",
+                                                 "status_counts",
+                                                 "best_peak_recovers_true_period",
+                                                 "refine_err") if k in ls}
     print(json.dumps(result), flush=True)
 
 

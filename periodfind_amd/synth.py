@@ -40,6 +40,22 @@ def uniform_curves(n_curves, n_points, baseline=1000.0, seed=SEED_BASE + 2, mix=
     return times, mags, errs
 
 
+def reference_bench_curves(n_curves, n_points, seed=42):
+    """The curves of the reference's own throughput benchmark
+    (benchmarks/throughput_bench.py:44-57): times sorted uniform in [0, 100],
+    mags = sin(2 pi t / 2.5) + N(0, 0.1), errs = 0.1, drawn from a legacy
+    RandomState(42), curve by curve."""
+    rng = np.random.RandomState(seed)
+    times, mags, errs = [], [], []
+    for _ in range(n_curves):
+        t = np.sort(rng.uniform(0, 100, n_points)).astype(np.float32)
+        m = (np.sin(2 * np.pi * t / 2.5) + rng.normal(0, 0.1, n_points)).astype(np.float32)
+        times.append(t)
+        mags.append(m)
+        errs.append(np.full(n_points, 0.1, dtype=np.float32))
+    return times, mags, errs
+
+
 def config1():
     """100 curves x 256 points, sinusoids, CE(10,10), 1000 periods."""
     t, m, e = uniform_curves(100, 256, seed=SEED_BASE + 1, mix=False)
