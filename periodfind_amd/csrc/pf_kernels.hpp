@@ -207,10 +207,21 @@ hipError_t launch_stats(const float* data, int64_t n_curves, int64_t T, int n_to
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 hipError_t noalloc_scratch(int device, void** ptr);
 
-// Peak selection (pf_peaks.hip).
+// Peak selection (pf_peaks.hip).  With a workspace, curves too few to fill
+// the device are scanned in segments by several waves and merged (exact for
+// min_distance 1, the only case split); without one, one wave per curve.
+constexpr int kPeaksSplitWaves = 2048;   // waves a split launch aims for
+constexpr int64_t kPeaksMinSegment = 512;  // fewest samples per segment
+constexpr int kPeaksMaxSegments = 256;
+struct PeaksWorkspace {
+    int64_t* idx;  // [n_curves * segments * n_peaks]
+    float* val;
+};
+// Segments per curve the split launch uses (1: no split).
+int peaks_segments(int64_t n_curves, int64_t T, int64_t min_distance);
 hipError_t launch_peaks(const float* data, int64_t n_curves, int64_t T, int64_t n_peaks,
                         int64_t min_distance, int use_max, int64_t* idx, float* val,
-                        hipStream_t s);
+                        hipStream_t s, const PeaksWorkspace* ws = nullptr);
 
 // Differentiable Lomb-Scargle in FP64 (pf_lsdiff.hip).  y is float when
 // y_f32, else double; offsets, t, y, omega [n_curves][K] are device arrays.

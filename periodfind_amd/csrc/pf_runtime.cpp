@@ -213,6 +213,8 @@ struct pf_ctx {
     std::vector<hipEvent_t> job_ev;
     // LS refinement: frequency grid (f64) and per-peak brackets.
     DevBuf ls_freq, ls_lo, ls_hi;
+    // Segment selections of a split peak scan (see pf::launch_peaks).
+    DevBuf peaks_ws;
     // Matched Filter templates, one table per bin count (rows of
     // mf_template_stride(nb)); a call with several MF jobs of different bin
     // counts reads each job's own table.
@@ -1056,6 +1058,19 @@ T* out_dev(pf_ctx* ctx, size_t off) {
     return reinterpret_cast<T*>(static_cast<char*>(ctx->outbuf.p) + off);
 }
 
+// Workspace of a peak scan over n_curves x T, grown as the split needs it
+// (the split reads it only when it splits; see pf::peaks_segments).
+int peaks_workspace(pf_ctx* ctx, int64_t n_curves, int64_t T, int64_t n_peaks,
+                    int64_t min_distance, pf::PeaksWorkspace* ws) {
+    const int n_seg = pf::peaks_segments(n_curves, T, min_distance);
+    const size_t n = static_cast<size_t>(n_curves) * n_seg * static_cast<size_t>(n_peaks);
+    const size_t idx_bytes = (n * sizeof(int64_t) + 255) & ~size_t(255);
+    PF_TRY(ensure(ctx->peaks_ws, idx_bytes + n * sizeof(float)));
+    ws->idx = as<int64_t>(ctx->peaks_ws);
+    ws->val = reinterpret_cast<float*>(static_cast<char*>(ctx->peaks_ws.p) + idx_bytes);
+    return PF_OK;
+}
+
 int ensure_host(pf_ctx* ctx, size_t bytes) {
     if (ctx->host_cap >= bytes) return PF_OK;
     if (ctx->host_stage) {
@@ -1139,10 +1154,12 @@ int peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* periods, i
     const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
     const size_t o_val = plan.add(val, pk * sizeof(float));
     PF_TRY(ensure(ctx->outbuf, plan.total));
+    pf::PeaksWorkspace ws;
+    PF_TRY(peaks_workspace(ctx, cs.n_curves, T, n_peaks, min_distance, &ws));
     const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
     PF_HIP(pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
                             out_dev<int64_t>(ctx, o_idx), out_dev<float>(ctx, o_val),
-                            ctx->stream));
+                            ctx->stream, &ws));
     return out_finish(ctx, plan);
 }
 
@@ -1226,6 +1243,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
     const size_t o_val = plan.add(val, pk * sizeof(float));
     PF_TRY(ensure(ctx->outbuf, plan.total));
+    pf::PeaksWorkspace ws;
+    PF_TRY(peaks_workspace(ctx, cs.n_curves, T, n_peaks, min_distance, &ws));
     PF_TRY(ensure_order(ctx, cs));
     const int64_t tblocks = (T + pf::kWaveHost - 1) / pf::kWaveHost;
     // Length buckets of the tuning table over the longest-first order: each
@@ -1284,7 +1303,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         static_cast<uint64_t>(p.bandwidth), static_cast<uint64_t>(p.margin),
         reinterpret_cast<uint64_t>(cs.mhf_pt.p), reinterpret_cast<uint64_t>(cs.mhf_sums.p),
         static_cast<uint64_t>(mhf_max_k(p.max_harmonics)),
-        static_cast<uint64_t>(p.algo == PF_ALGO_LS ? ls_fast_mode() : 0)};
+        static_cast<uint64_t>(p.algo == PF_ALGO_LS ? ls_fast_mode() : 0),
+        reinterpret_cast<uint64_t>(ws.idx), reinterpret_cast<uint64_t>(ws.val)};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
         key.push_back(static_cast<uint64_t>(b.r1));
@@ -1315,8 +1335,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         }
         if (rc == PF_OK &&
             pf::launch_peaks(as<float>(cs.out), cs.n_curves, T, n_peaks, min_distance, use_max,
-                             out_dev<int64_t>(ctx, o_idx), out_dev<float>(ctx, o_val),
-                             s) != hipSuccess)
+                             out_dev<int64_t>(ctx, o_idx), out_dev<float>(ctx, o_val), s,
+                             &ws) != hipSuccess)
             rc = fail(PF_E_HIP, "peaks launch failed during capture");
         hipGraph_t graph = nullptr;
         hipError_t ec = hipStreamEndCapture(s, &graph);
@@ -1468,8 +1488,10 @@ int ls_refine_on(pf_ctx* ctx, CurveSet& cs, const double* freqs, int64_t nf, int
     double* d_w = out_dev<double>(ctx, o_w);
     double* d_p = out_dev<double>(ctx, o_p);
     int* d_st = out_dev<int>(ctx, o_st);
+    pf::PeaksWorkspace ws;
+    PF_TRY(peaks_workspace(ctx, C, nf, n_peaks, min_distance, &ws));
     PF_HIP(pf::launch_peaks(as<float>(cs.out), C, nf, n_peaks, min_distance, 1, d_idx,
-                            out_dev<float>(ctx, o_val), s));
+                            out_dev<float>(ctx, o_val), s, &ws));
     PF_TRY(ensure(ctx->ls_freq, nf * sizeof(double)));
     PF_TRY(ensure(ctx->ls_lo, pk * sizeof(double)));
     PF_TRY(ensure(ctx->ls_hi, pk * sizeof(double)));
@@ -1560,7 +1582,7 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->outbuf, &ctx->ls_freq, &ctx->ls_lo,
-                      &ctx->ls_hi};
+                      &ctx->ls_hi, &ctx->peaks_ws};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (pf_ctx::MfTable& m : ctx->mf_tables)
@@ -1956,10 +1978,12 @@ int pf_find_top_peaks(pf_ctx* ctx, const float* data, int64_t n_curves, int64_t 
     const size_t o_idx = plan.add(idx, pk * sizeof(int64_t));
     const size_t o_val = plan.add(val, pk * sizeof(float));
     PF_TRY(ensure(ctx->outbuf, plan.total));
+    pf::PeaksWorkspace ws;
+    PF_TRY(peaks_workspace(ctx, n_curves, T, n_peaks, min_distance, &ws));
     PF_HIP(hipMemcpyAsync(cs.out.p, data, db, hipMemcpyHostToDevice, ctx->stream));
     PF_HIP(pf::launch_peaks(as<float>(cs.out), n_curves, T, n_peaks, min_distance,
                             use_max ? 1 : 0, out_dev<int64_t>(ctx, o_idx),
-                            out_dev<float>(ctx, o_val), ctx->stream));
+                            out_dev<float>(ctx, o_val), ctx->stream, &ws));
     ctx->timed = false;
     return out_finish(ctx, plan);
 }

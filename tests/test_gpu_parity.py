@@ -258,6 +258,42 @@ def test_find_top_peaks_device(ctx):
                 np.testing.assert_array_equal(gv, rv)
 
 
+@pytest.mark.parametrize("n_curves,T", [(1, 60000), (1, 1025), (1, 1026), (3, 5000),
+                                         (7, 200000), (1200, 700)])
+def test_find_top_peaks_split_scan(ctx, n_curves, T):
+    """Few curves are scanned in segments by several waves and merged; with
+    min_distance 1 that is exactly the serial scan.  Values quantized to
+    force ties, plus NaN and infinities; the oracle is the serial finder."""
+    rng = np.random.default_rng(T + n_curves)
+    data = np.round(rng.normal(size=(n_curves, T)) * 8.0).astype(np.float32) / 8.0
+    data[0, 5::997] = np.nan
+    data[0, 11::1999] = np.inf
+    data[-1, 17::3001] = -np.inf
+    for use_max in (True, False):
+        for n_peaks in (1, 8, 32, 64):
+            for md in (1, 2):
+                ri, rv = O.find_top_peaks_batched(data, n_peaks, md, use_max)
+                gi, gv = ctx.find_top_peaks(data, n_peaks, md, use_max)
+                np.testing.assert_array_equal(gi, ri, err_msg=f"{use_max} {n_peaks} {md}")
+                np.testing.assert_array_equal(gv, rv, err_msg=f"{use_max} {n_peaks} {md}")
+
+
+def test_single_curve_peaks_match_oracle(ctx):
+    """One curve per call, as the per-call ragged baseline makes: the grid of
+    60,000 trials, then the split peak scan, against the oracle."""
+    t, m, e = _curves([3000], seed=23)
+    P = (1.0 / np.linspace(0.05, 20.0, 20000)).astype(np.float32)
+    D = np.array([0.0, 1e-5, -1e-5], np.float32)
+    for algo in ("bls", "aov"):
+        prm = PARAMS[algo] if algo != "aov" else dict(n_phase=15)
+        gi, gv = ctx.peaks(algo, t, m, P, D, errs=e if algo == "bls" else None, n_peaks=32,
+                           min_distance=1, **prm)
+        ref = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+        oi, ov = O.find_top_peaks_batched(ref, 32, 1, use_max=True)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gv, ov)
+
+
 @pytest.mark.parametrize("algo,n_stats,shape", [
     ("ce", 1, (1000, 1)), ("bls", 3, (1000, 1)), ("ls", 5, (333, 3)), ("fpw", 1, (4096, 2)),
     ("aov", 2, (128, 1)),
