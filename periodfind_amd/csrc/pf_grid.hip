@@ -416,9 +416,23 @@ __device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, c
     epi(count, cell_count, ncell, n, col);
 }
 
+// Largest CE normaliser (points x extents) whose probabilities take the
+// division-free quotient.
+constexpr float kCeFastDivMax = 131072.0f;
+// Cells of a phase row taken together in the CE epilogue, so that their
+// count and table reads are in flight at once.
+constexpr int kCeCells = 5;
+
 template <bool HAS_PDT, bool EXT1, int G>
 __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, const LaneTrial lt) {
     constexpr bool PACK = EXT1 && G < 0;
+    // The epilogue's logf table (1/c, log c) in LDS: the subinterval varies
+    // from lane to lane, and an LDS read is the shortest wait for it.
+    __shared__ double2 logtab[16];
+    if (threadIdx.x < 16)
+        logtab[threadIdx.x] = make_double2(logf_glibc_data::kTab[threadIdx.x][0],
+                                           logf_glibc_data::kTab[threadIdx.x][1]);
+    __syncthreads();
     hist_block<HAS_PDT, EXT1, G>(g, a, lt, [&](auto count, auto cell_count, int ncell, int64_t n,
                                                  uint32_t*) {
     // ce.rs:66-86: p = count / (N*ext_p*ext_m); CE = sum_j sum_i p_ij ln(p_j/p_ij).
@@ -435,18 +449,73 @@ __device__ __forceinline__ void ce_block(const GridArgs& g, const CeArgs& a, con
             one_cell = total != static_cast<uint32_t>(n);
         }
     }
-    float ce = 0.0f;
-    for (int j = 0; j < a.n_phase && !one_cell; ++j) {
-        float pj = 0.0f;
-        for (int i = 0; i < a.n_mag; ++i) pj += static_cast<float>(count(j, i)) / norm;
-        for (int i = 0; i < a.n_mag; ++i) {
-            const float pij = static_cast<float>(count(j, i)) / norm;
-            if (pij != 0.0f) {
-                const float term = pij * logf_glibc(pj / pij);
-                ce += term;
+    // p = fl(count / norm): for norm <= kCeFastDivMax by one reciprocal
+    // multiply and one FMA correction, which gives the IEEE quotient for every
+    // integer count <= norm (tests/exactness/check_count_div.c checks all of
+    // them); above that, the division itself.
+    const float rn = 1.0f / norm;
+    const int nm = a.n_mag;
+    auto epilogue = [&](auto fast_tag) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        auto prob = [&](uint32_t c) -> float {
+            const float cf = static_cast<float>(c);
+            if constexpr (FAST) {
+                const float q0 = cf * rn;
+                return __builtin_fmaf(__builtin_fmaf(-q0, norm, cf), rn, q0);
+            } else {
+                return cf / norm;
+            }
+        };
+        // Cells i0 .. i0 + kCeCells - 1 of row j; past the row's end the
+        // last cell is read again and `valid` is false.
+        auto cells = [&](int j, int i0, float (&p)[kCeCells], bool (&valid)[kCeCells]) {
+            uint32_t c[kCeCells];
+#pragma unroll
+            for (int u = 0; u < kCeCells; ++u) c[u] = count(j, min(i0 + u, nm - 1));
+#pragma unroll
+            for (int u = 0; u < kCeCells; ++u) {
+                p[u] = prob(c[u]);
+                valid[u] = i0 + u < nm;
+            }
+        };
+        // Every p is >= +0, so p_j and the sum never hold -0 and adding +0
+        // in place of a skipped cell leaves them unchanged.
+        float ce = 0.0f;
+        for (int j = 0; j < a.n_phase && !one_cell; ++j) {
+            float pj = 0.0f;
+            for (int i0 = 0; i0 < nm; i0 += kCeCells) {
+                float p[kCeCells];
+                bool valid[kCeCells];
+                cells(j, i0, p, valid);
+#pragma unroll
+                for (int u = 0; u < kCeCells; ++u) pj += valid[u] ? p[u] : 0.0f;
+            }
+            for (int i0 = 0; i0 < nm; i0 += kCeCells) {
+                float pij[kCeCells];
+                bool valid[kCeCells];
+                cells(j, i0, pij, valid);
+                // p_j >= p_ij (a rounded sum of non-negative terms never falls
+                // below one of them), so p_j / p_ij is a positive normal float
+                // wherever p_ij != 0: logf's special cases cannot arise.
+                LogfSplit sp[kCeCells];
+                double2 tab[kCeCells];
+#pragma unroll
+                for (int u = 0; u < kCeCells; ++u) {
+                    const float x = pij[u] != 0.0f ? pj / pij[u] : 1.0f;
+                    sp[u] = logf_glibc_split(__builtin_bit_cast(uint32_t, x));
+                    tab[u] = logtab[sp[u].i];
+                }
+#pragma unroll
+                for (int u = 0; u < kCeCells; ++u) {
+                    const float term = pij[u] * logf_glibc_eval(sp[u], tab[u].x, tab[u].y);
+                    ce += valid[u] && pij[u] != 0.0f ? term : 0.0f;
+                }
             }
         }
-    }
+        return ce;
+    };
+    const float ce = norm <= kCeFastDivMax ? epilogue(std::true_type{})
+                                           : epilogue(std::false_type{});
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = ce;
     });
 }

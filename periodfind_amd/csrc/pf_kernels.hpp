@@ -161,6 +161,8 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);  // n <
 
 // LDS bytes per wave of each binned kernel.
 inline size_t lds_ce(int n_phase, int n_mag) { return size_t(n_phase + 1) * n_mag * 64 * 4; }
+// The CE kernel's static LDS beside the histogram: its 16-entry logf table.
+constexpr size_t kCeStaticLds = 16 * 16;
 // CE histogram with two 16-bit counts per 32-bit word.  Up to 65535 points no
 // count can carry into its neighbour; at exactly 65536 a carry happens only
 // when every point falls in one cell, which the kernel detects (the decoded

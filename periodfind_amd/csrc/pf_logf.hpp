@@ -64,8 +64,39 @@ constexpr double kTab[16][2] = {
 
 }  // namespace logf_glibc_data
 
-PF_HD inline float logf_glibc(float x) {
+// The algorithm in two steps for a positive, normal, finite x (no special
+// cases, no branches): logf_glibc_split reduces x to its subinterval i,
+// exponent k and z; logf_glibc_eval finishes with that subinterval's table
+// entry (1/c, log c).  A caller may keep the table where it is cheapest to
+// read (CE keeps it in LDS).
+struct LogfSplit {
+    uint32_t i;  // subinterval, 0..15
+    int k;       // exponent
+    double z;    // x / 2^k, in [OFF, 2 OFF)
+};
+
+PF_HD inline LogfSplit logf_glibc_split(uint32_t ix) {
+    const uint32_t tmp = ix - logf_glibc_data::kOff;
+    LogfSplit s;
+    s.i = (tmp >> 19) & 15u;
+    s.k = static_cast<int32_t>(tmp) >> 23;  // arithmetic shift
+    const uint32_t iz = ix - (tmp & (0x1ffu << 23));
+    s.z = static_cast<double>(__builtin_bit_cast(float, iz));
+    return s;
+}
+
+PF_HD inline float logf_glibc_eval(const LogfSplit& s, double invc, double logc) {
     namespace d = logf_glibc_data;
+    const double r = __builtin_fma(s.z, invc, -1.0);
+    const double y0 = __builtin_fma(static_cast<double>(s.k), d::kLn2, logc);
+    const double r2 = r * r;
+    double y = __builtin_fma(d::kA1, r, d::kA2);
+    y = __builtin_fma(d::kA0, r2, y);
+    y = __builtin_fma(y, r2, y0 + r);
+    return static_cast<float>(y);
+}
+
+PF_HD inline float logf_glibc(float x) {
     uint32_t ix = __builtin_bit_cast(uint32_t, x);
     if (ix == 0x3f800000u) return 0.0f;  // log(1) = +0 in every rounding mode
     if (ix - 0x00800000u >= 0x7f800000u - 0x00800000u) {
@@ -75,20 +106,8 @@ PF_HD inline float logf_glibc(float x) {
         if ((ix & 0x80000000u) || (ix << 1) >= 0xff000000u) return __builtin_nanf("");
         ix = __builtin_bit_cast(uint32_t, x * 0x1p23f) - (23u << 23);  // subnormal: normalise
     }
-    const uint32_t tmp = ix - d::kOff;
-    const int i = static_cast<int>((tmp >> 19) & 15u);
-    const int k = static_cast<int32_t>(tmp) >> 23;  // arithmetic shift
-    const uint32_t iz = ix - (tmp & (0x1ffu << 23));
-    const double invc = d::kTab[i][0];
-    const double logc = d::kTab[i][1];
-    const double z = static_cast<double>(__builtin_bit_cast(float, iz));
-    const double r = __builtin_fma(z, invc, -1.0);
-    const double y0 = __builtin_fma(static_cast<double>(k), d::kLn2, logc);
-    const double r2 = r * r;
-    double y = __builtin_fma(d::kA1, r, d::kA2);
-    y = __builtin_fma(d::kA0, r2, y);
-    y = __builtin_fma(y, r2, y0 + r);
-    return static_cast<float>(y);
+    const LogfSplit s = logf_glibc_split(ix);
+    return logf_glibc_eval(s, logf_glibc_data::kTab[s.i][0], logf_glibc_data::kTab[s.i][1]);
 }
 
 }  // namespace pf

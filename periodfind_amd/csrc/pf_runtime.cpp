@@ -365,7 +365,7 @@ int check_params(const Params& p, size_t lds_max) {
         case PF_ALGO_CE:
             if (p.n_phase < 1 || p.n_mag < 1 || p.phase_ext < 1 || p.mag_ext < 1)
                 return fail(PF_E_ARG, "CE needs n_phase, n_mag, extents >= 1");
-            if (pf::lds_ce(p.n_phase, p.n_mag) > lds_max)
+            if (pf::lds_ce(p.n_phase, p.n_mag) + pf::kCeStaticLds > lds_max)
                 return fail(PF_E_LIMIT, "CE histogram %dx%d exceeds LDS (%zu B per wave)",
                             p.n_phase, p.n_mag, lds_max);
             break;

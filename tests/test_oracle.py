@@ -292,7 +292,9 @@ def test_device_division_identities():
 def test_device_logf_is_the_c_library_logf():
     """The logf the CE and VN kernels use (periodfind_amd/csrc/pf_logf.hpp,
     compiled here for the host) equals the C library's logf, which the
-    reference's f32::ln calls, on every positive float and the special values."""
+    reference's f32::ln calls, on every positive float and the special values;
+    its branch-free split/evaluate pair (the CE epilogue's path) on every
+    positive normal float."""
     src = os.path.join(HERE, "exactness", "check_logf_glibc.cpp")
     with tempfile.TemporaryDirectory() as d:
         exe = os.path.join(d, "check_logf_glibc")
@@ -300,4 +302,17 @@ def test_device_logf_is_the_c_library_logf():
                        check=True)
         r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "bad=0 special_bad=0" in r.stdout
+    assert "bad=0 special_bad=0 split_bad=0" in r.stdout
+
+
+def test_ce_probability_division_identity():
+    """CE's division-free fl(count / norm) equals the IEEE quotient for every
+    integer count <= norm <= 2^17 (the kernel's bound for that path)."""
+    src = os.path.join(HERE, "exactness", "check_count_div.c")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "check_count_div")
+        subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-o", exe, src, "-lm"],
+                       check=True)
+        r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("bad=0")
