@@ -31,12 +31,13 @@ struct LaneTrial {
     bool active;
 };
 
-__device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tblocks, int64_t T) {
+__device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tblocks, int64_t T,
+                                                int lane = static_cast<int>(threadIdx.x)) {
     uint64_t L = xcd_remap(blockIdx.x, n_blocks);
     LaneTrial lt;
     lt.curve = static_cast<int64_t>(L / static_cast<uint64_t>(tblocks));
     int64_t tb = static_cast<int64_t>(L % static_cast<uint64_t>(tblocks));
-    int64_t tr = tb * kWave + threadIdx.x;
+    int64_t tr = tb * kWave + lane;
     lt.active = tr < T;
     lt.trial = lt.active ? tr : T - 1;
     return lt;
@@ -86,12 +87,13 @@ __device__ __forceinline__ Claim next_claim(const WorkQueue& q, const GridArgs& 
 }
 
 __device__ __forceinline__ LaneTrial queue_trial(const WorkQueue& q, const GridArgs& g,
-                                                 int64_t item) {
+                                                 int64_t item,
+                                                 int lane = static_cast<int>(threadIdx.x)) {
     LaneTrial lt;
     const int64_t rank = item / g.tblocks;
     const int64_t tb = item - rank * g.tblocks;
     lt.curve = q.order[rank];
-    const int64_t tr = tb * kWave + threadIdx.x;
+    const int64_t tr = tb * kWave + lane;
     lt.active = tr < g.T;
     lt.trial = lt.active ? tr : g.T - 1;
     return lt;
@@ -1765,18 +1767,36 @@ __global__ __launch_bounds__(kWave) void fourier_kernel(const int64_t* __restric
 // (duration, phase) search (bls.rs:100-127) is serial per lane, with uniform
 // loop bounds, so lanes never diverge on them.
 //
-// Exact max: the reference keeps max over fl(num/den).  Rather than one IEEE
-// division per candidate, a candidate is first tested with one f32 multiply
-// against the current best B, num >= fl(B*den), a test that every candidate
-// with fl(num/den) > B passes; only the few that pass have fl(num/den)
-// computed by IEEE division and compared with B.
+// Exact max without a division per window: windows compare by the exact
+// cross products of their (num, den) pairs, and the best pair is divided once.
 // ---------------------------------------------------------------------------
-template <bool HAS_PDT, int G>
-__device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, const LaneTrial lt) {
+// Waves per BLS workgroup.  Measured at 100 curves x 1000 periods (ms for
+// 256 / 1024 / 4096 / 8192 points): 1 wave 0.18 / 0.26 / 0.58 / 0.99, 2 waves
+// 0.12 / 0.20 / 0.53 / 0.93, 4 waves 0.12 / 0.23 / 0.66 / 1.26 (the kernel's
+// 144 VGPRs hold 12 waves per CU, so 4 waves cost workgroups per CU).
+constexpr int kBlsWaves = 2;
+
+// The largest float <= wt * 2^-150 for wt > 0 (the product is exact in f64).
+__device__ __forceinline__ float bls_window_floor(float wt) {
+    const double x = static_cast<double>(wt) * 0x1p-150;
+    float f = static_cast<float>(x);
+    if (static_cast<double>(f) > x) f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) - 1u);
+    return f;
+}
+
+// W waves share one (curve, 64-trial block): wave 0 bins the points and forms
+// the prefix sums (serial per trial, as Rust), then every wave searches the
+// phases p = wave, wave + W, ... and the waves' best pairs are merged.  The
+// LDS is the same for any W, so W > 1 adds search issue slots at the same
+// occupancy.
+template <bool HAS_PDT, int G, int W = 1>
+__device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, const LaneTrial lt,
+                                          const int lane = static_cast<int>(threadIdx.x),
+                                          const int wave = 0) {
     extern __shared__ float fl[];
-    const int lane = threadIdx.x;
     const int nb = a.n_bins;
     float* col = fl + lane;  // slot s: W at plane 0, Y at plane 1
+    if (wave == 0) {
     for (int b = 0; b < 2 * (nb + 1); ++b) col[b * kWave] = 0.0f;
     const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
     const int64_t p0 = g.off[lt.curve];
@@ -1829,65 +1849,41 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
             col[(2 * k + 1) * kWave] = py;
         }
     }
+    }  // wave 0
+    if constexpr (W > 1) __syncthreads();
     const float wt = col[(2 * nb) * kWave];
     const float ytot = col[(2 * nb + 1) * kWave];
-    float best = 0.0f;
+    double bn = 0.0, bd = 1.0;
     if (wt > 0.0f) {
-        // The reference's window test is 0 < fl(r/wt) < 1 (bls.rs:119).  For
-        // positive finite wt, fl(r/wt) < 1 iff r < wt (pred(wt)/wt <= 1 - 2^-24
-        // rounds below 1), and fl(r/wt) > 0 iff r/wt > 2^-150, i.e.
-        // r > wt * 2^-150, exact in f64 (rlo); that last test only matters for a
-        // window that would improve the best, so it sits on that path.
-        const double rlo = static_cast<double>(wt) * 0x1p-150;
-        // Candidate windows [p, p+d) are examined in groups of four: their
-        // sums (Rust's window sums, bls.rs:104-117) and the admission test
-        // num >= fl(best*den) are branch-free; only a group in which some
-        // lane admits a candidate takes the exact path, which decides each
-        // admitted candidate, in duration order, by the IEEE quotient as
-        // Rust does.  Admission is a superset of fl(num/den) > best: that
-        // implies num/den > best (rounding is monotone and best is a float),
-        // so num > best*den exactly, and then fl(best*den) <= num because
-        // num is a float at or above best*den.  It also admits ties, den ==
-        // 0 and candidates within an ulp below best; den == inf with best ==
-        // 0 gives NaN and is not admitted, as no quotient num/inf exceeds 0.
-        // A candidate not admitted under the group's starting best cannot
-        // beat a later, larger best.
-        // r in (0, wt) is min(r, wt - r) > 0: wt - r > 0 iff r < wt in f32.
-        auto admit = [&](float r, float s, float& num, float& den) -> bool {
+        // Window test (bls.rs:118-119), 0 < fl(r/wt) < 1: for positive wt,
+        // fl(r/wt) < 1 iff r < wt, i.e. t = fl(wt - r) > 0, and fl(r/wt) > 0
+        // iff r > wt * 2^-150 (the quotient must round up to the smallest
+        // subnormal), i.e. r > rlo, the largest float <= wt * 2^-150.  For
+        // wt = inf, rlo = inf and no window passes, as in Rust.
+        const float rlo = bls_window_floor(wt);
+        // Rust keeps the largest fl(num/den) over the windows that pass.
+        // Rounding is monotone, so that is fl(num/den) of the window with the
+        // largest exact quotient, and quotients compare exactly by cross
+        // products in f64 (a product of two floats has at most 48 significant
+        // bits).  The running best is the pair (bn, bd), from 0/1; the one
+        // division is at the end.  Special values keep the quotient order:
+        // den = 0 with num > 0 (+inf) beats every finite quotient and ties
+        // another +inf; 0/0, NaN operands and x/inf (0) never compare greater,
+        // as their products are NaN or not larger.  No branch depends on the
+        // data, so all lanes stay converged.
+        auto cand = [&](float r, float s) {
             const float t = wt - r;
-            num = s * s;
-            den = r * t;
-            return (fminf(r, t) > 0.0f) & (num >= best * den);
+            const float num = s * s;
+            const float den = r * t;
+            const double dn = num, dd = den;
+            const bool take = (r > rlo) & (t > 0.0f) & (dn * bd > bn * dd);
+            bn = take ? dn : bn;
+            bd = take ? dd : bd;
         };
-        auto improve = [&](bool ok, float r, float num, float den) {
-            if (ok && static_cast<double>(r) > rlo) {
-                const float v = num / den;
-                if (v > best) best = v;
-            }
-        };
-        constexpr int kG = 4;  // candidates per group
-        auto group = [&](const float (&r)[kG], const float (&s)[kG]) {
-            float num[kG], den[kG];
-            bool ok[kG];
-            bool any = false;
-#pragma unroll
-            for (int j = 0; j < kG; ++j) {
-                ok[j] = admit(r[j], s[j], num[j], den[j]);
-                any |= ok[j];
-            }
-            if (any) {
-#pragma unroll
-                for (int j = 0; j < kG; ++j) improve(ok[j], r[j], num[j], den[j]);
-            }
-        };
-        auto one = [&](float r, float s) {
-            float num, den;
-            const bool ok = admit(r, s, num, den);
-            if (ok) improve(ok, r, num, den);
-        };
+        constexpr int kG = 8;  // windows whose prefix sums are read together
         const float* pw = col;          // W plane of slot k at pw[2k*64]
         const float* py = col + kWave;  // Y plane
-        for (int p = 0; p < nb; ++p) {
+        for (int p = wave; p < nb; p += W) {
             const float w0 = pw[(2 * p) * kWave];
             const float y0 = py[(2 * p) * kWave];
             const float wtail = wt - w0;    // wp[nb] - wp[phi]
@@ -1903,9 +1899,10 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
                     r[j] = pw[(2 * (p + d + j)) * kWave] - w0;
                     s[j] = py[(2 * (p + d + j)) * kWave] - y0;
                 }
-                group(r, s);
+#pragma unroll
+                for (int j = 0; j < kG; ++j) cand(r[j], s[j]);
             }
-            for (; d <= d_in; ++d) one(pw[(2 * (p + d)) * kWave] - w0, py[(2 * (p + d)) * kWave] - y0);
+            for (; d <= d_in; ++d) cand(pw[(2 * (p + d)) * kWave] - w0, py[(2 * (p + d)) * kWave] - y0);
             for (; d + kG - 1 <= a.d_hi; d += kG) {
                 float r[kG], s[kG];
 #pragma unroll
@@ -1913,34 +1910,78 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
                     r[j] = wtail + pw[(2 * (p + d + j - nb)) * kWave];
                     s[j] = ytail + py[(2 * (p + d + j - nb)) * kWave];
                 }
-                group(r, s);
+#pragma unroll
+                for (int j = 0; j < kG; ++j) cand(r[j], s[j]);
             }
             for (; d <= a.d_hi; ++d)
-                one(wtail + pw[(2 * (p + d - nb)) * kWave], ytail + py[(2 * (p + d - nb)) * kWave]);
+                cand(wtail + pw[(2 * (p + d - nb)) * kWave], ytail + py[(2 * (p + d - nb)) * kWave]);
         }
     }
+    if constexpr (W > 1) {
+        // Merge: waves 1.. leave their pairs in slots 1.. (the prefix sums
+        // are no longer read), wave 0 keeps the largest quotient.  The pairs
+        // are float values, so they survive the round trip exactly.
+        __syncthreads();
+        if (wave > 0) {
+            col[(2 * wave) * kWave] = static_cast<float>(bn);
+            col[(2 * wave + 1) * kWave] = static_cast<float>(bd);
+        }
+        __syncthreads();
+        if (wave != 0) return;
+#pragma unroll
+        for (int v = 1; v < W; ++v) {
+            const double vn = col[(2 * v) * kWave];
+            const double vd = col[(2 * v + 1) * kWave];
+            const bool take = vn * bd > bn * vd;
+            bn = take ? vn : bn;
+            bd = take ? vd : bd;
+        }
+    }
+    const float best = static_cast<float>(bn) / static_cast<float>(bd);
     if (lt.active) g.out[lt.curve * g.T + lt.trial] = best;
 }
 
-// One workgroup per (curve, 64-trial block), hardware-scheduled.
+// One workgroup of kBlsWaves waves per (curve, 64-trial block),
+// hardware-scheduled.
 template <bool HAS_PDT, int G>
-__global__ __launch_bounds__(kWave) void bls_kernel(GridArgs g, BlsArgs a) {
-    bls_block<HAS_PDT, G>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T));
+__global__ __launch_bounds__(kWave * kBlsWaves) void bls_kernel(GridArgs g, BlsArgs a) {
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    bls_block<HAS_PDT, G, kBlsWaves>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T, lane), lane,
+                                     wave);
 }
 
-// Persistent: each wave pulls (curve, 64-trial block) items from the queue.
+// Persistent: each workgroup of kBlsWaves waves pulls (curve, 64-trial
+// block) items from the queue; wave 0 claims and hands the claim to the
+// other waves through LDS.
 template <bool HAS_PDT>
-__global__ __launch_bounds__(kWave) void bls_persistent(GridArgs g, BlsArgs a, WorkQueue q) {
-    int64_t seen = 0;
-    for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
-        for (int64_t it = cl.begin; it < cl.end; ++it) {
-            const LaneTrial lt = queue_trial(q, g, it);
-            switch (rank_group(q, it / g.tblocks)) {
-                case 2: bls_block<HAS_PDT, 2>(g, a, lt); break;
-                case 4: bls_block<HAS_PDT, 4>(g, a, lt); break;
-                default: bls_block<HAS_PDT, 1>(g, a, lt); break;
+__global__ __launch_bounds__(kWave * kBlsWaves) void bls_persistent(GridArgs g, BlsArgs a,
+                                                                    WorkQueue q) {
+    __shared__ int64_t claim[2];
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    int64_t seen = 0;  // used by wave 0 only
+    for (;;) {
+        if (wave == 0) {
+            const Claim cl = next_claim(q, g, seen);
+            if (lane == 0) {
+                claim[0] = cl.begin;
+                claim[1] = cl.end;
             }
         }
+        __syncthreads();
+        const int64_t begin = claim[0], end = claim[1];
+        __syncthreads();  // every wave holds the claim before wave 0 takes the next
+        if (begin >= q.total) break;
+        for (int64_t it = begin; it < end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it, lane);
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: bls_block<HAS_PDT, 2, kBlsWaves>(g, a, lt, lane, wave); break;
+                case 4: bls_block<HAS_PDT, 4, kBlsWaves>(g, a, lt, lane, wave); break;
+                default: bls_block<HAS_PDT, 1, kBlsWaves>(g, a, lt, lane, wave); break;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1953,26 +1994,29 @@ static inline dim3 grid_of(const GridArgs& g) { return dim3(static_cast<uint32_t
 
 // Resident single-wave workgroups of `kernel` with `lds` bytes, device-wide.
 template <class K>
-static uint32_t persistent_grid(K kernel, size_t lds) {
+static uint32_t persistent_grid(K kernel, size_t lds, int waves) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave * waves, lds) !=
+            hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     return static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
 }
 
 template <class K, class Args>
-void launch_grid_kernel(K kernel, const GridArgs& g, const Args& a, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(kernel, grid_of(g), dim3(kWave), lds, s, g, a);
+void launch_grid_kernel(K kernel, const GridArgs& g, const Args& a, size_t lds, hipStream_t s,
+                        int waves = 1) {
+    hipLaunchKernelGGL(kernel, grid_of(g), dim3(kWave * waves), lds, s, g, a);
 }
 
 template <class K, class Args>
 void launch_persistent(K kernel, const GridArgs& g, const Args& a, size_t lds,
-                       const WorkQueue& q, hipStream_t s) {
-    hipLaunchKernelGGL(kernel, dim3(persistent_grid(kernel, lds)), dim3(kWave), lds, s, g, a, q);
+                       const WorkQueue& q, hipStream_t s, int waves = 1) {
+    hipLaunchKernelGGL(kernel, dim3(persistent_grid(kernel, lds, waves)), dim3(kWave * waves),
+                       lds, s, g, a, q);
 }
 
 // Plain launches take the binning variant `grp`; persistent launches carry
@@ -2039,10 +2083,10 @@ void mf_g(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQue
 template <bool P>
 void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
-    if (q) launch_persistent(bls_persistent<P>, g, a, lds, *q, s);
-    else if (grp >= 4) launch_grid_kernel(bls_kernel<P, 4>, g, a, lds, s);
-    else if (grp == 2) launch_grid_kernel(bls_kernel<P, 2>, g, a, lds, s);
-    else launch_grid_kernel(bls_kernel<P, 1>, g, a, lds, s);
+    if (q) launch_persistent(bls_persistent<P>, g, a, lds, *q, s, kBlsWaves);
+    else if (grp >= 4) launch_grid_kernel(bls_kernel<P, 4>, g, a, lds, s, kBlsWaves);
+    else if (grp == 2) launch_grid_kernel(bls_kernel<P, 2>, g, a, lds, s, kBlsWaves);
+    else launch_grid_kernel(bls_kernel<P, 1>, g, a, lds, s, kBlsWaves);
 }
 
 hipError_t launch_ce(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,

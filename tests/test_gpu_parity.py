@@ -219,6 +219,51 @@ def test_grid_special_values(ctx, algo):
     _check(algo, got, ref)
 
 
+def _bls_extreme_curves():
+    """BLS curves whose windows reach the edges of f32:
+    A: 360 points of weight 1/(e*e) = 1.2e-38 (magnitude 1e30), one of them
+       at magnitude 1e36, and 20 points of weight 5e6.  A window of fewer than
+       ~6 small-weight points has fl(r / W) = 0 and is skipped, although its
+       s^2 / (r (W - r)) would be the largest when it holds the 1e36 point.
+    B: only small weights: r (W - r) underflows to 0, so s^2 / 0 = +inf.
+    C: magnitudes ~1e25 with ordinary weights: s^2 overflows to +inf.
+    D: errors log-uniform over 1e-4 .. 1e18 (weights over 44 decades)."""
+    rng = np.random.default_rng(23)
+    times, mags, errs = [], [], []
+
+    def add(t, m, e):
+        o = np.argsort(t)
+        times.append(t[o].astype(np.float32))
+        mags.append(m[o].astype(np.float32))
+        errs.append(e[o].astype(np.float32))
+
+    n_small, n_big = 361, 20
+    t = rng.uniform(0.0, 1000.0, n_small + n_big)
+    m = np.concatenate([1e30 * (1.0 + 0.01 * rng.normal(size=n_small)),
+                        15.0 + 0.1 * rng.normal(size=n_big)])
+    m[0] = 1e36
+    e = np.concatenate([np.full(n_small, 9e18), np.full(n_big, 4.47e-4)])
+    add(t, m, e)
+    add(rng.uniform(0.0, 1000.0, 300), 1e20 * (1.0 + 0.1 * rng.normal(size=300)),
+        np.full(300, 9e18))
+    add(rng.uniform(0.0, 1000.0, 300), 1e25 * (1.0 + 0.1 * rng.normal(size=300)),
+        np.full(300, 0.05))
+    add(rng.uniform(0.0, 1000.0, 500), 15.0 + rng.normal(size=500),
+        10.0 ** rng.uniform(-4.0, 18.0, 500))
+    return times, mags, errs
+
+
+def test_bls_extreme_windows(ctx):
+    t, m, e = _bls_extreme_curves()
+    P = np.linspace(0.5, 10.0, 120).astype(np.float32)
+    D = np.array([0.0, 1e-5], np.float32)
+    ref = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
+    got = _gpu(ctx, "bls", t, m, e, P, D, **PARAMS["bls"])
+    assert np.isinf(ref[1]).all() and np.isinf(ref[2]).all()
+    assert np.isfinite(ref[0]).all() and np.isfinite(ref[3]).all()
+    _check("bls", got, ref)
+
+
 def test_grid_config2_sample(ctx):
     """The benchmark workload's shape (8192 points, 1000 periods) on 6 curves."""
     t, m, e, P, D = synth.config2(8192, n_curves=6)
