@@ -301,6 +301,21 @@ void pf_curves_destroy(pf_curves* cv);
 int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float* const* t,
                    const float* const* y, const float* const* err, const int64_t* lengths);
 
+/* High-cadence filter on the device, in place.  Replaces the reference's
+ * CPU-only remove_high_cadence_batched (rust/src/lib.rs:1087-1115,
+ * rust/src/highcadence.rs:7-43; RemoveHighCadence.calc,
+ * periodfind/cpu/__init__.py:1335-1365).  Per curve it keeps point 0, then
+ * each point, in index order, whose time is at least cadence_days after the
+ * last kept point: the f32 test t[i] - t_last >= cadence_days.  The reference
+ * derives cadence_days as f32 cadence_minutes / 1440.  t, y and err of the
+ * set are compacted to the kept points and cached derived arrays are
+ * dropped, so later calls evaluate the filtered curves.  Optional host
+ * outputs, each NULL to skip: kept_offsets (n_curves + 1) and t, y, err,
+ * which need room for the set's point count before filtering.  A set without
+ * errors writes ones to err.  Synchronous. */
+int pf_curves_remove_high_cadence(pf_ctx* ctx, pf_curves* cv, float cadence_days,
+                                  int64_t* kept_offsets, float* t, float* y, float* err);
+
 /* Full grid over a resident set.  The periodogram stays in device memory
  * (pf_curves_result) and is also copied to `out` when out != NULL. */
 int pf_curves_grid(pf_ctx* ctx, pf_curves* cv, const pf_params* params, const float* periods,

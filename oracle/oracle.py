@@ -350,6 +350,39 @@ def prepare_magnitudes(mags, center, normalize):
     return mags
 
 
+def cadence_days(cadence_minutes):
+    """rust/src/lib.rs:1095: cadence_minutes as f32, divided by 1440 in f32."""
+    return np.float32(np.float32(cadence_minutes) / np.float32(1440.0))
+
+
+def remove_high_cadence_indices(times, cadence_days):
+    """rust/src/highcadence.rs:7-21: index 0, then each index whose time is at
+    least cadence_days after the last kept one, tested as the f32 difference
+    times[i] - last >= cadence_days (numpy float32 scalars), in index order."""
+    t = np.asarray(times, dtype=np.float32)
+    if t.size == 0:
+        return []
+    cad = np.float32(cadence_days)
+    kept = [0]
+    last = t[0]
+    with np.errstate(invalid="ignore", over="ignore"):
+        for i in range(1, t.size):
+            if t[i] - last >= cad:
+                kept.append(i)
+                last = t[i]
+    return kept
+
+
+def remove_high_cadence_batch(times, mags, errs, cadence_days):
+    """rust/src/highcadence.rs:27-43: one filtered (t, m, e) triple per curve."""
+    out = []
+    for t, m, e in zip(times, mags, errs):
+        idx = np.array(remove_high_cadence_indices(t, cadence_days), dtype=np.int64)
+        out.append((np.asarray(t, np.float32)[idx], np.asarray(m, np.float32)[idx],
+                    np.asarray(e, np.float32)[idx]))
+    return out
+
+
 def calc_grid(algo, times, mags, periods, period_dts, errs=None, threads=0, **params):
     """Full periodogram with the reference CPU classes' preprocessing defaults.
 

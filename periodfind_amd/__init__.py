@@ -20,7 +20,8 @@ from periodfind_amd.results import Periodogram, Statistics
 __all__ = [
     "set_device", "get_device", "ConditionalEntropy", "AOV", "LombScargle", "FPW",
     "BoxLeastSquares", "MatchedFilter", "ViterbiNarrowband", "MultiHarmonicFourier",
-    "FourierDecomposition", "Statistics", "Periodogram",
+    "FourierDecomposition", "RemoveHighCadence", "remove_high_cadence", "Statistics",
+    "Periodogram",
 ]
 
 _default_device = None  # None: auto-detect
@@ -118,3 +119,14 @@ def MultiHarmonicFourier(**kwargs):
 def FourierDecomposition(**kwargs):
     """Fourier decomposition features at given periods on the resolved device."""
     return _make("FourierDecomposition", kwargs)
+
+
+def RemoveHighCadence(**kwargs):
+    """High-cadence point removal on the resolved device."""
+    return _make("RemoveHighCadence", kwargs)
+
+
+def remove_high_cadence(times, mags, errs, cadence_minutes=30.0):
+    """Batch high-cadence removal (periodfind/__init__.py:232-250): list of
+    filtered (times, mags, errs) tuples, one per curve."""
+    return RemoveHighCadence(cadence_minutes=cadence_minutes).calc(times, mags, errs)

@@ -104,6 +104,9 @@ cdef extern from "periodfind_amd.h":
                          const int64_t* offsets, const float* t, const float* y,
                          const float* err) nogil
     void pf_curves_destroy(pf_curves* cv) nogil
+    int pf_curves_remove_high_cadence(pf_ctx* ctx, pf_curves* cv, float cadence_days,
+                                      int64_t* kept_offsets, float* t, float* y,
+                                      float* err) nogil
     int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float** t,
                        const float** y, const float** err, const int64_t* lengths) nogil
     int pf_curves_grid(pf_ctx* ctx, pf_curves* cv, const pf_params* params,
@@ -503,6 +506,36 @@ cdef class Context:
         if rc != 0:
             _raise(rc)
         return out
+
+    def remove_high_cadence(self, times, mags, errs, float cadence_days):
+        """Load the curves into this context's set, drop high-cadence points on
+        the device (pf_curves_remove_high_cadence) and return one
+        (times, mags, errs) tuple of float32 arrays per curve."""
+        cdef int64_t C, total
+        cdef int rc
+        cdef int64_t* op
+        cdef float* tp
+        cdef float* yp
+        cdef float* ep
+        with self.lock:
+            self.load(times, mags, errs)
+            C = self.n_loaded
+            total = sum(len(a) for a in times)
+            off = np.zeros(C + 1, dtype=np.int64)
+            t = np.empty(max(total, 1), dtype=np.float32)
+            y = np.empty(max(total, 1), dtype=np.float32)
+            e = np.empty(max(total, 1), dtype=np.float32)
+            op = <int64_t*> cnp.PyArray_DATA(off)
+            tp = <float*> cnp.PyArray_DATA(t)
+            yp = <float*> cnp.PyArray_DATA(y)
+            ep = <float*> cnp.PyArray_DATA(e)
+            with nogil:
+                rc = pf_curves_remove_high_cadence(self.ctx, self.cv, cadence_days, op, tp, yp,
+                                                   ep)
+        if rc != 0:
+            _raise(rc)
+        return [(t[off[i]:off[i + 1]].copy(), y[off[i]:off[i + 1]].copy(),
+                 e[off[i]:off[i + 1]].copy()) for i in range(C)]
 
     def fourier_features(self, cnp.ndarray off, cnp.ndarray t, cnp.ndarray y, err,
                          cnp.ndarray periods):

@@ -77,6 +77,16 @@ def _opt_f32p():
 
 _f32p_or_null = _opt_f32p()
 
+
+class _i64p_or_null:
+    """int64_t* that also accepts None (NULL)."""
+
+    @classmethod
+    def from_param(cls, obj):
+        if obj is None:
+            return None
+        return _i64p.from_param(obj)
+
 SIGNATURES = {
     "pf_device_count": (_int, [ctypes.POINTER(ctypes.c_int)]),
     "pf_last_error": (ctypes.c_char_p, []),
@@ -112,6 +122,8 @@ SIGNATURES = {
     "pf_curves_create": (_int, [_vp, ctypes.POINTER(_vp), _i64, _i64p, _f32p, _f32p,
                                 _f32p_or_null]),
     "pf_curves_destroy": (None, [_vp]),
+    "pf_curves_remove_high_cadence": (_int, [_vp, _vp, _flt, _i64p_or_null, _f32p_or_null,
+                                             _f32p_or_null, _f32p_or_null]),
     "pf_curves_load": (_int, [_vp, _vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                               ctypes.POINTER(_vp), _i64p]),
     "pf_curves_grid": (_int, [_vp, _vp, ctypes.POINTER(PfParams), _f32p, _i64, _f32p, _i64,
@@ -416,6 +428,7 @@ class CurveSet:
         _, y = pack(mags)
         e = pack(errs)[1] if errs is not None else None
         self.n_curves = len(times)
+        self.npts = int(off[-1])
         check(lib().pf_curves_create(ctx.h, ctypes.byref(self.h), len(times), off, t, y, e))
 
     def load(self, times, mags, errs=None):
@@ -433,7 +446,21 @@ class CurveSet:
         ptrs = [(_vp * max(n, 1))(*[a.ctypes.data for a in lst]) if lst else None for lst in arrs]
         check(lib().pf_curves_load(self.ctx.h, self.h, n, ptrs[0], ptrs[1], ptrs[2], lens))
         self.n_curves = n
+        self.npts = int(lens.sum())
         return self
+
+    def remove_high_cadence(self, cadence_days):
+        """Filter the set in place on the device (pf_curves_remove_high_cadence)
+        and return the kept curves: (offsets int64[n+1], t, y, err), float32
+        CSR arrays (err is ones for a set created without errors)."""
+        off = np.zeros(self.n_curves + 1, np.int64)
+        cap = max(self.npts, 1)
+        t, y, e = (np.empty(cap, np.float32) for _ in range(3))
+        check(lib().pf_curves_remove_high_cadence(self.ctx.h, self.h, np.float32(cadence_days),
+                                                  off, t, y, e))
+        self.npts = int(off[-1])
+        k = self.npts
+        return off, t[:k].copy(), y[:k].copy(), e[:k].copy()
 
     def close(self):
         if self.h:

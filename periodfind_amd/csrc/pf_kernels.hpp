@@ -187,6 +187,15 @@ hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, 
 // Serial per-curve reductions in the reference's order (one lane per curve).
 // prepare_magnitudes on the device (mode PF_MAG_CENTER or PF_MAG_NORMALIZE),
 // y -> yp per curve, numpy float32 semantics.
+// High-cadence filter (rust/src/highcadence.rs:7-21): per curve the local
+// indices of the kept points (into keep, CSR like the input) and their count,
+// then the compaction of t, y and err (src: the three planes back to back,
+// npts each; e may be nullptr) to the offsets noff.
+hipError_t launch_cadence_keep(const int64_t* off, const float* t, float cad, uint32_t* keep,
+                               int64_t* cnt, int64_t n_curves, hipStream_t s);
+hipError_t launch_cadence_gather(const int64_t* off, const int64_t* noff, const uint32_t* keep,
+                                 const float* src, int64_t npts, float* t, float* y, float* e,
+                                 int64_t n_curves, hipStream_t s);
 hipError_t launch_prep_mags(const int64_t* off, const float* y, float* yp, int64_t n_curves,
                             int mode, hipStream_t s);
 hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int64_t n_curves,

@@ -336,6 +336,62 @@ __global__ __launch_bounds__(kBlock) void mhf_prep_kernel(const int64_t* __restr
     }
 }
 
+// rust/src/highcadence.rs:7-21, one wave per curve: keep point 0, then each
+// point i, in index order, whose time is at least `cad` after the last kept
+// one, fl(t[i] - t_last) >= cad (a NaN on either side keeps nothing).  The
+// times are read 64 at a time; the kept points of a window are found in
+// order, each by a ballot over the lanes after the previous one.
+// keep[off[c] + j] is the local index of curve c's j-th kept point and
+// cnt[c] their number.
+__global__ __launch_bounds__(64) void cadence_keep_kernel(const int64_t* __restrict__ off,
+                                                          const float* __restrict__ t, float cad,
+                                                          uint32_t* __restrict__ keep,
+                                                          int64_t* __restrict__ cnt) {
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t a = off[c], n = off[c + 1] - a;
+    if (n <= 0) {
+        if (lane == 0) cnt[c] = 0;
+        return;
+    }
+    float last = t[a];
+    int64_t kept = 1;
+    if (lane == 0) keep[a] = 0u;
+    for (int64_t base = 1; base < n; base += 64) {
+        const bool in = base + lane < n;
+        const float v = in ? t[a + base + lane] : 0.0f;
+        uint64_t open = __ballot(in);  // lanes after the last kept point
+        for (;;) {
+            const bool ok = in && (v - last >= cad);
+            const uint64_t m = __ballot(ok) & open;
+            if (m == 0) break;
+            const int f = __builtin_ctzll(m);
+            last = __shfl(v, f);
+            if (lane == 0) keep[a + kept] = static_cast<uint32_t>(base + f);
+            ++kept;
+            open = f == 63 ? 0ull : open & (~0ull << (f + 1));
+        }
+    }
+    if (lane == 0) cnt[c] = kept;
+}
+
+// Compaction of the kept points, one workgroup per curve: point j of curve c
+// moves from src[plane][off[c] + keep[off[c] + j]] to dst[plane][noff[c] + j]
+// (planes t, y and, when e != nullptr, err; src holds them back to back).
+__global__ __launch_bounds__(kBlock) void cadence_gather_kernel(
+    const int64_t* __restrict__ off, const int64_t* __restrict__ noff,
+    const uint32_t* __restrict__ keep, const float* __restrict__ src, int64_t npts,
+    float* __restrict__ t, float* __restrict__ y, float* __restrict__ e) {
+    const int64_t c = blockIdx.x;
+    const int64_t a = off[c], b = noff[c], k = noff[c + 1] - b;
+    for (int64_t j = threadIdx.x; j < k; j += kBlock) {
+        const int64_t i = a + keep[a + j];
+        t[b + j] = src[i];
+        y[b + j] = src[npts + i];
+        if (e) e[b + j] = src[2 * npts + i];
+    }
+}
+
 }  // namespace
 
 hipError_t launch_mhf_prep(const int64_t* off, const float* t, const float* y, const float* e,
@@ -423,6 +479,23 @@ hipError_t launch_vn_lut(const int64_t* off, const int64_t* lut_off, float* lut,
     if (n_curves <= 0) return hipSuccess;
     hipLaunchKernelGGL(vn_lut_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock), 0, s,
                        off, lut_off, lut, ext);
+    return hipGetLastError();
+}
+
+hipError_t launch_cadence_keep(const int64_t* off, const float* t, float cad, uint32_t* keep,
+                                int64_t* cnt, int64_t n_curves, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cadence_keep_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, t, cad, keep, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cadence_gather(const int64_t* off, const int64_t* noff, const uint32_t* keep,
+                                 const float* src, int64_t npts, float* t, float* y, float* e,
+                                 int64_t n_curves, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cadence_gather_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(kBlock),
+                       0, s, off, noff, keep, src, npts, t, y, e);
     return hipGetLastError();
 }
 

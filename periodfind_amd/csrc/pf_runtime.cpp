@@ -215,6 +215,9 @@ struct pf_ctx {
     DevBuf ls_freq, ls_lo, ls_hi;
     // Segment selections of a split peak scan (see pf::launch_peaks).
     DevBuf peaks_ws;
+    // High-cadence filter: kept indices, counts, a copy of the set's planes
+    // and the new offsets.
+    DevBuf hc_keep, hc_cnt, hc_src, hc_off;
     // Matched Filter templates, one table per bin count (rows of
     // mf_template_stride(nb)); a call with several MF jobs of different bin
     // counts reads each job's own table.
@@ -1581,8 +1584,9 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
-    DevBuf* bufs[] = {&ctx->periods, &ctx->pdts, &ctx->outbuf, &ctx->ls_freq, &ctx->ls_lo,
-                      &ctx->ls_hi, &ctx->peaks_ws};
+    DevBuf* bufs[] = {&ctx->periods, &ctx->pdts,   &ctx->outbuf, &ctx->ls_freq,
+                      &ctx->ls_lo,   &ctx->ls_hi,  &ctx->peaks_ws, &ctx->hc_keep,
+                      &ctx->hc_cnt,  &ctx->hc_src, &ctx->hc_off};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (pf_ctx::MfTable& m : ctx->mf_tables)
@@ -2013,6 +2017,66 @@ int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float* co
     PF_HIP(hipSetDevice(ctx->device));
     PF_TRY(upload_lists(ctx, cv->cs, n_curves, t, y, err, lengths));
     return finish(ctx);
+}
+
+int pf_curves_remove_high_cadence(pf_ctx* ctx, pf_curves* cv, float cadence_days,
+                                  int64_t* kept_offsets, float* t, float* y, float* err) {
+    if (!ctx || !cv) return fail(PF_E_ARG, "ctx/cv is NULL");
+    if (cv->ctx != ctx) return fail(PF_E_ARG, "the curve set belongs to another context");
+    CurveSet& cs = cv->cs;
+    if (cs.max_len > static_cast<int64_t>(UINT32_MAX))
+        return fail(PF_E_LIMIT, "a curve of %lld points exceeds the filter's 2^32 - 1",
+                    static_cast<long long>(cs.max_len));
+    PF_HIP(hipSetDevice(ctx->device));
+    ctx->timed = false;
+    const int64_t nc = cs.n_curves, np = cs.npts;
+    const int planes = cs.has_err ? 3 : 2;
+    PF_TRY(ensure(ctx->hc_keep, static_cast<size_t>(np) * sizeof(uint32_t)));
+    PF_TRY(ensure(ctx->hc_cnt, static_cast<size_t>(nc) * sizeof(int64_t)));
+    PF_TRY(ensure(ctx->hc_src, static_cast<size_t>(planes) * np * sizeof(float)));
+    PF_TRY(ensure(ctx->hc_off, static_cast<size_t>(nc + 1) * sizeof(int64_t)));
+    hipStream_t s = ctx->stream;
+    PF_HIP(pf::launch_cadence_keep(as<int64_t>(cs.off), as<float>(cs.t), cadence_days,
+                                   as<uint32_t>(ctx->hc_keep), as<int64_t>(ctx->hc_cnt), nc, s));
+    std::vector<int64_t> cnt(static_cast<size_t>(nc));
+    if (nc) PF_HIP(hipMemcpyAsync(cnt.data(), ctx->hc_cnt.p, nc * sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> noff(static_cast<size_t>(nc) + 1, 0);
+    int64_t mx = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        noff[c + 1] = noff[c] + cnt[c];
+        if (cnt[c] > mx) mx = cnt[c];
+    }
+    const int64_t kept = noff[nc];
+    // The set's planes are copied aside, then gathered back compacted.
+    float* src = as<float>(ctx->hc_src);
+    const size_t nb = static_cast<size_t>(np) * sizeof(float);
+    if (np) {
+        PF_HIP(hipMemcpyAsync(src, cs.t.p, nb, hipMemcpyDeviceToDevice, s));
+        PF_HIP(hipMemcpyAsync(src + np, cs.y.p, nb, hipMemcpyDeviceToDevice, s));
+        if (cs.has_err)
+            PF_HIP(hipMemcpyAsync(src + 2 * np, cs.e.p, nb, hipMemcpyDeviceToDevice, s));
+    }
+    PF_HIP(hipMemcpyAsync(ctx->hc_off.p, noff.data(), (nc + 1) * sizeof(int64_t),
+                          hipMemcpyHostToDevice, s));
+    PF_HIP(pf::launch_cadence_gather(as<int64_t>(cs.off), as<int64_t>(ctx->hc_off),
+                                     as<uint32_t>(ctx->hc_keep), src, np, as<float>(cs.t),
+                                     as<float>(cs.y), cs.has_err ? as<float>(cs.e) : nullptr, nc,
+                                     s));
+    PF_HIP(hipMemcpyAsync(cs.off.p, ctx->hc_off.p, (nc + 1) * sizeof(int64_t),
+                          hipMemcpyDeviceToDevice, s));
+    set_layout(cs, nc, noff.data(), kept, mx, cs.has_err);  // drops the derived arrays
+    if (kept_offsets) std::copy(noff.begin(), noff.end(), kept_offsets);
+    const size_t kb = static_cast<size_t>(kept) * sizeof(float);
+    if (kb) {
+        if (t) PF_HIP(hipMemcpyAsync(t, cs.t.p, kb, hipMemcpyDeviceToHost, s));
+        if (y) PF_HIP(hipMemcpyAsync(y, cs.y.p, kb, hipMemcpyDeviceToHost, s));
+        if (err && cs.has_err) PF_HIP(hipMemcpyAsync(err, cs.e.p, kb, hipMemcpyDeviceToHost, s));
+    }
+    PF_TRY(finish(ctx));
+    if (err && !cs.has_err) std::fill(err, err + kept, 1.0f);  // a set without errors: ones
+    return PF_OK;
 }
 
 void pf_curves_destroy(pf_curves* cv) {

@@ -501,6 +501,37 @@ class FourierDecomposition:
         return context(self.device).fourier_features(off, t, y, e, np.ascontiguousarray(periods))
 
 
+class RemoveHighCadence:
+    """High-cadence point removal on the device (the CPU class of
+    periodfind/cpu/__init__.py:1335-1365; rust/src/highcadence.rs:7-43).
+    Per curve, keeps the first point, then each point whose time is at least
+    `cadence_minutes` after the last kept one, in index order."""
+
+    def __init__(self, cadence_minutes=30.0, device=0):
+        self.cadence_minutes = cadence_minutes
+        self.device = device
+
+    def calc(self, times, mags, errs):
+        """Filter a batch of light curves -> list of (times, mags, errs)
+        float32 tuples, one per curve."""
+        ensure_float32(times, "times")
+        ensure_float32(mags, "mags")
+        ensure_float32(errs, "errs")
+        if len(mags) != len(times) or len(errs) != len(times):
+            raise ValueError(
+                f"times, mags and errs must hold the same number of arrays, "
+                f"got {len(times)}, {len(mags)} and {len(errs)}")
+        # rust/src/lib.rs:1095: cadence_days = (cadence_minutes as f32) / 1440 in f32.
+        cadence_days = np.float32(np.float32(self.cadence_minutes) / np.float32(1440.0))
+        return context(self.device).remove_high_cadence(times, mags, errs, cadence_days)
+
+
+def remove_high_cadence(times, mags, errs, cadence_minutes=30.0, device=0):
+    """Batch high-cadence removal on the device (periodfind/__init__.py:232-250)."""
+    return RemoveHighCadence(cadence_minutes=cadence_minutes, device=device).calc(times, mags,
+                                                                                  errs)
+
+
 def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTANCE_DEFAULT,
                            use_max=True, device=0):
     """Peaks of host periodograms data[n_curves, n_periods, n_pdts] on the device
@@ -511,4 +542,4 @@ def find_top_peaks_batched(data, n_peaks=N_PEAKS_DEFAULT, min_distance=MIN_DISTA
 
 __all__ = ["ConditionalEntropy", "AOV", "LombScargle", "FPW", "BoxLeastSquares", "MatchedFilter",
            "ViterbiNarrowband", "MultiHarmonicFourier", "FourierDecomposition",
-           "find_top_peaks_batched", "context"]
+           "RemoveHighCadence", "remove_high_cadence", "find_top_peaks_batched", "context"]

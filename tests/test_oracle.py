@@ -316,3 +316,56 @@ def test_ce_probability_division_identity():
         r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("bad=0")
+
+
+# --- high-cadence filter (rust/src/highcadence.rs) ----------------------------
+
+
+def test_high_cadence_rust_kats():
+    """The unit tests of rust/src/highcadence.rs:50-117 as known answers."""
+    f32 = np.float32
+    assert O.remove_high_cadence_indices(np.zeros(0, f32), 0.5) == []
+    assert O.remove_high_cadence_indices(np.array([1.0], f32), 0.5) == [0]
+    within = np.arange(10, dtype=f32) * f32(0.1)
+    assert O.remove_high_cadence_indices(within, 1.0) == [0]
+    beyond = np.arange(5, dtype=f32) * f32(2.0)
+    assert O.remove_high_cadence_indices(beyond, 1.0) == [0, 1, 2, 3, 4]
+    hand = np.array([0.0, 0.01, 0.02, 0.025, 0.05, 0.06, 0.10], f32)
+    assert O.remove_high_cadence_indices(hand, f32(30.0) / f32(1440.0)) == [0, 3, 4, 6]
+    t1, m1, e1 = (np.array(v, f32) for v in ([0, 1, 2, 3], [10, 11, 12, 13], [0.1] * 4))
+    t2, m2, e2 = (np.array(v, f32) for v in ([0, 0.001, 0.002, 5], [20, 21, 22, 23], [0.2] * 4))
+    r = O.remove_high_cadence_batch([t1, t2], [m1, m2], [e1, e2], 0.5)
+    assert len(r) == 2
+    np.testing.assert_array_equal(r[0][0], t1)
+    np.testing.assert_array_equal(r[1][0], np.array([0, 5], f32))
+    np.testing.assert_array_equal(r[1][1], np.array([20, 23], f32))
+    np.testing.assert_array_equal(r[1][2], np.array([0.2, 0.2], f32))
+
+
+def test_high_cadence_python_wrapper_kats():
+    """The reference's RemoveHighCadence tests (tests/test_batch_features.py:
+    21-93) at cadence_minutes = 30, through cadence_days as lib.rs derives it."""
+    f32 = np.float32
+    cd = O.cadence_days(30.0)
+    assert cd == f32(30.0) / f32(1440.0)
+    basic = np.array([0.0, 0.001, 0.002, 0.025, 0.05], f32)
+    assert O.remove_high_cadence_indices(basic, cd) == [0, 3, 4]
+    assert O.remove_high_cadence_indices(np.array([0.0, 1.0, 2.0, 3.0], f32), cd) == [0, 1, 2, 3]
+    assert O.remove_high_cadence_indices(np.array([0.0, 0.001, 5.0], f32), cd) == [0, 2]
+
+
+def test_high_cadence_f32_difference_form():
+    """The test is fl(t - last) >= c, not t >= fl(last + c): with last =
+    1 + 2^-23, t = 2 and c = 1 - 2^-24, t - last = 1 - 2^-23 exactly (Sterbenz)
+    is below c, so the point is dropped, while last + c rounds to 2 and the
+    other form would keep it."""
+    f32 = np.float32
+    last = f32(1.0) + f32(2.0**-23)
+    c = f32(1.0) - f32(2.0**-24)
+    assert f32(2.0) - last == f32(1.0) - f32(2.0**-23) < c
+    assert last + c == f32(2.0)
+    assert O.remove_high_cadence_indices(np.array([last, 2.0], f32), c) == [0]
+    # NaN: a NaN first time keeps only point 0; a NaN later time is skipped.
+    nan = f32(np.nan)
+    assert O.remove_high_cadence_indices(np.array([nan, 1.0, 2.0], f32), 0.5) == [0]
+    assert O.remove_high_cadence_indices(np.array([0.0, nan, 2.0], f32), 0.5) == [0, 2]
