@@ -767,7 +767,7 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const uint32_t k = div_bin(ph[j], db);
-                    off[j] = ((k == nbu) ? 0u : k) * kBinBytes;
+                    off[j] = __umul24((k == nbu) ? 0u : k, kBinBytes);
                     val[0][j] = 1.0f;
                     val[1][j] = v[j].a;
                     val[2][j] = v[j].b;
@@ -780,7 +780,8 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
             [&](const auto& ph, const auto& v) {
                 const uint32_t k = div_bin(ph[0], db);
                 for (int o = 0; o < a.ext; ++o) {
-                    const uint32_t off1[1] = {wrap_add(k, o, nbu) * kBinBytes};
+                    const uint32_t off1[1] = {
+                        static_cast<uint32_t>(__umul24(wrap_add(k, o, nbu), kBinBytes))};
                     const float val1[3][1] = {{1.0f}, {v[0].a}, {v[0].b}};
                     lds_rmw(col, off1, val1);
                 }
@@ -1033,7 +1034,7 @@ __device__ __forceinline__ void mf_block(const GridArgs& g, const MfArgs& a, con
             constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                off[j] = mul_bin(ph[j], nf, nm1) * kBinBytes;
+                off[j] = __umul24(mul_bin(ph[j], nf, nm1), kBinBytes);
                 val[0][j] = v[j].a;
                 val[1][j] = v[j].b;
                 val[2][j] = v[j].c;
