@@ -20,7 +20,8 @@
  *
  * Conventions
  *   - Host pointers are float32 / int64 C-contiguous arrays owned by the caller.
- *   - Outputs are written in place; every call is synchronous on return.
+ *   - Outputs are written in place; every call that returns outputs is
+ *     synchronous on return (pf_curves_load returns once its inputs are staged).
  *   - Output layout is out[curve][period][pdt] (row-major), as in the reference.
  *   - Return value: PF_OK (0) or a negative PF_E_* code; pf_last_error() gives
  *     the message of the last failing call on the calling thread.
@@ -296,8 +297,9 @@ void pf_curves_destroy(pf_curves* cv);
  * reusable pinned staging buffers chunk by chunk, and each chunk's
  * host-to-device copy is issued before the next chunk is packed, so packing
  * overlaps the transfer.  The set's device buffers grow as needed and its
- * cached derived arrays are dropped.  Returns when the data are on the
- * device (the caller may then reuse its arrays). */
+ * cached derived arrays are dropped.  Returns once every array is packed
+ * into staging, so the caller may reuse its arrays; the copies complete in
+ * stream order before any later work on the set. */
 int pf_curves_load(pf_ctx* ctx, pf_curves* cv, int64_t n_curves, const float* const* t,
                    const float* const* y, const float* const* err, const int64_t* lengths);
 

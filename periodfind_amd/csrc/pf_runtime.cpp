@@ -602,9 +602,14 @@ int upload_lists(pf_ctx* ctx, CurveSet& cs, int64_t n_curves, const float* const
     if (rc == PF_OK) copy(cs.t.p, t);
     if (rc == PF_OK) copy(cs.y.p, y);
     if (rc == PF_OK && e) copy(cs.e.p, e);
-    const int rd = st.drain();
-    if (rc == PF_OK) rc = rd;
-    if (rc != PF_OK) set_layout(cs, 0, off.data(), 0, 0, false);  // an empty, consistent set
+    // The copies complete in stream order, before any later work on the set
+    // (every use is on ctx->stream, or on side streams that wait for it);
+    // a staging chunk is reused only after its copy's event.  So a load does
+    // not wait for its copies, except to leave a failed one quiet.
+    if (rc != PF_OK) {
+        (void)st.drain();
+        set_layout(cs, 0, off.data(), 0, 0, false);  // an empty, consistent set
+    }
     return rc;
 }
 

@@ -57,6 +57,14 @@ def pack(arrays):
     return off, np.ascontiguousarray(flat)
 
 
+def _pick(arr, ix):
+    """[arr[i] for i in ix] as the caller's sequence indexes: one gather for a
+    1-D ndarray (elements of its dtype), element by element otherwise."""
+    if isinstance(arr, np.ndarray) and arr.ndim == 1:
+        return list(arr[ix])
+    return [arr[int(i)] for i in ix]
+
+
 def _grid_arrays(periods, period_dts):
     P = np.ascontiguousarray(periods, dtype=np.float32).reshape(-1)
     D = np.ascontiguousarray(period_dts, dtype=np.float32).reshape(-1)
@@ -132,6 +140,15 @@ class _Algorithm:
                                                     significance_type=significance_type)
                     for i in range(n_curves)]
         nd = D.size
+        n = int(n_stats)
+        # Field values are gathered column-wise (one numpy gather per field)
+        # rather than per object; the scalars keep their types: the caller's
+        # period / period_dt elements, float32 values and summaries.
+        flat = np.clip(idx.reshape(-1), 0, P.size * nd - 1)  # fallback rows hold no index
+        per = _pick(params[0], flat // nd)
+        pdt = _pick(params[1], flat % nd)
+        vals = list(val.reshape(-1))
+        mean, std, med, mad = (list(col) for col in summ.T)
         grid = None
         res = []
         for c in range(n_curves):
@@ -141,10 +158,9 @@ class _Algorithm:
                 res.append(Statistics.statistics_from_data(
                     grid[c], params, self._use_max, n=n_stats, significance_type=significance_type))
                 continue
-            mean, std, med, mad = summ[c]
-            best = [Statistics([params[0][int(i) // nd], params[1][int(i) % nd]], v, mean, std, med,
-                               mad, significance_type) for i, v in zip(idx[c], val[c])]
-            res.append(best[0] if n_stats == 1 else best)
+            best = [Statistics([per[k], pdt[k]], vals[k], mean[c], std[c], med[c], mad[c],
+                               significance_type) for k in range(c * n, c * n + n)]
+            res.append(best[0] if n == 1 else best)
         return res
 
 
