@@ -8,7 +8,7 @@ Bars (per statistic):
     (periodfind_amd/csrc/pf_logf.hpp, equal to the host's logf on every
     float).
   LS: sin/cos come from the hardware (revolution-input) unit, so values agree
-    to rtol 1e-4 / atol 1e-4 * max and the best
+    to rtol 1e-4 / atol 1e-4 * max (the path's stated LS tolerance) and the best
     index is identical.  Curves of fewer than 16 points are held to the
     reference's own cross-backend LS bar, rtol 5e-4 / atol 1e-3
     (tests/test_cpu_vs_cuda.py:311): with 2-3 points the 2x2 normal matrix of
