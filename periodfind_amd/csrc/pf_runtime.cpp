@@ -2065,12 +2065,22 @@ int pf_curves_remove_high_cadence(pf_ctx* ctx, pf_curves* cv, float cadence_days
     }
     PF_HIP(hipMemcpyAsync(ctx->hc_off.p, noff.data(), (nc + 1) * sizeof(int64_t),
                           hipMemcpyHostToDevice, s));
-    PF_HIP(pf::launch_cadence_gather(as<int64_t>(cs.off), as<int64_t>(ctx->hc_off),
-                                     as<uint32_t>(ctx->hc_keep), src, np, as<float>(cs.t),
-                                     as<float>(cs.y), cs.has_err ? as<float>(cs.e) : nullptr, nc,
-                                     s));
-    PF_HIP(hipMemcpyAsync(cs.off.p, ctx->hc_off.p, (nc + 1) * sizeof(int64_t),
-                          hipMemcpyDeviceToDevice, s));
+    // From the compaction on, the planes no longer match the old offsets: a
+    // failure there leaves an empty, consistent set.
+    auto compact = [&]() -> int {
+        PF_HIP(pf::launch_cadence_gather(as<int64_t>(cs.off), as<int64_t>(ctx->hc_off),
+                                         as<uint32_t>(ctx->hc_keep), src, np, as<float>(cs.t),
+                                         as<float>(cs.y), cs.has_err ? as<float>(cs.e) : nullptr,
+                                         nc, s));
+        PF_HIP(hipMemcpyAsync(cs.off.p, ctx->hc_off.p, (nc + 1) * sizeof(int64_t),
+                              hipMemcpyDeviceToDevice, s));
+        return PF_OK;
+    };
+    if (const int rc = compact(); rc != PF_OK) {
+        const int64_t zero = 0;
+        set_layout(cs, 0, &zero, 0, 0, false);
+        return rc;
+    }
     set_layout(cs, nc, noff.data(), kept, mx, cs.has_err);  // drops the derived arrays
     if (kept_offsets) std::copy(noff.begin(), noff.end(), kept_offsets);
     const size_t kb = static_cast<size_t>(kept) * sizeof(float);

@@ -121,3 +121,29 @@ def test_failed_resident_set_stays_usable(fresh_ctx, monkeypatch):
         np.testing.assert_array_equal(got, ref)
     finally:
         cs.close()
+
+
+def test_failed_high_cadence_filter_leaves_the_set(fresh_ctx, monkeypatch):
+    """The filter's work buffers are grown before the set is touched: a failed
+    grow leaves the set as it was, and the filter works once the fault is gone."""
+    t, m, e = _curves(5, 700)
+    P = np.linspace(0.5, 10.0, 120).astype(np.float32)
+    D = np.zeros(1, np.float32)
+    cs = capi.CurveSet(fresh_ctx, t, m, e)
+    try:
+        want = cs.grid("fpw", P, D, n_bins=10)
+        cad = O.cadence_days(240.0)
+        monkeypatch.setenv("PF_FAULT_ALLOC", "1")
+        _nomem(lambda: cs.remove_high_cadence(cad))
+        monkeypatch.delenv("PF_FAULT_ALLOC")
+        np.testing.assert_array_equal(cs.grid("fpw", P, D, n_bins=10), want)
+        kept = O.remove_high_cadence_batch(t, m, e, cad)
+        off, kt, ky, ke = cs.remove_high_cadence(cad)
+        assert [off[c + 1] - off[c] for c in range(len(t))] == [k[0].size for k in kept]
+        np.testing.assert_array_equal(kt, np.concatenate([k[0] for k in kept]))
+        got = cs.grid("fpw", P, D, n_bins=10)
+        ref = O.calc_grid("fpw", [k[0] for k in kept], [k[1] for k in kept], P, D,
+                          errs=[k[2] for k in kept], n_bins=10)
+        np.testing.assert_array_equal(got, ref)
+    finally:
+        cs.close()
