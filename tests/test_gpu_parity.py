@@ -171,6 +171,33 @@ def test_grid_params(ctx, algo, params):
     _check(algo, got, ref)
 
 
+CHUNK_LENGTHS = [1, 7, 8, 9, 15, 16, 17, 23, 24, 25, 31, 32, 33, 40, 47, 48, 49, 56, 100,
+                 257, 1031, 2049]
+
+
+@pytest.mark.parametrize("algo", ["aov", "fpw", "bls"])
+@pytest.mark.parametrize("group", [1, 2, 4])
+@pytest.mark.parametrize("pdt", [False, True])
+def test_rmw_group_variants(ctx, algo, group, pdt):
+    """Every read-modify-write group size of the binned statistics, forced
+    for the whole launch through the tuning table, is bit-identical to the
+    oracle, with and without period derivatives.  The lengths cover 0 to 7
+    whole 8-point chunks with every remainder, 16-point chunks, and longer
+    curves."""
+    t, m, e = _curves(CHUNK_LENGTHS, seed=17)
+    m = _prep(algo, m)
+    P = np.linspace(0.4, 11.0, 70).astype(np.float32)
+    D = np.array([0.0, 1e-5, -1e-5] if pdt else [0.0], np.float32)
+    saved = ctx.get_tuning()
+    try:
+        ctx.set_tuning([(algo, 0, group, 4)])
+        got = _gpu(ctx, algo, t, m, e, P, D, **PARAMS[algo])
+    finally:
+        ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
+    ref = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+    _check(algo, got, ref)
+
+
 def test_ls_precise_path(ctx, monkeypatch):
     """PF_LS_PRECISE=1 takes sincosf(fl(2 pi phase)), the exact fold and the
     reference's separate multiply and add; the default takes the hardware
