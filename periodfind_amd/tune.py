@@ -11,6 +11,8 @@ pf_ctx_create reads: tuning_<arch>.txt next to libpfhip.so.
 
     python -m periodfind_amd.tune [--out PATH] [--algos ce,aov,ls,fpw,bls]
                                   [--trials 6000] [--reps 3] [--report PATH]
+                                  [--regime uniform|long|short|gappy]
+    python -m periodfind_amd.tune --compare REPORT [REPORT ...]
 """
 
 import argparse
@@ -31,7 +33,21 @@ PARAMS = {"ce": dict(n_phase=10, n_mag=10), "aov": dict(n_phase=15), "ls": {},
           "fpw": dict(n_bins=10), "bls": dict(n_bins=50, qmin=0.01, qmax=0.5)}
 
 
-def bucket_curves(lo, hi, n_trials, target_evals, seed):
+# Regimes of trial period against point spacing (P/dt).  With a long period
+# relative to the spacing, consecutive points of a trial fall in the same or
+# neighbouring bins; with a short one, in unrelated bins.  The choices could
+# in principle depend on that (a read-modify-write group whose points repeat a
+# bin, same-address atomics in a row), so the tuner can be run per regime.
+REGIMES = {
+    # name: (trial periods lo, hi in days, cadence)
+    "uniform": (0.5, 20.0, "uniform"),   # times uniform over 1500 d
+    "long": (50.0, 500.0, "uniform"),    # P/dt 10-100x larger
+    "short": (0.05, 0.5, "uniform"),     # P/dt 10-40x smaller
+    "gappy": (0.05, 20.0, "gappy"),      # nightly clusters, seasonal gaps (ragged workload)
+}
+
+
+def bucket_curves(lo, hi, n_trials, target_evals, seed, cadence="uniform"):
     """Curves with lengths log-uniform in [max(lo, 16), hi), mixed signals,
     sized so that sum(len) * n_trials ~ target_evals."""
     from periodfind_amd import synth
@@ -43,11 +59,21 @@ def bucket_curves(lo, hi, n_trials, target_evals, seed):
     lens = np.exp(rng.uniform(np.log(a), np.log(hi), n)).astype(np.int64)
     times, mags, errs = [], [], []
     for i, L in enumerate(lens):
-        t = np.sort(rng.uniform(0.0, 1500.0, int(L))).astype(np.float32)
+        if cadence == "gappy":
+            t = synth.gappy_cadence(rng, int(L)).astype(np.float32)
+        else:
+            t = np.sort(rng.uniform(0.0, 1500.0, int(L))).astype(np.float32)
         times.append(t)
         mags.append(synth._signal(rng, t.astype(np.float64), i % 3))
         errs.append(np.full(int(L), 0.05, np.float32))
     return times, mags, errs
+
+
+def period_over_spacing(times, periods):
+    """Median trial period over the median spacing of consecutive points."""
+    gaps = np.concatenate([np.diff(t.astype(np.float64)) for t in times if t.size > 1])
+    gaps = gaps[gaps > 0]
+    return float(np.median(periods) / np.median(gaps)) if gaps.size else float("inf")
 
 
 def prepared(algo, mags):
@@ -60,18 +86,23 @@ def prepared(algo, mags):
     return mags
 
 
-def tune(algos, n_trials=6000, reps=3, target_evals=1.5e10, device=0, log=print):
+def tune(algos, n_trials=6000, reps=3, target_evals=1.5e10, device=0, log=print,
+         regime="uniform"):
     from periodfind_amd import capi
 
     ctx = capi.Context(device)
-    periods = np.linspace(0.5, 20.0, n_trials).astype(np.float32)
+    p_lo, p_hi, cadence = REGIMES[regime]
+    periods = np.linspace(p_lo, p_hi, n_trials).astype(np.float32)
     pdts = np.zeros(1, np.float32)
-    report = {"n_trials": n_trials, "reps": reps, "buckets": {}}
+    report = {"n_trials": n_trials, "reps": reps, "regime": regime,
+              "periods": [p_lo, p_hi], "cadence": cadence, "buckets": {}}
     table = []
     for bi, (lo, hi) in enumerate(BUCKETS):
-        t, m, e = bucket_curves(lo, hi, n_trials, target_evals, seed=20261203 + 100 + bi)
+        t, m, e = bucket_curves(lo, hi, n_trials, target_evals, seed=20261203 + 100 + bi,
+                                cadence=cadence)
         key = f"{lo}-{hi}"
-        report["buckets"][key] = {"n_curves": len(t), "n_points": int(sum(len(x) for x in t))}
+        report["buckets"][key] = {"n_curves": len(t), "n_points": int(sum(len(x) for x in t)),
+                                  "period_over_spacing": period_over_spacing(t, periods)}
         for algo in algos:
             cs = capi.CurveSet(ctx, t, prepared(algo, m), e if algo in ("fpw", "bls") else None)
             ref = None
@@ -118,6 +149,31 @@ def merge(table):
     return out
 
 
+def compare_reports(reports, base=0):
+    """Rows (bucket, algo, [(regime, best choice, P/dt, loss)]) comparing
+    tuning reports of several regimes: for each regime, its own best choice and
+    how much slower the base report's best choice is there (from the timings
+    of every candidate)."""
+    base_r = reports[base]
+    rows = []
+    worst = 0.0
+    for key, bucket in base_r["buckets"].items():
+        for algo, d0 in bucket.items():
+            if not isinstance(d0, dict):
+                continue
+            b0 = d0["best"]
+            pick = f"g{b0['group']}_c{b0['claim']}"
+            cells = []
+            for r in reports:
+                d = r["buckets"][key][algo]
+                loss = d["all"][pick] / d["best"]["s"] - 1.0
+                worst = max(worst, loss)
+                cells.append((r.get("regime", "?"), d["best"]["group"], d["best"]["claim"],
+                              r["buckets"][key]["period_over_spacing"], loss))
+            rows.append((key, algo, cells))
+    return rows, worst
+
+
 def device_arch(device=0):
     """Architecture name of `device` as the runtime reports it (e.g. gfx950)."""
     from periodfind_amd import capi
@@ -148,7 +204,22 @@ def main(argv=None):
     ap.add_argument("--trials", type=int, default=6000)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--target-evals", type=float, default=1.5e10)
+    ap.add_argument("--regime", default="uniform", choices=sorted(REGIMES),
+                    help="trial periods against point spacing (the shipped table: uniform)")
+    ap.add_argument("--compare", nargs="+", metavar="REPORT",
+                    help="only compare tuning reports of several regimes (the first is the "
+                         "base) and print the table")
     args = ap.parse_args(argv)
+    if args.compare:
+        reports = [json.load(open(p)) for p in args.compare]
+        rows, worst = compare_reports(reports)
+        print("bucket       algo  " + "  ".join(
+            f"{r.get('regime', '?'):>30s}" for r in reports))
+        for key, algo, cells in rows:
+            print(f"{key:12s} {algo:4s}  " + "  ".join(
+                f"g{g}c{c:<3d} P/dt={pd:9.2f} +{loss * 100:4.1f}%" for _, g, c, pd, loss in cells))
+        print(f"largest loss of the base choice in another regime: {worst * 100:.1f}%")
+        return 0
     algos = [a for a in args.algos.split(",") if a]
     for a in algos:
         if a not in GROUPS:
@@ -157,9 +228,9 @@ def main(argv=None):
     os.environ["PF_TUNE_TABLE"] = "none"
     out = args.out or os.path.join(HERE, f"tuning_{device_arch()}.txt")
     entries, report = tune(algos, args.trials, args.reps, args.target_evals,
-                           log=lambda s: print(s, flush=True))
+                           log=lambda s: print(s, flush=True), regime=args.regime)
     write_table(out, entries, [f"measured by periodfind_amd.tune: {args.trials} trials, "
-                               f"{args.reps} reps, buckets {BUCKETS}"])
+                               f"{args.reps} reps, regime {args.regime}, buckets {BUCKETS}"])
     print(f"tune: wrote {out}", flush=True)
     if args.report:
         with open(args.report, "w") as f:
