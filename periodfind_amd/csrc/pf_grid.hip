@@ -1986,6 +1986,253 @@ __global__ __launch_bounds__(kWave * kBlsWaves) void bls_persistent(GridArgs g, 
 }
 
 // ---------------------------------------------------------------------------
+// BLS with equal weights (BlsArgs.wsum set): every curve's weights w are
+// bitwise equal, so the serial f32 sum of the w of a bin's c points is the
+// curve's running sum wsum[c - 1], the same additions in the same order.  The
+// scan then counts points per bin (16-bit counts, two lanes' per LDS word,
+// curves of up to 65536 points; non-returning atomics: no dependency between
+// points)
+// and keeps only the Y
+// plane as a lane-private read-modify-write.  LDS per 64-trial block drops
+// from 2 (nb + 1) to nb + 1 + (nb + 1) / 2 words per lane (26.1 -> 19.5 KiB at
+// nb = 50), so 8 blocks fit per CU instead of 6.  The weight prefix sums are
+// formed from the counts after the scan and held in registers for the search
+// (two 32-element vectors read with the wave-uniform window index), the Y
+// prefix sums stay in LDS.  One wave per block.  Same value for every trial as
+// bls_block: the bins, the prefix sums and the search are the reference's
+// f32 operations in the reference's order.
+// ---------------------------------------------------------------------------
+typedef float f32x32 __attribute__((ext_vector_type(32)));
+
+// Weight prefix sums wp[0..nb] of one lane, nb <= 63.
+struct BlsWeightPrefix {
+    f32x32 lo, hi;
+    __device__ __forceinline__ float operator()(int q) const {
+        return q < 32 ? lo[q] : hi[q - 32];
+    }
+};
+
+template <bool HAS_PDT, int G, int W>
+__device__ __forceinline__ void bls_block_ew(const GridArgs& g, const BlsArgs& a,
+                                             const LaneTrial lt, const int lane,
+                                             const int wave) {
+    extern __shared__ float fl[];
+    const int nb = a.n_bins;
+    float* ycol = fl + lane;  // Y slot s at ycol[s * 64]
+    // Counts: lanes l and l + 32 share a word, l < 32 in the low half, so the
+    // count of bin k is at ccol[k * 32] (one multiply-add, as Y's slot) and
+    // the addend is a per-lane constant.
+    uint32_t* ccol = reinterpret_cast<uint32_t*>(fl + (nb + 1) * kWave) + (lane & 31);
+    const uint32_t cshift = static_cast<uint32_t>(lane >> 5) << 4;
+    const uint32_t cadd = 1u << cshift;
+    const int64_t p0 = g.off[lt.curve];
+    const int64_t n = g.off[lt.curve + 1] - p0;
+    if (wave == 0) {
+    for (int b = 0; b <= nb; ++b) ycol[b * kWave] = 0.0f;
+    for (int k = 0; k < nb; ++k) ccol[k * 32] = 0u;
+    const FoldTrial f = make_fold_trial(g.periods[lt.trial / g.nd], g.pdts[lt.trial % g.nd]);
+    const float* __restrict__ yw = a.yw + p0;
+    const float nf = static_cast<float>(nb);
+    const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
+
+    // Bin k: Y into slot k + 1, one count into the lane's half of count k.
+    // (8-point chunks: fewer registers than BLS's 16.)
+    scan_points_rmw<HAS_PDT, G, 1, 8, false>(
+        g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return yw[i]; }, ycol + kWave,
+        [&](const auto& ph, const auto& v, auto& off, auto& val) {
+            constexpr int K = group_size_v<decltype(ph)>;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const uint32_t k = mul_bin(ph[j], nf, nm1);
+                off[j] = k * kPlaneBytes;
+                val[0][j] = v[j];
+                __hip_atomic_fetch_add(ccol + k * 32, cadd, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        });
+    // Y prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
+    {
+        float py = 0.0f;
+        int k = 1;
+        for (; k + 7 <= nb; k += 8) {
+            float by[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) by[j] = ycol[(k + j) * kWave];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                py = py + by[j];
+                ycol[(k + j) * kWave] = py;
+            }
+        }
+        for (; k <= nb; ++k) {
+            py = py + ycol[k * kWave];
+            ycol[k * kWave] = py;
+        }
+    }
+    }  // wave 0
+    if constexpr (W > 1) __syncthreads();
+    // Weight of bin k: c_k points of equal weight, wsum[c_k - 1] (0 for an
+    // empty bin); prefix sums wp[k + 1] = wp[k] + W_k in bin order.  Indices
+    // are compile-time (registers); bins k >= nb read word 0 and count 0.
+    // (An empty curve has no running sums to read: every count is 0.)
+    const float* __restrict__ ws = a.wsum + p0;
+    BlsWeightPrefix wp;
+    float pw = 0.0f;
+    // Sixteen bins at a time: their count reads and table reads are in flight
+    // together.  `total` sums the decoded counts.
+    uint32_t total = 0;
+    auto bin_weights = [&](auto q_tag, f32x32& dst) {
+        constexpr int q = decltype(q_tag)::value;  // quarter of the 64 bins
+        constexpr int k0 = q * 16, j0 = (q & 1) * 16;
+        float wk[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int k = k0 + j;
+            const bool in = k < nb;
+            const uint32_t word = ccol[(in ? k : 0) * 32];
+            const uint32_t c = in ? ((word >> cshift) & 0xffffu) : 0u;
+            const float x = n > 0 ? ws[c > 0 ? c - 1 : 0] : 0.0f;
+            wk[j] = c > 0 ? x : 0.0f;
+            total += c;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            dst[j0 + j] = pw;  // wp[k0 + j]
+            pw = pw + wk[j];
+        }
+    };
+    bin_weights(std::integral_constant<int, 0>{}, wp.lo);
+    bin_weights(std::integral_constant<int, 1>{}, wp.lo);
+    bin_weights(std::integral_constant<int, 2>{}, wp.hi);
+    bin_weights(std::integral_constant<int, 3>{}, wp.hi);
+    // For k > nb the stored values are wp[nb] (the empty bins add 0), so
+    // wp(nb) is the total weight.  A 16-bit count overflows only for a curve
+    // of exactly 65536 points all in one bin; the decoded counts then no
+    // longer sum to n, and the statistic is 0 (every window holds all the
+    // weight or none, so none passes 0 < r/W < 1).
+    const bool overflow = total != static_cast<uint32_t>(n);
+    const float wt = overflow ? 0.0f : wp(nb);
+    const float ytot = ycol[nb * kWave];
+    double bn = 0.0, bd = 1.0;
+    if (wt > 0.0f) {
+        // As bls_block: exact maximum of fl(s^2 / (r (W - r))) over the
+        // windows with 0 < fl(r/W) < 1 (bls.rs:100-127).
+        const float rlo = bls_window_floor(wt);
+        auto cand = [&](float r, float sv) {
+            const float t = wt - r;
+            const float num = sv * sv;
+            const float den = r * t;
+            const double dn = num, dd = den;
+            const bool take = (r > rlo) & (t > 0.0f) & (dn * bd > bn * dd);
+            bn = take ? dn : bn;
+            bd = take ? dd : bd;
+        };
+        constexpr int kG = 8;
+        for (int p = wave; p < nb; p += W) {
+            const float w0 = wp(p);
+            const float y0 = ycol[p * kWave];
+            const float wtail = wt - w0;
+            const float ytail = ytot - y0;
+            const int d_in = min(a.d_hi, nb - p);
+            int d = a.d_lo;
+            for (; d + kG - 1 <= d_in; d += kG) {
+                float r[kG], sv[kG];
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    r[j] = wp(p + d + j) - w0;
+                    sv[j] = ycol[(p + d + j) * kWave] - y0;
+                }
+#pragma unroll
+                for (int j = 0; j < kG; ++j) cand(r[j], sv[j]);
+            }
+            for (; d <= d_in; ++d) cand(wp(p + d) - w0, ycol[(p + d) * kWave] - y0);
+            for (; d + kG - 1 <= a.d_hi; d += kG) {
+                float r[kG], sv[kG];
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    r[j] = wtail + wp(p + d + j - nb);
+                    sv[j] = ytail + ycol[(p + d + j - nb) * kWave];
+                }
+#pragma unroll
+                for (int j = 0; j < kG; ++j) cand(r[j], sv[j]);
+            }
+            for (; d <= a.d_hi; ++d)
+                cand(wtail + wp(p + d - nb), ytail + ycol[(p + d - nb) * kWave]);
+        }
+    }
+    if constexpr (W > 1) {
+        // Merge as bls_block: waves 1.. leave their pairs in Y slots 2v-2,
+        // 2v-1 (no longer read once every wave has searched), wave 0 keeps
+        // the largest quotient.
+        __syncthreads();
+        if (wave > 0) {
+            ycol[(2 * wave - 2) * kWave] = static_cast<float>(bn);
+            ycol[(2 * wave - 1) * kWave] = static_cast<float>(bd);
+        }
+        __syncthreads();
+        if (wave != 0) return;
+#pragma unroll
+        for (int v = 1; v < W; ++v) {
+            const double vn = ycol[(2 * v - 2) * kWave];
+            const double vd = ycol[(2 * v - 1) * kWave];
+            const bool take = vn * bd > bn * vd;
+            bn = take ? vn : bn;
+            bd = take ? vd : bd;
+        }
+    }
+    const float best = static_cast<float>(bn) / static_cast<float>(bd);
+    if (lt.active) g.out[lt.curve * g.T + lt.trial] = best;
+}
+
+// Waves per equal-weight BLS workgroup: wave 0 scans, all search.
+constexpr int kBlsEwWaves = 2;
+
+// One workgroup per (curve, 64-trial block), hardware-scheduled.
+template <bool HAS_PDT, int G>
+__global__ __launch_bounds__(kWave * kBlsEwWaves)
+__attribute__((amdgpu_waves_per_eu(2 * kBlsEwWaves))) void bls_ew_kernel(GridArgs g, BlsArgs a) {
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    // (wave-uniform: the search indexes registers by p = wave + 2k)
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    bls_block_ew<HAS_PDT, G, kBlsEwWaves>(g, a, lane_trial(g.n_blocks, g.tblocks, g.T, lane),
+                                          lane, wave);
+}
+
+// Persistent: each workgroup pulls (curve, 64-trial block) items from the
+// queue; wave 0 claims and hands the claim to the other waves through LDS.
+template <bool HAS_PDT>
+__global__ __launch_bounds__(kWave * kBlsEwWaves)
+__attribute__((amdgpu_waves_per_eu(2 * kBlsEwWaves))) void bls_ew_persistent(GridArgs g, BlsArgs a,
+                                                                WorkQueue q) {
+    __shared__ int64_t claim[2];
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    // (wave-uniform: the search indexes registers by p = wave + 2k)
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    int64_t seen = 0;  // used by wave 0 only
+    for (;;) {
+        if (wave == 0) {
+            const Claim cl = next_claim(q, g, seen);
+            if (lane == 0) {
+                claim[0] = cl.begin;
+                claim[1] = cl.end;
+            }
+        }
+        __syncthreads();
+        const int64_t begin = claim[0], end = claim[1];
+        __syncthreads();  // every wave holds the claim before wave 0 takes the next
+        if (begin >= q.total) break;
+        for (int64_t it = begin; it < end; ++it) {
+            const LaneTrial lt = queue_trial(q, g, it, lane);
+            switch (rank_group(q, it / g.tblocks)) {
+                case 2: bls_block_ew<HAS_PDT, 2, kBlsEwWaves>(g, a, lt, lane, wave); break;
+                case 4: bls_block_ew<HAS_PDT, 4, kBlsEwWaves>(g, a, lt, lane, wave); break;
+                default: bls_block_ew<HAS_PDT, 1, kBlsEwWaves>(g, a, lt, lane, wave); break;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers.  `grp` selects the lds_rmw group size (1, 2 or 4; CE
 // also 0 = LDS atomics).  With q == nullptr the grid kernel runs one
 // workgroup per (curve, trial block); otherwise the persistent kernel runs
@@ -2084,6 +2331,13 @@ void mf_g(const GridArgs& g, const MfArgs& a, size_t lds, int grp, const WorkQue
 template <bool P>
 void bls_g(const GridArgs& g, const BlsArgs& a, size_t lds, int grp, const WorkQueue* q,
            hipStream_t s) {
+    if (a.wsum) {
+        if (q) launch_persistent(bls_ew_persistent<P>, g, a, lds, *q, s, kBlsEwWaves);
+        else if (grp >= 4) launch_grid_kernel(bls_ew_kernel<P, 4>, g, a, lds, s, kBlsEwWaves);
+        else if (grp == 2) launch_grid_kernel(bls_ew_kernel<P, 2>, g, a, lds, s, kBlsEwWaves);
+        else launch_grid_kernel(bls_ew_kernel<P, 1>, g, a, lds, s, kBlsEwWaves);
+        return;
+    }
     if (q) launch_persistent(bls_persistent<P>, g, a, lds, *q, s, kBlsWaves);
     else if (grp >= 4) launch_grid_kernel(bls_kernel<P, 4>, g, a, lds, s, kBlsWaves);
     else if (grp == 2) launch_grid_kernel(bls_kernel<P, 2>, g, a, lds, s, kBlsWaves);

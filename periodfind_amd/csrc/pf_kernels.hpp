@@ -134,7 +134,18 @@ struct BlsArgs {
     const float* w;         // [npts] 1/(e*e)
     const float* yw;        // [npts] w*(y - weighted mean)
     int n_bins, d_lo, d_hi;
+    // Equal weights (every curve's w all bitwise equal): running serial sums
+    // of w per curve (launch_bls_wsum), so a bin of c points weighs
+    // wsum[c - 1]; the kernel then counts points per bin instead of summing
+    // w (bls_ew_*).  nullptr: the general kernel.
+    const float* wsum;
 };
+// The equal-weight kernel's limits: 16-bit counts (as CE's packed counts, a
+// curve of 65536 points overflows one only with every point in one bin, which
+// the kernel detects), and the weight prefix sums (n_bins + 1 values) held in
+// 64 registers.
+constexpr int64_t kBlsEwMaxLen = 65536;
+constexpr int kBlsEwMaxBins = 63;
 
 // `grp`: points per LDS read-modify-write group (1, 2 or 4), a tuning knob.
 // q == nullptr: one workgroup per (curve, trial block); else a persistent
@@ -175,6 +186,9 @@ inline size_t lds_aov(int nb) { return size_t(3) * nb * 64 * 4; }
 inline size_t lds_fpw(int nb) { return size_t(2) * nb * 64 * 4; }
 inline size_t lds_mf(int nb) { return size_t(3) * nb * 64 * 4; }
 inline size_t lds_bls(int nb) { return size_t(2) * (nb + 1) * 64 * 4; }
+// Equal weights: the Y plane (nb + 1 slots of 64 lanes) and 16-bit counts
+// (nb words of 32 lane pairs).
+inline size_t lds_bls_ew(int nb) { return size_t(nb + 1) * 64 * 4 + size_t(nb) * 32 * 4; }
 
 // Per-point / per-curve preparation kernels (pf_prep.hip).
 hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t npts,
@@ -202,6 +216,13 @@ hipError_t launch_curve_mean(const int64_t* off, const float* y, float* avg, int
                              hipStream_t s);
 hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w, float* mean,
                               float* yw, int64_t n_curves, int64_t max_len, hipStream_t s);
+// *unequal |= 1 if some curve's weights are not all bitwise equal (*unequal
+// must be cleared before); running serial sums of each curve's first weight
+// into wsum (the serial sums of its weights when they are all equal).
+hipError_t launch_bls_unequal(const int64_t* off, const float* w, int* unequal,
+                              int64_t n_curves, hipStream_t s);
+hipError_t launch_bls_wsum(const int64_t* off, const float* w, float* wsum, int64_t n_curves,
+                           hipStream_t s);
 
 // Per-curve summary statistics and top-n (pf_stats.hip).
 constexpr int64_t kStatsMaxT = 8192;

@@ -8,6 +8,7 @@
 #include "pf_device.hpp"
 #include "pf_kernels.hpp"
 #include "pf_numpy.hpp"
+#include "pf_wsum.hpp"
 
 namespace pf {
 
@@ -127,6 +128,58 @@ __global__ __launch_bounds__(64) void bls_mean_kernel(const int64_t* __restrict_
         sy = serial_add(sy, cp, cnt);
     }
     if (lane == 0) mean[c] = (tw > 0.0f) ? sy / tw : 0.0f;
+}
+
+// Equal weights, one wave per curve: *unequal = 1 if some weight of the
+// curve differs bitwise from its first (coalesced reads, one ballot).
+__global__ __launch_bounds__(64) void bls_unequal_kernel(const int64_t* __restrict__ off,
+                                                         const float* __restrict__ w,
+                                                         int* __restrict__ unequal) {
+    const int64_t c = blockIdx.x;
+    const int64_t a = off[c], b = off[c + 1];
+    if (a >= b) return;
+    const uint32_t w0 = __builtin_bit_cast(uint32_t, w[a]);
+    uint32_t diff = 0;
+    for (int64_t i = a + threadIdx.x; i < b; i += 64) diff |= __builtin_bit_cast(uint32_t, w[i]) ^ w0;
+    if (__ballot(diff != 0) && threadIdx.x == 0) atomicOr(unequal, 1);
+}
+
+// Running serial sums of equal weights, one wave per curve: wsum[a + k] =
+// (((w0 + w0) + ...) + w0), k + 1 terms, in f32 from 0, where w0 is the
+// curve's first weight: the serial sum of the weights of any k + 1 points of
+// a curve whose weights all equal w0.  equal_sums (pf_wsum.hpp) gives them as
+// a few f32 additions per binade of the sums plus arithmetic runs, which the
+// wave's lanes write together; a weight outside [2^-100, 2^100] (zero, tiny,
+// huge, inf, NaN) takes the serial loop in lane 0.
+__global__ __launch_bounds__(64) void bls_wsum_kernel(const int64_t* __restrict__ off,
+                                                      const float* __restrict__ w,
+                                                      float* __restrict__ wsum) {
+    const int64_t c = blockIdx.x;
+    const int64_t a = off[c], b = off[c + 1];
+    if (a >= b) return;
+    const int lane = static_cast<int>(threadIdx.x);
+    const float w0 = w[a];
+    float* __restrict__ out = wsum + a;
+    const int64_t n = b - a;
+    if (!(w0 >= 0x1p-100f && w0 <= 0x1p100f) || n > (int64_t(1) << 24)) {
+        if (lane == 0) {
+            float s = 0.0f;
+            for (int64_t k = 0; k < n; ++k) {
+                s = s + w0;
+                out[k] = s;
+            }
+        }
+        return;
+    }
+    equal_sums(
+        w0, n,
+        [&](int64_t k, float v) {
+            if (lane == 0) out[k] = v;
+        },
+        [&](int64_t k0, double base, double inc, int64_t j) {
+            for (int64_t i = lane; i < j; i += 64)
+                out[k0 + i] = static_cast<float>(base + static_cast<double>(i + 1) * inc);
+        });
 }
 
 // `bx` blocks per curve: yw[i] = w[i] * (y[i] - mean[c]).
@@ -462,6 +515,22 @@ hipError_t launch_bls_prepare(const int64_t* off, const float* y, const float* w
     if (bx > 64) bx = 64;
     hipLaunchKernelGGL(bls_center_kernel, dim3(static_cast<uint32_t>(bx * n_curves)),
                        dim3(kBlock), 0, s, off, y, w, mean, yw, bx);
+    return hipGetLastError();
+}
+
+hipError_t launch_bls_unequal(const int64_t* off, const float* w, int* unequal,
+                              int64_t n_curves, hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bls_unequal_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, w, unequal);
+    return hipGetLastError();
+}
+
+hipError_t launch_bls_wsum(const int64_t* off, const float* w, float* wsum, int64_t n_curves,
+                           hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bls_wsum_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, w, wsum);
     return hipGetLastError();
 }
 

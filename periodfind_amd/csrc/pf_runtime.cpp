@@ -111,6 +111,11 @@ struct CurveSet {
     bool fpw_ok = false;
     DevBuf bls_yw, bls_mean;
     bool bls_ok = false;
+    // BLS equal weights: running serial sums of w per curve, and whether every
+    // curve's weights are bitwise equal (then BLS counts points per bin,
+    // pf::BlsArgs::wsum).  Built from w alone, so kept across magnitude modes.
+    DevBuf bls_wsum, bls_flag;
+    bool bls_ew_ok = false, bls_ew = false;
     DevBuf mf_wyy;  // (w*y)*y (MF); w and w*y are shared with FPW
     bool mf_ok = false;
     DevBuf vn_lut, vn_lut_off;  // VN emission tables, for extents product vn_ext
@@ -139,6 +144,7 @@ struct CurveSet {
     }
     void invalidate() {
         t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = mhf_ok = order_ok = false;
+        bls_ew_ok = bls_ew = false;
         vn_ext = -1;
         ce_nmag = -1;
         out_T = 0;
@@ -154,8 +160,8 @@ struct CurveSet {
     }
     std::vector<DevBuf*> buffers() {
         return {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                &fpw_wy, &bls_yw, &bls_mean, &mf_wyy, &vn_lut, &vn_lut_off, &mhf_pt,
-                &mhf_sums, &mhf_part, &out, &order, &counter};
+                &fpw_wy, &bls_yw, &bls_mean, &bls_wsum, &bls_flag, &mf_wyy, &vn_lut,
+                &vn_lut_off, &mhf_pt, &mhf_sums, &mhf_part, &out, &order, &counter};
     }
     // A set over memory it does not own: no buffer may grow (a grow fails
     // instead of allocating).
@@ -776,6 +782,62 @@ const float* mags(CurveSet& cs, const Params& p) {
     return p.mag_prep == PF_MAG_AS_IS ? as<float>(cs.y) : as<float>(cs.yp);
 }
 
+// BLS equal weights.  The counting kernel (pf::BlsArgs::wsum) fits 8 blocks
+// per CU where the general one fits 6, at a somewhat higher cost per block
+// (one more LDS operation per point, the weight prefix read from registers in
+// the search).  So a launch takes it only where that saves a round of
+// blocks, or for the persistent launch, and never when it shares the device
+// with other statistics' launches (`shared`: overlapped with the jobs of a
+// multi-statistic call, the general kernel measured faster for the whole
+// call).  Sets over fixed memory (the allocation-free entry point), curves
+// beyond the 16-bit counts and more than kBlsEwMaxBins bins keep the general
+// kernel.  PF_BLS_EQUAL_WEIGHTS=0 disables the counting kernel, =1 takes it
+// for every launch it can serve (both for tests and comparisons).
+bool bls_ew_wanted(const pf_ctx* ctx, const CurveSet& cs, const Params& p, int64_t T,
+                   bool persistent, bool shared) {
+    if (p.algo != PF_ALGO_BLS || p.n_bins > pf::kBlsEwMaxBins) return false;
+    if (cs.bls_wsum.fixed || cs.max_len > pf::kBlsEwMaxLen || cs.npts == 0) return false;
+    const char* sw = std::getenv("PF_BLS_EQUAL_WEIGHTS");
+    if (sw && sw[0] == '0') return false;
+    if (sw && sw[0] == '1') return true;
+    if (shared) return false;
+    if (persistent) return true;
+    const int64_t blocks = cs.n_curves * ((T + pf::kWaveHost - 1) / pf::kWaveHost);
+    const int64_t slots_g =
+        std::max<int64_t>(1, static_cast<int64_t>(ctx->lds_max / pf::lds_bls(p.n_bins))) * ctx->n_cu;
+    const int64_t slots_e =
+        std::max<int64_t>(1, static_cast<int64_t>(ctx->lds_max / pf::lds_bls_ew(p.n_bins))) *
+        ctx->n_cu;
+    return (blocks + slots_g - 1) / slots_g > (blocks + slots_e - 1) / slots_e;
+}
+
+// The counting kernel's inputs (cs.w must be built), once per set: whether
+// every curve's weights are bitwise equal (without errors all weights are 1;
+// otherwise the device compares them and one flag is read back, which waits
+// for the stream), and the running sums of each curve's weight.  Call outside
+// graph capture.
+int ensure_bls_ew(CurveSet& cs, hipStream_t s) {
+    if (cs.bls_ew_ok) return PF_OK;
+    cs.bls_ew = false;
+    int unequal = 0;
+    if (cs.has_err) {
+        PF_TRY(ensure(cs.bls_flag, sizeof(int)));
+        PF_HIP(hipMemsetAsync(cs.bls_flag.p, 0, sizeof(int), s));
+        PF_HIP(pf::launch_bls_unequal(as<int64_t>(cs.off), as<float>(cs.w), as<int>(cs.bls_flag),
+                                      cs.n_curves, s));
+        PF_HIP(hipMemcpyAsync(&unequal, cs.bls_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PF_HIP(hipStreamSynchronize(s));
+    }
+    if (unequal == 0) {
+        PF_TRY(ensure(cs.bls_wsum, static_cast<size_t>(cs.npts) * sizeof(float)));
+        PF_HIP(pf::launch_bls_wsum(as<int64_t>(cs.off), as<float>(cs.w), as<float>(cs.bls_wsum),
+                                   cs.n_curves, s));
+        cs.bls_ew = true;
+    }
+    cs.bls_ew_ok = true;
+    return PF_OK;
+}
+
 // Build (or reuse) the derived arrays params.algo needs.
 int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt, hipStream_t s) {
     const int64_t n = cs.npts;
@@ -906,7 +968,7 @@ TrialGrid ctx_grid(const pf_ctx* ctx) {
 // context's events (not allowed while the stream is being captured).
 int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd,
                 const TrialGrid& tg, const pf::WorkQueue* q, bool timed, int group,
-                hipStream_t s) {
+                hipStream_t s, bool shared = false) {
     const int64_t T = np * nd;
     const bool has_pdt = tg.has_pdt;
 
@@ -1003,8 +1065,14 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
         case PF_ALGO_BLS: {
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
-            pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi};
-            le = pf::launch_bls(g, a, pf::lds_bls(p.n_bins), group, q, s);
+            // Equal weights: the counting kernel where bls_ew_wanted (the
+            // caller ran ensure_bls_ew).
+            const bool ew = cs.bls_ew_ok && cs.bls_ew &&
+                            bls_ew_wanted(ctx, cs, p, T, q != nullptr, shared);
+            pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi,
+                          ew ? as<float>(cs.bls_wsum) : nullptr};
+            le = pf::launch_bls(g, a, ew ? pf::lds_bls_ew(p.n_bins) : pf::lds_bls(p.n_bins),
+                                group, q, s);
             break;
         }
     }
@@ -1023,6 +1091,7 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     ctx->timed = false;
     if (cs.n_curves == 0 || T == 0) return PF_OK;
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
+    if (bls_ew_wanted(ctx, cs, p, T, false, false)) PF_TRY(ensure_bls_ew(cs, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
@@ -1244,6 +1313,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     }
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
+    if (bls_ew_wanted(ctx, cs, p, T, true, false)) PF_TRY(ensure_bls_ew(cs, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
@@ -1304,6 +1374,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
+        reinterpret_cast<uint64_t>(cs.bls_ew ? cs.bls_wsum.p : nullptr),
         reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
         reinterpret_cast<uint64_t>(cs.mf_wyy.p),
         reinterpret_cast<uint64_t>(p.algo == PF_ALGO_MF ? mf_templates(ctx, p.n_bins) : nullptr),
@@ -2205,6 +2276,8 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         if (cs.n_curves == 0) continue;
         const Params p = to_params(&params[j]);
         PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
+        if (bls_ew_wanted(ctx, cs, p, T, false, n_jobs > 1))
+            PF_TRY(ensure_bls_ew(cs, ctx->stream));
         cs.out_T = 0;
         PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
         cs.out_T = T;
@@ -2257,7 +2330,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         if (cs.n_curves > 0) {
             const Params p = to_params(&params[j]);
             PF_TRY(launch_grid(ctx, cs, p, n_periods, n_pdts, ctx_grid(ctx), nullptr, false,
-                               tune_choice(ctx, p.algo, cs.rep_len).group, s));
+                               tune_choice(ctx, p.algo, cs.rep_len).group, s, n_jobs > 1));
             const int use_max = p.algo == PF_ALGO_CE ? 0 : 1;
             PF_HIP(pf::launch_stats(as<float>(cs.out), cs.n_curves, T, n_stats, use_max,
                                     out_dev<float>(ctx, o_sum[j]), out_dev<int64_t>(ctx, o_idx[j]),

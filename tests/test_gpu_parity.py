@@ -198,6 +198,122 @@ def test_rmw_group_variants(ctx, algo, group, pdt):
     _check(algo, got, ref)
 
 
+def _const_err_curves(lengths, seed, consts):
+    """Seeded curves whose errors are constant per curve (consts cycled)."""
+    t, m, _ = _curves(lengths, seed)
+    e = [np.full(len(x), consts[i % len(consts)], np.float32) for i, x in enumerate(t)]
+    return t, m, e
+
+
+@pytest.mark.parametrize("ew", ["1", "0"])
+@pytest.mark.parametrize("group", [1, 2, 4])
+@pytest.mark.parametrize("pdt", [False, True])
+def test_bls_equal_weights(ctx, monkeypatch, ew, group, pdt):
+    """Curves whose errors are constant take BLS's equal-weight kernel: a bin
+    counts its points and weighs the curve's running serial sum of that many
+    weights (the same f32 additions as summing them), the weight prefix sums
+    sit in registers for the search.  PF_BLS_EQUAL_WEIGHTS=1 takes it for
+    every launch it can serve (by default only where it saves a round of
+    blocks: not at these small grids), =0 forces the general kernel.  Both
+    bit-identical to the oracle, for every group size,
+    with and without period derivatives, on 0 to 7 whole chunks with every
+    remainder, an empty curve and longer curves; constant errors include 0
+    (weight inf), NaN, negative, tiny (weight overflows to inf) and huge
+    (weight underflows to 0)."""
+    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", ew)
+    consts = [0.05, 0.1, 1.0, 0.013, 7.5, -0.2, 0.0, np.nan, 1e-30, 1e30]
+    lengths = CHUNK_LENGTHS + [0, 3000]
+    t, m, e = _const_err_curves(lengths, seed=23, consts=consts)
+    P = np.linspace(0.4, 11.0, 70).astype(np.float32)
+    D = np.array([0.0, 1e-5, -1e-5] if pdt else [0.0], np.float32)
+    saved = ctx.get_tuning()
+    try:
+        ctx.set_tuning([("bls", 0, group, 4)])
+        got = _gpu(ctx, "bls", t, m, e, P, D, **PARAMS["bls"])
+    finally:
+        ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
+    ref = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
+    _check("bls", got, ref)
+
+
+@pytest.mark.parametrize("params", [
+    dict(n_bins=2, qmin=0.01, qmax=1.0),
+    dict(n_bins=20, qmin=0.05, qmax=0.3),
+    dict(n_bins=31, qmin=0.0, qmax=1.0),
+    dict(n_bins=32, qmin=0.01, qmax=0.5),
+    dict(n_bins=33, qmin=0.01, qmax=0.9),
+    dict(n_bins=63, qmin=0.0, qmax=1.0),   # the largest the equal-weight kernel takes
+    dict(n_bins=64, qmin=0.01, qmax=0.5),  # general kernel
+    dict(n_bins=40, qmin=0.6, qmax=0.3),   # no duration: 0
+])
+def test_bls_equal_weights_params(ctx, monkeypatch, params):
+    """Bin counts around the equal-weight kernel's register halves (32) and
+    limit (63), errors given and absent (all weights 1)."""
+    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", "1")
+    t, m, e = _const_err_curves([200, 999, 40, 1], seed=24, consts=[0.05, 0.3])
+    P = np.linspace(0.5, 10.0, 130).astype(np.float32)
+    D = np.zeros(1, np.float32)
+    _check("bls", _gpu(ctx, "bls", t, m, e, P, D, **params),
+           _oracle("bls", t, m, e, P, D, **params))
+    ones = [np.ones(len(x), np.float32) for x in t]
+    got = ctx.grid("bls", t, m, P, D, errs=None, **params)
+    _check("bls", got, _oracle("bls", t, m, ones, P, D, **params))
+
+
+def test_bls_equal_weights_count_limit(ctx, monkeypatch):
+    """The equal-weight kernel's 16-bit counts at their limit: 65536 points
+    at one time put every point of a trial in one bin, even or odd (the count
+    overflows; BLS is 0 there, as every window holds all of the weight or
+    none); 65536 and 65535 spread points do not overflow.  Bit-identical to
+    the oracle."""
+    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", "1")
+    rng = np.random.default_rng(27)
+    t = [np.full(65536, 123.4, np.float32),
+         np.sort(rng.uniform(0.0, 3000.0, 65536)).astype(np.float32),
+         np.sort(rng.uniform(0.0, 3000.0, 65535)).astype(np.float32)]
+    m = [synth._signal(rng, x.astype(np.float64), i % 3) for i, x in enumerate(t)]
+    e = [np.full(x.size, 0.05, np.float32) for x in t]
+    P = np.geomspace(0.05, 100.0, 40).astype(np.float32)
+    D = np.zeros(1, np.float32)
+    got = _gpu(ctx, "bls", t, m, e, P, D, **PARAMS["bls"])
+    ref = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
+    _check("bls", got, ref)
+    assert np.all(got[0] == 0.0)
+
+
+def test_bls_equal_weights_ragged(ctx):
+    """Ragged persistent + graph launch on equal-weight curves (every length
+    bucket's group size) == plain launch == oracle peaks; a set with one
+    curve of unequal errors takes the general kernels and agrees too."""
+    from periodfind_amd import capi
+
+    rng = np.random.default_rng(25)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(6000), 40))).astype(int)
+    lens[7] = 0
+    t, m, e = _const_err_curves(list(lens), seed=26, consts=[0.05, 0.02, 0.11])
+    P = (1.0 / np.linspace(0.05, 5.0, 700)).astype(np.float32)
+    D = np.array([0.0, 1e-5, -1e-5], np.float32)
+    grid = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
+    oi, ov = O.find_top_peaks_batched(grid, 32, 2, use_max=True)
+    for unequal in (False, True):
+        if unequal:
+            e = [x.copy() for x in e]
+            e[3][::2] *= 1.5
+            grid = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
+            oi, ov = O.find_top_peaks_batched(grid, 32, 2, use_max=True)
+        cs = capi.CurveSet(ctx, t, m, errs=e)
+        ri, rv = cs.peaks("bls", P, D, n_peaks=32, min_distance=2, ragged=False,
+                          **PARAMS["bls"])
+        np.testing.assert_array_equal(ri, oi)
+        np.testing.assert_array_equal(rv, ov)
+        for _ in range(2):
+            gi, gv = cs.peaks("bls", P, D, n_peaks=32, min_distance=2, ragged=True,
+                              **PARAMS["bls"])
+            np.testing.assert_array_equal(gi, oi)
+            np.testing.assert_array_equal(gv, ov)
+        cs.close()
+
+
 def test_ls_precise_path(ctx, monkeypatch):
     """PF_LS_PRECISE=1 takes sincosf(fl(2 pi phase)), the exact fold and the
     reference's separate multiply and add; the default takes the hardware

@@ -305,6 +305,20 @@ def test_device_logf_is_the_c_library_logf():
     assert "bad=0 special_bad=0 split_bad=0" in r.stdout
 
 
+def test_bls_equal_weight_sums():
+    """The BLS equal-weight kernel's running sums of n equal weights
+    (periodfind_amd/csrc/pf_wsum.hpp: explicit additions plus arithmetic runs
+    per binade, compiled here for the host) equal the serial f32 loop bit for
+    bit, for 65536 terms of random, typical and rounding-tie weights."""
+    src = os.path.join(HERE, "exactness", "check_wsum.cpp")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "check_wsum")
+        subprocess.run(["g++", "-O2", "-ffp-contract=off", "-o", exe, src], check=True)
+        r = subprocess.run([exe, "4000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("bad=0")
+
+
 def test_ce_probability_division_identity():
     """CE's division-free fl(count / norm) equals the IEEE quotient for every
     integer count <= norm <= 2^17 (the kernel's bound for that path)."""
