@@ -736,9 +736,11 @@ __global__ __launch_bounds__(kWave) void vn_persistent(GridArgs g, VnArgs a, Wor
 
 // ---------------------------------------------------------------------------
 // AOV — rust/src/aov.rs:12-89.
-// LDS: per bin three planes (count, sum y, sum y^2), [bin][3][64] f32; the
-// count is an exact f32 integer (Rust's u32 count converted to f32 in the
-// epilogue, aov.rs:68).  Per-curve mean is the serial f32 mean (aov.rs:27).
+// LDS: per bin three planes (count, sum y, sum y^2), [bin][3][64]; the count
+// is a u32, as Rust's (converted to f32 in the epilogue, aov.rs:68), added by
+// a non-returning LDS atomic (counts do not depend on order); the two f32
+// sums are lane-private read-modify-writes in point order.  Per-curve mean
+// is the serial f32 mean (aov.rs:27).
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, bool EXT1, int G>
 __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, const LaneTrial lt) {
@@ -759,18 +761,26 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
     constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
 
     if constexpr (EXT1) {
-        scan_points_rmw<HAS_PDT, G, 3>(
+        // Offsets from the LDS base, this lane's column included (one
+        // multiply-add; bin nb, phase 1.0, wraps to bin 0, aov.rs:57-58):
+        // the count at +0, the sums y and y^2 at +256 and +512 (planes 0 and
+        // 1 of the read-modify-write, from base + 256).
+        const uint32_t lane4 = static_cast<uint32_t>(lane) * 4u;
+        char* base = reinterpret_cast<char*>(lds);
+        scan_points_rmw<HAS_PDT, G, 2>(
             g.t64 + p0, g.tt + p0, n, f,
-            [&](int64_t i) { return F2{uniform_load(y + i), uniform_load(yy + i)}; }, col,
+            [&](int64_t i) { return F2{uniform_load(y + i), uniform_load(yy + i)}; },
+            reinterpret_cast<float*>(base + kPlaneBytes),
             [&](const auto& ph, const auto& v, auto& off, auto& val) {
                 constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const uint32_t k = div_bin(ph[j], db);
-                    off[j] = __umul24((k == nbu) ? 0u : k, kBinBytes);
-                    val[0][j] = 1.0f;
-                    val[1][j] = v[j].a;
-                    val[2][j] = v[j].b;
+                    off[j] = __umul24((k == nbu) ? 0u : k, kBinBytes) + lane4;
+                    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(base + off[j]), 1u,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    val[0][j] = v[j].a;
+                    val[1][j] = v[j].b;
                 }
             });
     } else {
@@ -782,8 +792,10 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
                 for (int o = 0; o < a.ext; ++o) {
                     const uint32_t off1[1] = {
                         static_cast<uint32_t>(__umul24(wrap_add(k, o, nbu), kBinBytes))};
-                    const float val1[3][1] = {{1.0f}, {v[0].a}, {v[0].b}};
-                    lds_rmw(col, off1, val1);
+                    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(col) + off1[0] / 4, 1u,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    const float val1[2][1] = {{v[0].a}, {v[0].b}};
+                    lds_rmw(col + kWave, off1, val1);
                 }
             });
     }
@@ -791,8 +803,9 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
     // aov.rs:67-84.
     const float avg = a.avg[lt.curve];
     float s1 = 0.0f, s2 = 0.0f;
+    const uint32_t* ccount = reinterpret_cast<const uint32_t*>(col);
     for (int b = 0; b < nb; ++b) {
-        const float c = col[(3 * b) * kWave];
+        const float c = static_cast<float>(ccount[(3 * b) * kWave]);
         if (c != 0.0f) {
             const float aux = col[(3 * b + 1) * kWave] / c;
             const float res = aux - avg;
