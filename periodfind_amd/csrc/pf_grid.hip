@@ -932,8 +932,12 @@ __global__ __launch_bounds__(kWave) void ls_persistent(GridArgs g, LsArgs a, Wor
 
 // ---------------------------------------------------------------------------
 // FPW — rust/src/fpw.rs:13-67.
-// LDS: [bin][2][64] f32 (sum w, sum w*y); w = 1/(e*e), wy = w*y prepared per
-// curve set (fpw.rs:26-27, uncentred).
+// LDS: [bin][2][64] (sum w, sum w*y); w = 1/(e*e), wy = w*y prepared per
+// curve set (fpw.rs:26-27, uncentred).  With equal weights (FpwArgs::wsum,
+// every curve's w bitwise equal, *unequal == 0) the first plane counts the
+// bin's points by non-returning atomics instead (u32), and a bin of c points
+// weighs wsum[c - 1], the curve's running serial sum of c weights: the same
+// value as adding them; only the w*y plane stays a read-modify-write.
 // ---------------------------------------------------------------------------
 template <bool HAS_PDT, int G>
 __device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, const LaneTrial lt) {
@@ -949,23 +953,48 @@ __device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, c
     const float* __restrict__ wy = a.wy + p0;
     const float nf = static_cast<float>(nb);
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
+    // (wave-uniform: one scalar load of the set's flag)
+    const bool ew = a.wsum != nullptr && uniform_load(a.unequal) == 0;
 
-    scan_points_rmw<HAS_PDT, G, 2>(
-        g.t64 + p0, g.tt + p0, n, f,
-        [&](int64_t i) { return F2{uniform_load(w + i), uniform_load(wy + i)}; }, col,
-        [&](const auto& ph, const auto& v, auto& off, auto& val) {
-            constexpr int K = group_size_v<decltype(ph)>;
+    if (ew) {
+        uint32_t* ccol = reinterpret_cast<uint32_t*>(col);
+        scan_points_rmw<HAS_PDT, G, 1>(
+            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(wy + i); },
+            col + kWave, [&](const auto& ph, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(ph)>;
 #pragma unroll
-            for (int j = 0; j < K; ++j) {
-                off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
-                val[0][j] = v[j].a;
-                val[1][j] = v[j].b;
-            }
-        });
+                for (int j = 0; j < K; ++j) {
+                    off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
+                    val[0][j] = v[j];
+                    __hip_atomic_fetch_add(ccol + off[j] / 4, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
+            });
+    } else {
+        scan_points_rmw<HAS_PDT, G, 2>(
+            g.t64 + p0, g.tt + p0, n, f,
+            [&](int64_t i) { return F2{uniform_load(w + i), uniform_load(wy + i)}; }, col,
+            [&](const auto& ph, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(ph)>;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
+                    val[0][j] = v[j].a;
+                    val[1][j] = v[j].b;
+                }
+            });
+    }
     // fpw.rs:55-60.
+    const float* __restrict__ ws = ew ? a.wsum + p0 : nullptr;
     float s = 0.0f;
     for (int b = 0; b < nb; ++b) {
-        const float W = col[(2 * b) * kWave];
+        float W;
+        if (ew) {
+            const uint32_t c = reinterpret_cast<const uint32_t*>(col)[(2 * b) * kWave];
+            W = c > 0 ? ws[c - 1] : 0.0f;
+        } else {
+            W = col[(2 * b) * kWave];
+        }
         if (W > 0.0f) {
             const float Y = col[(2 * b + 1) * kWave];
             const float num = Y * Y;

@@ -128,6 +128,11 @@ struct FpwArgs {
     const float* w;         // [npts] 1/(e*e)
     const float* wy;        // [npts] w*y
     int n_bins;
+    // Equal weights: running serial sums of each curve's first weight (as
+    // BlsArgs::wsum) and a device flag, 0 when every curve's weights are
+    // bitwise equal; the kernel then counts points per bin.  nullptr: sums w.
+    const float* wsum;
+    const int* unequal;
 };
 
 struct BlsArgs {

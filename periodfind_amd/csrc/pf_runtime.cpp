@@ -111,11 +111,14 @@ struct CurveSet {
     bool fpw_ok = false;
     DevBuf bls_yw, bls_mean;
     bool bls_ok = false;
-    // BLS equal weights: running serial sums of w per curve, and whether every
-    // curve's weights are bitwise equal (then BLS counts points per bin,
-    // pf::BlsArgs::wsum).  Built from w alone, so kept across magnitude modes.
-    DevBuf bls_wsum, bls_flag;
-    bool bls_ew_ok = false, bls_ew = false;
+    // Equal weights: running serial sums of each curve's first weight, and a
+    // device flag, nonzero unless every curve's weights are bitwise equal
+    // (then FPW and BLS count points per bin and weigh a bin of c points by
+    // wsum[c - 1]).  The flag's value on the host once read back (BLS picks
+    // its kernel on the host).  Built from w alone, so kept across magnitude
+    // modes.
+    DevBuf wsum, w_flag;
+    bool wsum_ok = false, w_equal_known = false, w_equal = false;
     DevBuf mf_wyy;  // (w*y)*y (MF); w and w*y are shared with FPW
     bool mf_ok = false;
     DevBuf vn_lut, vn_lut_off;  // VN emission tables, for extents product vn_ext
@@ -144,7 +147,7 @@ struct CurveSet {
     }
     void invalidate() {
         t64_ok = tt_ok = aov_ok = w_ok = fpw_ok = bls_ok = mf_ok = mhf_ok = order_ok = false;
-        bls_ew_ok = bls_ew = false;
+        wsum_ok = w_equal_known = w_equal = false;
         vn_ext = -1;
         ce_nmag = -1;
         out_T = 0;
@@ -160,7 +163,7 @@ struct CurveSet {
     }
     std::vector<DevBuf*> buffers() {
         return {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
-                &fpw_wy, &bls_yw, &bls_mean, &bls_wsum, &bls_flag, &mf_wyy, &vn_lut,
+                &fpw_wy, &bls_yw, &bls_mean, &wsum, &w_flag, &mf_wyy, &vn_lut,
                 &vn_lut_off, &mhf_pt, &mhf_sums, &mhf_part, &out, &order, &counter};
     }
     // A set over memory it does not own: no buffer may grow (a grow fails
@@ -782,24 +785,49 @@ const float* mags(CurveSet& cs, const Params& p) {
     return p.mag_prep == PF_MAG_AS_IS ? as<float>(cs.y) : as<float>(cs.yp);
 }
 
-// BLS equal weights.  The counting kernel (pf::BlsArgs::wsum) fits 8 blocks
-// per CU where the general one fits 6, at a somewhat higher cost per block
-// (one more LDS operation per point, the weight prefix read from registers in
-// the search).  So a launch takes it only where that saves a round of
-// blocks, or for the persistent launch, and never when it shares the device
-// with other statistics' launches (`shared`: overlapped with the jobs of a
+// Equal weights (every curve's weights bitwise equal, as with constant or
+// absent errors): a bin's serial sum of weights is then the curve's running
+// sum wsum[c - 1] for its c points, so FPW and BLS count points per bin
+// instead of summing weights.  PF_EQUAL_WEIGHTS=0 disables this, =1 makes
+// BLS take its counting kernel for every launch it can serve (both for tests
+// and comparisons).
+int ew_switch() {
+    const char* sw = std::getenv("PF_EQUAL_WEIGHTS");
+    return sw && sw[0] == '0' ? 0 : (sw && sw[0] == '1' ? 1 : -1);
+}
+
+// The running sums and the device flag, once per set (cs.w must be built).
+// No wait: the kernels read the flag.  Sets over fixed memory (the
+// allocation-free entry point) have neither.
+int ensure_wsum(CurveSet& cs, hipStream_t s) {
+    if (cs.wsum_ok || cs.wsum.fixed || cs.npts == 0) return PF_OK;
+    PF_TRY(ensure(cs.w_flag, sizeof(int)));
+    PF_TRY(ensure(cs.wsum, static_cast<size_t>(cs.npts) * sizeof(float)));
+    PF_HIP(hipMemsetAsync(cs.w_flag.p, 0, sizeof(int), s));
+    if (cs.has_err)  // without errors every weight is 1
+        PF_HIP(pf::launch_bls_unequal(as<int64_t>(cs.off), as<float>(cs.w), as<int>(cs.w_flag),
+                                      cs.n_curves, s));
+    PF_HIP(pf::launch_bls_wsum(as<int64_t>(cs.off), as<float>(cs.w), as<float>(cs.wsum),
+                               cs.n_curves, s));
+    cs.wsum_ok = true;
+    return PF_OK;
+}
+
+// BLS's counting kernel (pf::BlsArgs::wsum) fits 8 blocks per CU where the
+// general one fits 6, at a somewhat higher cost per block (one more LDS
+// operation per point, the weight prefix read from registers in the search).
+// So a launch takes it only where that saves a round of blocks, or for the
+// persistent launch, and never when it shares the device with other
+// statistics' launches (`shared`: overlapped with the jobs of a
 // multi-statistic call, the general kernel measured faster for the whole
-// call).  Sets over fixed memory (the allocation-free entry point), curves
-// beyond the 16-bit counts and more than kBlsEwMaxBins bins keep the general
-// kernel.  PF_BLS_EQUAL_WEIGHTS=0 disables the counting kernel, =1 takes it
-// for every launch it can serve (both for tests and comparisons).
+// call).  Curves beyond the 16-bit counts and more than kBlsEwMaxBins bins
+// keep the general kernel.
 bool bls_ew_wanted(const pf_ctx* ctx, const CurveSet& cs, const Params& p, int64_t T,
                    bool persistent, bool shared) {
     if (p.algo != PF_ALGO_BLS || p.n_bins > pf::kBlsEwMaxBins) return false;
-    if (cs.bls_wsum.fixed || cs.max_len > pf::kBlsEwMaxLen || cs.npts == 0) return false;
-    const char* sw = std::getenv("PF_BLS_EQUAL_WEIGHTS");
-    if (sw && sw[0] == '0') return false;
-    if (sw && sw[0] == '1') return true;
+    if (cs.wsum.fixed || cs.max_len > pf::kBlsEwMaxLen || cs.npts == 0) return false;
+    const int sw = ew_switch();
+    if (sw >= 0) return sw == 1;
     if (shared) return false;
     if (persistent) return true;
     const int64_t blocks = cs.n_curves * ((T + pf::kWaveHost - 1) / pf::kWaveHost);
@@ -811,30 +839,27 @@ bool bls_ew_wanted(const pf_ctx* ctx, const CurveSet& cs, const Params& p, int64
     return (blocks + slots_g - 1) / slots_g > (blocks + slots_e - 1) / slots_e;
 }
 
-// The counting kernel's inputs (cs.w must be built), once per set: whether
-// every curve's weights are bitwise equal (without errors all weights are 1;
-// otherwise the device compares them and one flag is read back, which waits
-// for the stream), and the running sums of each curve's weight.  Call outside
-// graph capture.
-int ensure_bls_ew(CurveSet& cs, hipStream_t s) {
-    if (cs.bls_ew_ok) return PF_OK;
-    cs.bls_ew = false;
-    int unequal = 0;
-    if (cs.has_err) {
-        PF_TRY(ensure(cs.bls_flag, sizeof(int)));
-        PF_HIP(hipMemsetAsync(cs.bls_flag.p, 0, sizeof(int), s));
-        PF_HIP(pf::launch_bls_unequal(as<int64_t>(cs.off), as<float>(cs.w), as<int>(cs.bls_flag),
-                                      cs.n_curves, s));
-        PF_HIP(hipMemcpyAsync(&unequal, cs.bls_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+// FPW counts points per bin whenever the weights are equal: the count takes
+// the place of the weight plane, the kernel reads the device flag itself.
+bool fpw_ew_wanted(const CurveSet& cs, const Params& p) {
+    return p.algo == PF_ALGO_FPW && !cs.wsum.fixed && cs.npts > 0 && ew_switch() != 0;
+}
+
+// The equal-weight inputs a launch of `p` will use, before it is queued (and
+// outside graph capture: BLS reads the flag back, which waits for the
+// stream, once per set).
+int prepare_equal_weights(const pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t T,
+                          bool persistent, bool shared, hipStream_t s) {
+    if (fpw_ew_wanted(cs, p)) return ensure_wsum(cs, s);
+    if (!bls_ew_wanted(ctx, cs, p, T, persistent, shared)) return PF_OK;
+    PF_TRY(ensure_wsum(cs, s));
+    if (!cs.w_equal_known) {
+        int unequal = 0;
+        PF_HIP(hipMemcpyAsync(&unequal, cs.w_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PF_HIP(hipStreamSynchronize(s));
+        cs.w_equal = unequal == 0;
+        cs.w_equal_known = true;
     }
-    if (unequal == 0) {
-        PF_TRY(ensure(cs.bls_wsum, static_cast<size_t>(cs.npts) * sizeof(float)));
-        PF_HIP(pf::launch_bls_wsum(as<int64_t>(cs.off), as<float>(cs.w), as<float>(cs.bls_wsum),
-                                   cs.n_curves, s));
-        cs.bls_ew = true;
-    }
-    cs.bls_ew_ok = true;
     return PF_OK;
 }
 
@@ -1031,7 +1056,9 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             break;
         }
         case PF_ALGO_FPW: {
-            pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins};
+            const bool ew = cs.wsum_ok && fpw_ew_wanted(cs, p);
+            pf::FpwArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), p.n_bins,
+                          ew ? as<float>(cs.wsum) : nullptr, ew ? as<int>(cs.w_flag) : nullptr};
             le = pf::launch_fpw(g, a, pf::lds_fpw(p.n_bins), group, q, s);
             break;
         }
@@ -1066,11 +1093,11 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             int lo, hi;
             bls_duration_range(p.n_bins, p.qmin, p.qmax, &lo, &hi);
             // Equal weights: the counting kernel where bls_ew_wanted (the
-            // caller ran ensure_bls_ew).
-            const bool ew = cs.bls_ew_ok && cs.bls_ew &&
+            // caller ran prepare_equal_weights).
+            const bool ew = cs.w_equal_known && cs.w_equal &&
                             bls_ew_wanted(ctx, cs, p, T, q != nullptr, shared);
             pf::BlsArgs a{as<float>(cs.w), as<float>(cs.bls_yw), p.n_bins, lo, hi,
-                          ew ? as<float>(cs.bls_wsum) : nullptr};
+                          ew ? as<float>(cs.wsum) : nullptr};
             le = pf::launch_bls(g, a, ew ? pf::lds_bls_ew(p.n_bins) : pf::lds_bls(p.n_bins),
                                 group, q, s);
             break;
@@ -1091,7 +1118,7 @@ int run_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t nd)
     ctx->timed = false;
     if (cs.n_curves == 0 || T == 0) return PF_OK;
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
-    if (bls_ew_wanted(ctx, cs, p, T, false, false)) PF_TRY(ensure_bls_ew(cs, ctx->stream));
+    PF_TRY(prepare_equal_weights(ctx, cs, p, T, false, false, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
@@ -1313,7 +1340,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
     }
     PF_TRY(upload_grid(ctx, periods, np, pdts, nd));
     PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
-    if (bls_ew_wanted(ctx, cs, p, T, true, false)) PF_TRY(ensure_bls_ew(cs, ctx->stream));
+    PF_TRY(prepare_equal_weights(ctx, cs, p, T, true, false, ctx->stream));
     cs.out_T = 0;
     PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
     cs.out_T = T;
@@ -1374,7 +1401,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
-        reinterpret_cast<uint64_t>(cs.bls_ew ? cs.bls_wsum.p : nullptr),
+        reinterpret_cast<uint64_t>(cs.wsum_ok ? cs.wsum.p : nullptr),
+        static_cast<uint64_t>(cs.w_equal_known && cs.w_equal),
         reinterpret_cast<uint64_t>(cs.order.p), reinterpret_cast<uint64_t>(cs.counter.p),
         reinterpret_cast<uint64_t>(cs.mf_wyy.p),
         reinterpret_cast<uint64_t>(p.algo == PF_ALGO_MF ? mf_templates(ctx, p.n_bins) : nullptr),
@@ -2276,8 +2304,7 @@ int pf_curves_stats_many(pf_ctx* ctx, int64_t n_jobs, pf_curves* const* sets,
         if (cs.n_curves == 0) continue;
         const Params p = to_params(&params[j]);
         PF_TRY(prepare(ctx, cs, p, ctx->has_pdt, ctx->stream));
-        if (bls_ew_wanted(ctx, cs, p, T, false, n_jobs > 1))
-            PF_TRY(ensure_bls_ew(cs, ctx->stream));
+        PF_TRY(prepare_equal_weights(ctx, cs, p, T, false, n_jobs > 1, ctx->stream));
         cs.out_T = 0;
         PF_TRY(ensure(cs.out, static_cast<size_t>(cs.n_curves) * T * sizeof(float)));
         cs.out_T = T;

@@ -205,22 +205,23 @@ def _const_err_curves(lengths, seed, consts):
     return t, m, e
 
 
+@pytest.mark.parametrize("algo", ["fpw", "bls"])
 @pytest.mark.parametrize("ew", ["1", "0"])
 @pytest.mark.parametrize("group", [1, 2, 4])
 @pytest.mark.parametrize("pdt", [False, True])
-def test_bls_equal_weights(ctx, monkeypatch, ew, group, pdt):
-    """Curves whose errors are constant take BLS's equal-weight kernel: a bin
-    counts its points and weighs the curve's running serial sum of that many
-    weights (the same f32 additions as summing them), the weight prefix sums
-    sit in registers for the search.  PF_BLS_EQUAL_WEIGHTS=1 takes it for
-    every launch it can serve (by default only where it saves a round of
-    blocks: not at these small grids), =0 forces the general kernel.  Both
-    bit-identical to the oracle, for every group size,
+def test_equal_weights(ctx, monkeypatch, algo, ew, group, pdt):
+    """Curves whose errors are constant: FPW and BLS count points per bin and
+    weigh a bin of c points by the curve's running serial sum of c weights
+    (the same f32 additions as summing them); BLS's counting kernel holds the
+    weight prefix sums in registers for the search.  PF_EQUAL_WEIGHTS=1 makes
+    BLS take it for every launch it can serve (by default only where it saves
+    a round of blocks: not at these small grids), =0 turns counting off for
+    both.  Both bit-identical to the oracle, for every group size,
     with and without period derivatives, on 0 to 7 whole chunks with every
     remainder, an empty curve and longer curves; constant errors include 0
     (weight inf), NaN, negative, tiny (weight overflows to inf) and huge
     (weight underflows to 0)."""
-    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", ew)
+    monkeypatch.setenv("PF_EQUAL_WEIGHTS", ew)
     consts = [0.05, 0.1, 1.0, 0.013, 7.5, -0.2, 0.0, np.nan, 1e-30, 1e30]
     lengths = CHUNK_LENGTHS + [0, 3000]
     t, m, e = _const_err_curves(lengths, seed=23, consts=consts)
@@ -228,12 +229,32 @@ def test_bls_equal_weights(ctx, monkeypatch, ew, group, pdt):
     D = np.array([0.0, 1e-5, -1e-5] if pdt else [0.0], np.float32)
     saved = ctx.get_tuning()
     try:
-        ctx.set_tuning([("bls", 0, group, 4)])
-        got = _gpu(ctx, "bls", t, m, e, P, D, **PARAMS["bls"])
+        ctx.set_tuning([(algo, 0, group, 4)])
+        got = _gpu(ctx, algo, t, m, e, P, D, **PARAMS[algo])
     finally:
         ctx.set_tuning([(a, ml, g, c) for a, ml, g, c in saved])
-    ref = _oracle("bls", t, m, e, P, D, **PARAMS["bls"])
-    _check("bls", got, ref)
+    ref = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+    _check(algo, got, ref)
+
+
+def test_fpw_equal_weights_sets(ctx):
+    """FPW over resident sets: constant errors, no errors (all weights 1),
+    and one curve of unequal errors (the flag turns counting off for the
+    whole set), alone and inside a multi-statistic call; all equal to the
+    oracle."""
+    from periodfind_amd import capi
+
+    t, m, e = _const_err_curves([300, 1000, 5, 0, 2049], seed=28, consts=[0.05, 0.2])
+    P = np.linspace(0.5, 10.0, 200).astype(np.float32)
+    D = np.array([0.0, 2e-5], np.float32)
+    e2 = [x.copy() for x in e]
+    e2[1][::3] *= 1.7
+    ones = [np.ones(len(x), np.float32) for x in t]
+    for errs, ref_e in ((e, e), (None, ones), (e2, e2)):
+        cs = capi.CurveSet(ctx, t, m, errs=errs)
+        got = cs.grid("fpw", P, D, n_bins=10)
+        _check("fpw", got, _oracle("fpw", t, m, ref_e, P, D, n_bins=10))
+        cs.close()
 
 
 @pytest.mark.parametrize("params", [
@@ -249,7 +270,7 @@ def test_bls_equal_weights(ctx, monkeypatch, ew, group, pdt):
 def test_bls_equal_weights_params(ctx, monkeypatch, params):
     """Bin counts around the equal-weight kernel's register halves (32) and
     limit (63), errors given and absent (all weights 1)."""
-    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", "1")
+    monkeypatch.setenv("PF_EQUAL_WEIGHTS", "1")
     t, m, e = _const_err_curves([200, 999, 40, 1], seed=24, consts=[0.05, 0.3])
     P = np.linspace(0.5, 10.0, 130).astype(np.float32)
     D = np.zeros(1, np.float32)
@@ -266,7 +287,7 @@ def test_bls_equal_weights_count_limit(ctx, monkeypatch):
     overflows; BLS is 0 there, as every window holds all of the weight or
     none); 65536 and 65535 spread points do not overflow.  Bit-identical to
     the oracle."""
-    monkeypatch.setenv("PF_BLS_EQUAL_WEIGHTS", "1")
+    monkeypatch.setenv("PF_EQUAL_WEIGHTS", "1")
     rng = np.random.default_rng(27)
     t = [np.full(65536, 123.4, np.float32),
          np.sort(rng.uniform(0.0, 3000.0, 65536)).astype(np.float32),
