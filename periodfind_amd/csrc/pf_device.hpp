@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pf_fastbin.hpp"
 #include "pf_logf.hpp"
 
 namespace pf {
@@ -110,6 +111,114 @@ __device__ __forceinline__ void fold_phases(const double (&t)[K], const double (
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < K; ++j) ph[j] = static_cast<float>(q0[j]);
+}
+
+// The lane's fast-bin rule (pf_fastbin.hpp) for n bins of the division rule
+// (div) or the multiplication rule, from its curve's time bound; without a
+// bound (tmax == nullptr) eps is NaN and every point takes the exact path.
+__device__ __forceinline__ BinRule lane_bin_rule(const float* tmax, int64_t curve,
+                                                 const FoldTrial& f, int n, bool div) {
+    const float tm = tmax ? tmax[curve] : __builtin_nanf("");
+    return make_bin_rule(n, div, tm, f.k, f.p);
+}
+
+// Bins of K points, stage by stage (as fold_phases): the fast bin of
+// pf_fastbin.hpp where it is certain, otherwise exact(phase) of the exact
+// fold (the statistic's own bin rule, with its wrap or clamp).  The fast
+// bin is six operations per point against the exact fold's five plus the
+// bin's three or four; a point within eps of a bin edge sends its wave
+// through one extra branch for the chunk.  The conversion of y saturates in
+// hardware (NaN to 0); such lanes are exactly those replaced.
+//
+// Split in two so a caller can put independent work between the compare and
+// the branch that depends on it: fold_bins_start (no branch) gives the fast
+// bins and, per point, whether it needs the exact path; fold_bins_fix takes
+// that path where needed.
+template <bool HAS_PDT, int K>
+__device__ __forceinline__ void fold_bins_start(const double (&t)[K], const double (&tt)[K],
+                                                const FoldTrial& f, const BinRule& br,
+                                                uint32_t (&k)[K], bool (&slow)[K]) {
+    double y[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = (HAS_PDT ? t[j] - f.k * tt[j] : t[j]) * f.inv_p;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = __builtin_amdgcn_fract(y[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = __builtin_fma(y[j], br.c, br.eps);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        k[j] = static_cast<uint32_t>(y[j]);
+        slow[j] = !(__builtin_amdgcn_fract(y[j]) > br.eps2);
+    }
+}
+
+template <bool HAS_PDT, int K, class Exact>
+__device__ __forceinline__ void fold_bins_fix(const double (&t)[K], const double (&tt)[K],
+                                              const FoldTrial& f, Exact exact,
+                                              const bool (&slow)[K], uint32_t (&k)[K]) {
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < K; ++j) any |= slow[j];
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (slow[j]) k[j] = exact(fold_phase<HAS_PDT>(t[j], tt[j], f));
+    }
+}
+
+// fold_bins of a whole chunk, the form the atomic histograms take: one
+// branch per chunk, on the smallest fract(y) of the lane (one v_min per
+// point instead of a compare and a scalar or).  A NaN fract(y) drops out of
+// the minimum, but it only arises with an infinite or NaN eps (a NaN or
+// infinite time bounds its curve as +inf; a zero or non-finite period makes
+// q infinite or NaN), whose lanes fail the test anyway.  The exact branch
+// recomputes each point's test.
+template <bool HAS_PDT, int K, class Exact>
+__device__ __forceinline__ void fold_bins(const double (&t)[K], const double (&tt)[K],
+                                          const FoldTrial& f, const BinRule& br, Exact exact,
+                                          uint32_t (&k)[K]) {
+    double y[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = (HAS_PDT ? t[j] - f.k * tt[j] : t[j]) * f.inv_p;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = __builtin_amdgcn_fract(y[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) y[j] = __builtin_fma(y[j], br.c, br.eps);
+    __builtin_amdgcn_sched_barrier(0);
+    double lo = 1.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        k[j] = static_cast<uint32_t>(y[j]);
+        lo = __builtin_fmin(lo, __builtin_amdgcn_fract(y[j]));
+    }
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(lo > br.eps2)) != 0, 0)) {
+        // y again (the same operations), so it need not stay live.
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const double q = (HAS_PDT ? t[j] - f.k * tt[j] : t[j]) * f.inv_p;
+            const double yj = __builtin_fma(__builtin_amdgcn_fract(q), br.c, br.eps);
+            if (!(__builtin_amdgcn_fract(yj) > br.eps2))
+                k[j] = exact(fold_phase<HAS_PDT>(t[j], tt[j], f));
+        }
+    }
+}
+
+// A statistic's fast-bin inputs for the scans: the lane's rule and its exact
+// bin of a phase (wrapped or clamped as the statistic requires).
+template <class Exact>
+struct FastBins {
+    BinRule rule;
+    Exact exact;
+};
+
+template <class Exact>
+__device__ __forceinline__ FastBins<Exact> fast_bins(const BinRule& r, Exact e) {
+    return FastBins<Exact>{r, e};
 }
 
 // Exact `(phase / b) as usize` with b = fl(1/n) (rust/src/ce.rs:52,

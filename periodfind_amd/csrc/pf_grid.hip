@@ -119,6 +119,22 @@ __device__ __forceinline__ void wait_scalar_loads() {
 template <class A>
 constexpr int group_size_v = static_cast<int>(std::extent<std::remove_reference_t<A>>::value);
 
+// A value the register allocator must keep rather than recompute: an LDS
+// offset used by a read and, a group later, by the write back (MF: the
+// allocator otherwise re-derives it from the bin, 200 instead of 186 VALU
+// per 16 points; elsewhere keeping it measured no better).  An empty asm
+// statement with the value as an in/out operand; no instruction.
+__device__ __forceinline__ uint32_t keep_reg(uint32_t v) {
+    asm("" : "+v"(v));
+    return v;
+}
+
+// The scans hand their bodies phases (NoBins) or, given FastBins, the
+// points' final bins (fold_bins: fast where certain, else the exact rule).
+struct NoBins {};
+template <class B>
+constexpr bool is_fast_bins_v = !std::is_same<B, NoBins>::value;
+
 // Walk the curve's points in order.  Points are loaded in chunks of C: the
 // wave-uniform loads of chunk i+1 (times and the statistic's aux record,
 // scalar loads) are issued before chunk i is folded and accumulated, so the
@@ -126,11 +142,18 @@ constexpr int group_size_v = static_cast<int>(std::extent<std::remove_reference_
 // counter) falls after a whole chunk of VALU work.  The prefetch index is
 // clamped to the last full chunk, so it never reads past the curve.
 // `body(ph[G], rec[G])` receives the folded phases of G consecutive points
-// (G inside full chunks, 1 for the remainder).
-template <bool HAS_PDT, int G, bool EXACT = true, class Load, class Body>
+// (G inside full chunks, 1 for the remainder); with FastBins, `body(k[G],
+// rec[G])` receives their bins instead, a whole chunk's computed together.
+// SCALAR = false reads the times with vector loads (the caller's `load`
+// must too): they count on vmcnt, so no wait on the point stream also waits
+// for the chunk's LDS operations.
+template <bool HAS_PDT, int G, bool EXACT = true, bool SCALAR = true, class Load, class Body,
+          class Bins = NoBins>
 __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
                                             const double* __restrict__ tt, int64_t n,
-                                            const FoldTrial& f, Load load, Body body) {
+                                            const FoldTrial& f, Load load, Body body,
+                                            const Bins& bins = Bins{}) {
+    constexpr bool FASTB = is_fast_bins_v<Bins>;
     using Rec = decltype(load(int64_t(0)));
     constexpr int C = HAS_PDT ? kChunk / 2 : kChunk;
     static_assert(C % G == 0, "chunk must hold whole groups");
@@ -142,17 +165,19 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
     auto fetch = [&](int64_t base, Chunk& ch) {
 #pragma unroll
         for (int j = 0; j < C; ++j) {
-            ch.t[j] = uniform_load(t64 + base + j);
-            ch.t2[j] = HAS_PDT ? uniform_load(tt + base + j) : 0.0;
+            ch.t[j] = SCALAR ? uniform_load(t64 + base + j) : t64[base + j];
+            ch.t2[j] = HAS_PDT ? (SCALAR ? uniform_load(tt + base + j) : tt[base + j]) : 0.0;
             ch.r[j] = load(base + j);
         }
     };
+    using Ph = std::conditional_t<FASTB, uint32_t, float>;
     auto run = [&](const Chunk& ch) {
-        float phc[C];
-        fold_phases<HAS_PDT, C, EXACT>(ch.t, ch.t2, f, phc);
+        Ph phc[C];
+        if constexpr (FASTB) fold_bins<HAS_PDT, C>(ch.t, ch.t2, f, bins.rule, bins.exact, phc);
+        else fold_phases<HAS_PDT, C, EXACT>(ch.t, ch.t2, f, phc);
 #pragma unroll
         for (int gi = 0; gi < C; gi += G) {
-            float ph[G];
+            Ph ph[G];
             Rec rc[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
@@ -174,11 +199,11 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
         Chunk a, b;
         fetch(0, a);
         for (; i + 2 * C <= n; i += 2 * C) {
-            wait_scalar_loads();
+            if (SCALAR) wait_scalar_loads();
             fetch(i + C, b);
             __builtin_amdgcn_sched_barrier(0);
             run(a);
-            wait_scalar_loads();
+            if (SCALAR) wait_scalar_loads();
             const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
             fetch(na, a);
             __builtin_amdgcn_sched_barrier(0);
@@ -186,7 +211,10 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
         }
     }
     for (; i < n; ++i) {
-        float ph[1] = {fold_phase<HAS_PDT, EXACT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
+        const float p1 = fold_phase<HAS_PDT, EXACT>(t64[i], HAS_PDT ? tt[i] : 0.0, f);
+        Ph ph[1];
+        if constexpr (FASTB) ph[0] = bins.exact(p1);
+        else ph[0] = p1;
         Rec rc[1] = {load(i)};
         body(ph, rc);
     }
@@ -207,12 +235,16 @@ __device__ __forceinline__ void scan_points(const double* __restrict__ t64,
 // for the next chunk's prefetch; vector loads count on vmcnt instead, at the
 // cost of one copy of the point per lane.  Measured at 8192 points: scalar
 // is 3-5% faster for AOV and FPW, vector 28% faster for BLS (16-point
-// chunks, one LDS wait per point).
-template <bool HAS_PDT, int G, int A, int CH = kChunk, bool SCALAR = true, class Load, class Prep>
+// chunks, one LDS wait per point).  With FastBins, `prep` receives the
+// group's bins (k[K]) in place of its phases.
+template <bool HAS_PDT, int G, int A, int CH = kChunk, bool SCALAR = true, class Load, class Prep,
+          class Bins = NoBins>
 __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
                                                 const double* __restrict__ tt, int64_t n,
                                                 const FoldTrial& f, Load load, float* col,
-                                                Prep prep) {
+                                                Prep prep, const Bins& bins = Bins{}) {
+    constexpr bool FASTB = is_fast_bins_v<Bins>;
+    using Ph = std::conditional_t<FASTB, uint32_t, float>;
     using Rec = decltype(load(int64_t(0)));
     constexpr int C = HAS_PDT ? CH / 2 : CH;
     constexpr int NG = C / G;
@@ -234,18 +266,34 @@ __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
             ch.r[j] = load(base + j);
         }
     };
-    auto prep_group = [&](const Chunk& ch, int gi, Group& gr) {
-        float ph[G];
-        Rec rc[G];
+    // A group's points (times and records) out of the chunk.
+    struct Pts {
         double t[G], t2[G];
+        Rec rc[G];
+    };
+    auto pts_of = [&](const Chunk& ch, int gi) {
+        Pts p;
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            t[j] = ch.t[gi + j];
-            t2[j] = ch.t2[gi + j];
-            rc[j] = ch.r[gi + j];
+            p.t[j] = ch.t[gi + j];
+            p.t2[j] = ch.t2[gi + j];
+            p.rc[j] = ch.r[gi + j];
         }
-        fold_phases<HAS_PDT, G>(t, t2, f, ph);
-        prep(ph, rc, gr.off, gr.val);
+        return p;
+    };
+    auto prep_group = [&](const Chunk& ch, int gi, Group& gr) {
+        const Pts p = pts_of(ch, gi);
+        Ph ph[G];
+        if constexpr (FASTB) fold_bins<HAS_PDT, G>(p.t, p.t2, f, bins.rule, bins.exact, ph);
+        else fold_phases<HAS_PDT, G>(p.t, p.t2, f, ph);
+        prep(ph, p.rc, gr.off, gr.val);
+    };
+    // Fast bins: group g+1's bins are computed while group g's reads are in
+    // flight, and its exact-path branch (which waits on the compares) comes
+    // after group g is written back.
+    struct Pending {
+        uint32_t k[G];
+        bool slow[G];
     };
     auto run = [&](const Chunk& ch) {
         Group cur, nxt;
@@ -255,24 +303,41 @@ __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
             float r[A][G];
             lds_read_group(col, cur.off, r);
             __builtin_amdgcn_sched_barrier(0);
-            if (gi + 1 < NG) prep_group(ch, (gi + 1) * G, nxt);
-            __builtin_amdgcn_sched_barrier(0);
-            lds_finish_group(col, cur.off, cur.val, r);
-            if (gi + 1 < NG) cur = nxt;
+            if constexpr (FASTB) {
+                Pending pn;
+                Pts p;
+                if (gi + 1 < NG) {
+                    p = pts_of(ch, (gi + 1) * G);
+                    fold_bins_start<HAS_PDT, G>(p.t, p.t2, f, bins.rule, pn.k, pn.slow);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                lds_finish_group(col, cur.off, cur.val, r);
+                if (gi + 1 < NG) {
+                    fold_bins_fix<HAS_PDT, G>(p.t, p.t2, f, bins.exact, pn.slow, pn.k);
+                    prep(pn.k, p.rc, nxt.off, nxt.val);
+                    cur = nxt;
+                }
+            } else {
+                if (gi + 1 < NG) prep_group(ch, (gi + 1) * G, nxt);
+                __builtin_amdgcn_sched_barrier(0);
+                lds_finish_group(col, cur.off, cur.val, r);
+                if (gi + 1 < NG) cur = nxt;
+            }
         }
     };
-    // Ping-pong chunk prefetch, as in scan_points.
+    // Ping-pong chunk prefetch, as in scan_points (vector loads need no
+    // explicit wait: the compiler waits on vmcnt for the chunk in use).
     int64_t i = 0;
     if (n >= 2 * C) {
         const int64_t last = n - C;
         Chunk a, b;
         fetch(0, a);
         for (; i + 2 * C <= n; i += 2 * C) {
-            wait_scalar_loads();
+            if (SCALAR) wait_scalar_loads();
             fetch(i + C, b);
             __builtin_amdgcn_sched_barrier(0);
             run(a);
-            wait_scalar_loads();
+            if (SCALAR) wait_scalar_loads();
             const int64_t na = (i + 2 * C <= last) ? i + 2 * C : last;
             fetch(na, a);
             __builtin_amdgcn_sched_barrier(0);
@@ -280,7 +345,10 @@ __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
         }
     }
     for (; i < n; ++i) {
-        float ph[1] = {fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f)};
+        const float p1 = fold_phase<HAS_PDT>(t64[i], HAS_PDT ? tt[i] : 0.0, f);
+        Ph ph[1];
+        if constexpr (FASTB) ph[0] = bins.exact(p1);
+        else ph[0] = p1;
         Rec rc[1] = {load(i)};
         uint32_t off[1];
         float val[A][1];
@@ -292,6 +360,19 @@ __device__ __forceinline__ void scan_points_rmw(const double* __restrict__ t64,
 struct F2 {
     float a, b;
 };
+
+// How AOV, FPW and MF read their point stream in scan_points_rmw: by vector
+// loads.  With a scalar prefetch in flight, every wait for a group's LDS
+// reads is lgkmcnt(0), which also waits for that prefetch (scalar loads
+// return out of order and share the counter); rocprofv3 showed these waves
+// waiting 52-55% of their time.  Vector loads count on vmcnt instead.
+constexpr bool kRmwScalarStream = false;
+
+template <typename T>
+__device__ __forceinline__ T rmw_load(const T* p) {
+    if constexpr (kRmwScalarStream) return uniform_load(p);
+    else return *p;
+}
 
 // ---------------------------------------------------------------------------
 // Conditional Entropy — rust/src/ce.rs:13-91.
@@ -328,30 +409,35 @@ __device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, c
     const bool nm_even = (nm & 1u) == 0;
     const uint32_t half_row_bytes = (nm / 2) * kPlaneBytes;
 
-    // Scalar loads for the point stream here: no LDS reads wait inside a chunk.
     // The atomic variant takes whole chunks (the histogram is order-free), so
     // the bins of a chunk are computed stage by stage across its points.
+    // The point stream comes by scalar loads.  (Vector loads, whose waits do
+    // not also wait for the chunk's LDS atomics, measured 6% slower at 8192
+    // points: 102 instead of 54 VGPRs and a scalar move of each magnitude
+    // row.)
     constexpr int kCeChunk = HAS_PDT ? kChunk / 2 : kChunk;
     // The point body for a compile-time choice of the packed-even layout
     // (kept out of the loop: one straight-line loop per variant).
     auto body_for = [&](auto even_tag) {
         constexpr bool EVEN = decltype(even_tag)::value;
+        // ph: the points' phase bins with EXT1 (scan with fast bins), else
+        // their phases.
         return [&](const auto& ph, const auto& mb) {
             constexpr int K = group_size_v<decltype(ph)>;
             if constexpr (PACK) {
-                uint32_t kb[K];
-                div_bins(ph, db, kb);
+                const auto& kb = ph;
                 if constexpr (EVEN) {
                     // Cell kb*nm + m: word kb*(nm/2) + m/2, half m & 1.  The
-                    // magnitude part is wave-uniform (scalar operations).
+                    // magnitude part (word offset, increment) comes with the
+                    // point: mb[j] is its CeArgs::mpk record.
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        const uint32_t m = mb[j] / kPlaneBytes;
+                        const uint32_t word = static_cast<uint32_t>(mb[j]);
+                        const uint32_t inc = static_cast<uint32_t>(mb[j] >> 32);
                         char* cell = reinterpret_cast<char*>(col) +
-                                     __umul24(kb[j], half_row_bytes) + (m >> 1) * kPlaneBytes;
-                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(cell),
-                                               1u << ((m & 1u) << 4), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                                     __umul24(kb[j], half_row_bytes) + word;
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(cell), inc,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     }
                 } else {
 #pragma unroll
@@ -363,8 +449,7 @@ __device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, c
                     }
                 }
             } else if constexpr (EXT1 && G == 0) {
-                uint32_t kb[K];
-                div_bins(ph, db, kb);
+                const auto& kb = ph;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     uint32_t* cell = reinterpret_cast<uint32_t*>(
@@ -376,7 +461,7 @@ __device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, c
                 uint32_t one[1][K];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    off[j] = __umul24(div_bin(ph[j], db), row_bytes) + mb[j];
+                    off[j] = __umul24(ph[j], row_bytes) + mb[j];
                     one[0][j] = 1u;
                 }
                 lds_rmw(col, off, one);
@@ -398,12 +483,28 @@ __device__ __forceinline__ void hist_block(const GridArgs& g, const CeArgs& a, c
             }
         };
     };
-    auto scan = [&](auto body) {
-        scan_points<HAS_PDT, EXT1 ? (G > 0 ? G : kCeChunk) : 1>(
-            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(mrow + i); }, body);
+    // With extents of 1, bins: bin n (phase 1.0) is kept, for the spare row.
+    auto scan = [&](auto body, auto load) {
+        if constexpr (EXT1)
+            scan_points<HAS_PDT, (G > 0 ? G : kCeChunk)>(
+                g.t64 + p0, g.tt + p0, n, f, load, body,
+                fast_bins(lane_bin_rule(g.tmax, lt.curve, f, a.n_phase, true),
+                          [&](float ph) { return div_bin(ph, db); }));
+        else
+            scan_points<HAS_PDT, 1>(g.t64 + p0, g.tt + p0, n, f, load, body);
     };
-    if (PACK && nm_even) scan(body_for(std::true_type{}));
-    else scan(body_for(std::false_type{}));
+    auto rows = [&](int64_t i) { return uniform_load(mrow + i); };
+    if constexpr (PACK) {
+        // (read as one 64-bit word: x in the low half)
+        const uint64_t* __restrict__ mpk =
+            a.mpk ? reinterpret_cast<const uint64_t*>(a.mpk + p0) : nullptr;
+        if (nm_even && mpk)
+            scan(body_for(std::true_type{}), [&](int64_t i) { return uniform_load(mpk + i); });
+        else
+            scan(body_for(std::false_type{}), rows);
+    } else {
+        scan(body_for(std::false_type{}), rows);
+    }
     // Count of cell (row j, mag i); with EXT1 the spare row n_phase (phase
     // 1.0) belongs to row 0 (ce.rs:57 wraps it there).
     auto cell_count = [&](int c) -> uint32_t {
@@ -767,22 +868,26 @@ __device__ __forceinline__ void aov_block(const GridArgs& g, const AovArgs& a, c
         // 1 of the read-modify-write, from base + 256).
         const uint32_t lane4 = static_cast<uint32_t>(lane) * 4u;
         char* base = reinterpret_cast<char*>(lds);
-        scan_points_rmw<HAS_PDT, G, 2>(
+        // Bins already wrapped (the fast bins never reach n).
+        scan_points_rmw<HAS_PDT, G, 2, kChunk, kRmwScalarStream>(
             g.t64 + p0, g.tt + p0, n, f,
-            [&](int64_t i) { return F2{uniform_load(y + i), uniform_load(yy + i)}; },
+            [&](int64_t i) { return F2{rmw_load(y + i), rmw_load(yy + i)}; },
             reinterpret_cast<float*>(base + kPlaneBytes),
-            [&](const auto& ph, const auto& v, auto& off, auto& val) {
-                constexpr int K = group_size_v<decltype(ph)>;
+            [&](const auto& kb, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    const uint32_t k = div_bin(ph[j], db);
-                    off[j] = __umul24((k == nbu) ? 0u : k, kBinBytes) + lane4;
+                    off[j] = __umul24(kb[j], kBinBytes) + lane4;
                     __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(base + off[j]), 1u,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     val[0][j] = v[j].a;
                     val[1][j] = v[j].b;
                 }
-            });
+            },
+            fast_bins(lane_bin_rule(g.tmax, lt.curve, f, nb, true), [&](float ph) {
+                const uint32_t k = div_bin(ph, db);
+                return (k == nbu) ? 0u : k;
+            }));
     } else {
         scan_points<HAS_PDT, 1>(
             g.t64 + p0, g.tt + p0, n, f,
@@ -955,34 +1060,41 @@ __device__ __forceinline__ void fpw_block(const GridArgs& g, const FpwArgs& a, c
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
     // (wave-uniform: one scalar load of the set's flag)
     const bool ew = a.wsum != nullptr && uniform_load(a.unequal) == 0;
+    // Bins already clamped to n - 1 (fpw.rs:48; the fast bins never reach n).
+    const auto bins = fast_bins(lane_bin_rule(g.tmax, lt.curve, f, nb, false),
+                                [&](float ph) { return mul_bin(ph, nf, nm1); });
 
     if (ew) {
         uint32_t* ccol = reinterpret_cast<uint32_t*>(col);
-        scan_points_rmw<HAS_PDT, G, 1>(
-            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return uniform_load(wy + i); },
-            col + kWave, [&](const auto& ph, const auto& v, auto& off, auto& val) {
-                constexpr int K = group_size_v<decltype(ph)>;
+        scan_points_rmw<HAS_PDT, G, 1, kChunk, kRmwScalarStream>(
+            g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return rmw_load(wy + i); },
+            col + kWave,
+            [&](const auto& kb, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
+                    off[j] = kb[j] * (2 * kPlaneBytes);
                     val[0][j] = v[j];
-                    __hip_atomic_fetch_add(ccol + off[j] / 4, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    __hip_atomic_fetch_add(
+                        reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ccol) + off[j]), 1u,
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 }
-            });
+            },
+            bins);
     } else {
-        scan_points_rmw<HAS_PDT, G, 2>(
+        scan_points_rmw<HAS_PDT, G, 2, kChunk, kRmwScalarStream>(
             g.t64 + p0, g.tt + p0, n, f,
-            [&](int64_t i) { return F2{uniform_load(w + i), uniform_load(wy + i)}; }, col,
-            [&](const auto& ph, const auto& v, auto& off, auto& val) {
-                constexpr int K = group_size_v<decltype(ph)>;
+            [&](int64_t i) { return F2{rmw_load(w + i), rmw_load(wy + i)}; }, col,
+            [&](const auto& kb, const auto& v, auto& off, auto& val) {
+                constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    off[j] = mul_bin(ph[j], nf, nm1) * (2 * kPlaneBytes);
+                    off[j] = kb[j] * (2 * kPlaneBytes);
                     val[0][j] = v[j].a;
                     val[1][j] = v[j].b;
                 }
-            });
+            },
+            bins);
     }
     // fpw.rs:55-60.
     const float* __restrict__ ws = ew ? a.wsum + p0 : nullptr;
@@ -1068,20 +1180,23 @@ __device__ __forceinline__ void mf_block(const GridArgs& g, const MfArgs& a, con
     const uint32_t nm1 = static_cast<uint32_t>(nb - 1);
     constexpr uint32_t kBinBytes = 3 * kPlaneBytes;
 
-    scan_points_rmw<HAS_PDT, G, 3>(
+    scan_points_rmw<HAS_PDT, G, 3, kChunk, kRmwScalarStream>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) {
-            return F3{uniform_load(w + i), uniform_load(wy + i), uniform_load(wyy + i)};
+            return F3{rmw_load(w + i), rmw_load(wy + i), rmw_load(wyy + i)};
         }, col,
-        [&](const auto& ph, const auto& v, auto& off, auto& val) {
-            constexpr int K = group_size_v<decltype(ph)>;
+        [&](const auto& kb, const auto& v, auto& off, auto& val) {
+            constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                off[j] = __umul24(mul_bin(ph[j], nf, nm1), kBinBytes);
+                off[j] = keep_reg(__umul24(kb[j], kBinBytes));
                 val[0][j] = v[j].a;
                 val[1][j] = v[j].b;
                 val[2][j] = v[j].c;
             }
-        });
+        },
+        // Bins clamped to n - 1 (mf.rs:193).
+        fast_bins(lane_bin_rule(g.tmax, lt.curve, f, nb, false),
+                  [&](float ph) { return mul_bin(ph, nf, nm1); }));
 
     // Bins (mf.rs:196-291).  The profile replaces Y in LDS (0 for an empty bin).
     uint32_t n_filled = 0, n_rms = 0;
@@ -1855,15 +1970,18 @@ __device__ __forceinline__ void bls_block(const GridArgs& g, const BlsArgs& a, c
     // (slower for AOV and FPW, which keep the default).
     scan_points_rmw<HAS_PDT, G, 2, 16, false>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return F2{w[i], yw[i]}; }, col1,
-        [&](const auto& ph, const auto& v, auto& off, auto& val) {
-            constexpr int K = group_size_v<decltype(ph)>;
+        [&](const auto& kb, const auto& v, auto& off, auto& val) {
+            constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                off[j] = mul_bin(ph[j], nf, nm1) * kSlotBytes;
+                off[j] = kb[j] * kSlotBytes;
                 val[0][j] = v[j].a;
                 val[1][j] = v[j].b;
             }
-        });
+        },
+        // Bins clamped to n - 1 (bls.rs:79).
+        fast_bins(lane_bin_rule(g.tmax, lt.curve, f, nb, false),
+                  [&](float ph) { return mul_bin(ph, nf, nm1); }));
     // Prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
     // Eight slots are read before any is added, so the LDS latency is paid
     // once per eight slots rather than once per slot.
@@ -2081,17 +2199,20 @@ __device__ __forceinline__ void bls_block_ew(const GridArgs& g, const BlsArgs& a
     // (8-point chunks: fewer registers than BLS's 16.)
     scan_points_rmw<HAS_PDT, G, 1, 8, false>(
         g.t64 + p0, g.tt + p0, n, f, [&](int64_t i) { return yw[i]; }, ycol + kWave,
-        [&](const auto& ph, const auto& v, auto& off, auto& val) {
-            constexpr int K = group_size_v<decltype(ph)>;
+        [&](const auto& kb, const auto& v, auto& off, auto& val) {
+            constexpr int K = group_size_v<decltype(kb)>;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                const uint32_t k = mul_bin(ph[j], nf, nm1);
+                const uint32_t k = kb[j];
                 off[j] = k * kPlaneBytes;
                 val[0][j] = v[j];
                 __hip_atomic_fetch_add(ccol + k * 32, cadd, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
-        });
+        },
+        // Bins clamped to n - 1 (bls.rs:79).
+        fast_bins(lane_bin_rule(g.tmax, lt.curve, f, nb, false),
+                  [&](float ph) { return mul_bin(ph, nf, nm1); }));
     // Y prefix sums in place (bls.rs:85-91): slot k+1 <- slot k + bin k.
     {
         float py = 0.0f;

@@ -21,6 +21,8 @@ struct GridArgs {
     uint64_t n_blocks;      // n_curves*tblocks workgroups
     float* out;             // [n_curves][T]
     int has_pdt;            // any nonzero period_dt
+    const float* tmax;      // [n_curves] largest |t| of each curve (+inf if a time is
+                            // NaN), for the fast phase bins; nullptr: exact bins only
 };
 
 // Persistent-launch work queue (see pf_grid.hip).
@@ -42,6 +44,10 @@ struct WorkQueue {
 struct CeArgs {
     const uint32_t* mrow;   // [npts] ((mag bin) % n_mag) * 256: byte offset of the column
     int n_phase, n_mag, phase_ext, mag_ext;
+    // [npts] for even n_mag (else nullptr), the packed-count layout's column
+    // m = (mag bin) % n_mag: {(m / 2) * 256, 1 << (16 * (m % 2))}, the byte
+    // offset of its word within a phase row and the increment of its half.
+    const uint2* mpk;
 };
 
 // Viterbi Narrowband (rust/src/vn.rs): the CE histogram, then a circular
@@ -198,8 +204,12 @@ inline size_t lds_bls_ew(int nb) { return size_t(nb + 1) * 64 * 4 + size_t(nb) *
 // Per-point / per-curve preparation kernels (pf_prep.hip).
 hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t npts,
                               hipStream_t s);
-hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, int64_t npts, int n_mag,
-                              hipStream_t s);
+// Largest |t| of each curve (+inf when a time is NaN) into tmax[n_curves].
+hipError_t launch_time_bound(const int64_t* off, const float* t, float* tmax, int64_t n_curves,
+                             hipStream_t s);
+// mpk may be nullptr (odd n_mag: not built).
+hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, uint2* mpk, int64_t npts,
+                              int n_mag, hipStream_t s);
 hipError_t launch_square(const float* y, float* yy, int64_t npts, hipStream_t s);
 hipError_t launch_inv_var(const float* e, float* w, int64_t npts, hipStream_t s);
 hipError_t launch_mul(const float* a, const float* b, float* out, int64_t npts, hipStream_t s);

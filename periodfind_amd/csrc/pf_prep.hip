@@ -29,12 +29,31 @@ __global__ void widen_times_kernel(const float* __restrict__ t, double* __restri
     if (tt) tt[i] = d * d;  // exact: 24x24-bit product
 }
 
+// Largest |t| of each curve, one wave per curve; a NaN time counts as +inf,
+// so the curve's fast phase bins are all replaced by exact ones
+// (pf_fastbin.hpp).
+__global__ __launch_bounds__(64) void time_bound_kernel(const int64_t* __restrict__ off,
+                                                        const float* __restrict__ t,
+                                                        float* __restrict__ tmax) {
+    const int64_t c = blockIdx.x;
+    const int64_t a = off[c], b = off[c + 1];
+    float m = 0.0f;
+    for (int64_t i = a + threadIdx.x; i < b; i += 64) {
+        const float v = fabsf(t[i]);
+        m = fmaxf(m, v == v ? v : __builtin_inff());
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) tmax[c] = m;
+}
+
 // `(m / fl(1/n_mag)) as usize` (rust/src/ce.rs:53; saturating, NaN -> 0),
 // reduced mod n_mag so that (mb + mo) % n_mag can be formed by subtraction,
 // stored as the byte offset of that column in the CE kernel's lane-interleaved
 // histogram (one 64-lane row of u32 per bin).
+// With mpk, also the packed-count layout's {word offset, half increment} of
+// that column (even n_mag).
 __global__ void ce_mag_rows_kernel(const float* __restrict__ y, uint32_t* __restrict__ mrow,
-                                   int64_t n, int n_mag) {
+                                   uint2* __restrict__ mpk, int64_t n, int n_mag) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const float bsz = 1.0f / static_cast<float>(n_mag);
@@ -43,7 +62,9 @@ __global__ void ce_mag_rows_kernel(const float* __restrict__ y, uint32_t* __rest
     if (!(q > 0.0f)) mb = 0;
     else if (q >= 18446744073709551616.0f) mb = UINT64_MAX;
     else mb = static_cast<uint64_t>(q);
-    mrow[i] = static_cast<uint32_t>(mb % static_cast<uint64_t>(n_mag)) * (kWave * 4u);
+    const uint32_t m = static_cast<uint32_t>(mb % static_cast<uint64_t>(n_mag));
+    mrow[i] = m * (kWave * 4u);
+    if (mpk) mpk[i] = make_uint2((m >> 1) * (kWave * 4u), 1u << ((m & 1u) << 4));
 }
 
 __global__ void square_kernel(const float* __restrict__ y, float* __restrict__ yy, int64_t n) {
@@ -468,11 +489,19 @@ hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t n
     return hipGetLastError();
 }
 
-hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, int64_t npts, int n_mag,
-                              hipStream_t s) {
+hipError_t launch_time_bound(const int64_t* off, const float* t, float* tmax, int64_t n_curves,
+                             hipStream_t s) {
+    if (n_curves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(time_bound_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+                       off, t, tmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_ce_mag_rows(const float* y, uint32_t* mrow, uint2* mpk, int64_t npts,
+                              int n_mag, hipStream_t s) {
     if (npts <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ce_mag_rows_kernel, blocks_for(npts), dim3(kBlock), 0, s, y, mrow, npts,
-                       n_mag);
+    hipLaunchKernelGGL(ce_mag_rows_kernel, blocks_for(npts), dim3(kBlock), 0, s, y, mrow, mpk,
+                       npts, n_mag);
     return hipGetLastError();
 }
 

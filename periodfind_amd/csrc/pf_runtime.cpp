@@ -98,10 +98,11 @@ struct CurveSet {
     DevBuf yp;
     int yp_mode = -1;
     int derived_mode = PF_MAG_AS_IS;
-    // Derived, with validity flags.
-    DevBuf t64, tt;
+    // Derived, with validity flags.  tmax (per curve, the largest |t|) is
+    // built with t64.
+    DevBuf t64, tt, tmax;
     bool t64_ok = false, tt_ok = false;
-    DevBuf ce_mrow;
+    DevBuf ce_mrow, ce_mpk;  // mpk: built with mrow for an even n_mag
     int ce_nmag = -1;
     DevBuf aov_yy, aov_avg;
     bool aov_ok = false;
@@ -162,7 +163,7 @@ struct CurveSet {
         drop_graph();
     }
     std::vector<DevBuf*> buffers() {
-        return {&off, &t, &y, &e, &yp, &t64, &tt, &ce_mrow, &aov_yy, &aov_avg, &w,
+        return {&off, &t, &y, &e, &yp, &t64, &tt, &tmax, &ce_mrow, &ce_mpk, &aov_yy, &aov_avg, &w,
                 &fpw_wy, &bls_yw, &bls_mean, &wsum, &w_flag, &mf_wyy, &vn_lut,
                 &vn_lut_off, &mhf_pt, &mhf_sums, &mhf_part, &out, &order, &counter};
     }
@@ -785,6 +786,22 @@ const float* mags(CurveSet& cs, const Params& p) {
     return p.mag_prep == PF_MAG_AS_IS ? as<float>(cs.y) : as<float>(cs.yp);
 }
 
+// The per-curve time bound the binned kernels' fast phase bins read
+// (prepare() builds it with the f64 times), or nullptr: every bin then
+// comes from the exact fold, as with PF_EXACT_BINS set (read at every
+// launch; captured graphs key on the pointer).  MHF folds its own records.
+const float* time_bound(CurveSet& cs, const Params& p) {
+    if (p.algo == PF_ALGO_MHF || std::getenv("PF_EXACT_BINS")) return nullptr;
+    return as<float>(cs.tmax);
+}
+
+// The packed-count columns of CE and VN (built with the magnitude rows for
+// an even n_mag), or nullptr; without them the kernels derive the packed
+// cell from the magnitude row.
+const uint2* ce_packed_cols(CurveSet& cs, const Params& p) {
+    return p.n_mag % 2 == 0 ? as<uint2>(cs.ce_mpk) : nullptr;
+}
+
 // Equal weights (every curve's weights bitwise equal, as with constant or
 // absent errors): a bin's serial sum of weights is then the curve's running
 // sum wsum[c - 1] for its c points, so FPW and BLS count points per bin
@@ -889,8 +906,10 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt, hipStream_
         cs.t64_ok = cs.tt_ok = false;
         PF_TRY(ensure(cs.t64, n * sizeof(double)));
         if (has_pdt) PF_TRY(ensure(cs.tt, n * sizeof(double)));
+        PF_TRY(ensure(cs.tmax, static_cast<size_t>(cs.n_curves) * sizeof(float)));
         PF_HIP(pf::launch_widen_times(as<float>(cs.t), as<double>(cs.t64),
                                       has_pdt ? as<double>(cs.tt) : nullptr, n, s));
+        PF_HIP(pf::launch_time_bound(off, as<float>(cs.t), as<float>(cs.tmax), cs.n_curves, s));
         cs.t64_ok = true;
         cs.tt_ok = has_pdt;
     }
@@ -919,8 +938,12 @@ int prepare(pf_ctx* ctx, CurveSet& cs, const Params& p, bool has_pdt, hipStream_
         case PF_ALGO_CE:
             if (cs.ce_nmag != p.n_mag) {
                 cs.ce_nmag = -1;
+                const bool even = p.n_mag % 2 == 0;
                 PF_TRY(ensure(cs.ce_mrow, n * sizeof(uint32_t)));
-                PF_HIP(pf::launch_ce_mag_rows(y, as<uint32_t>(cs.ce_mrow), n, p.n_mag, s));
+                if (even) PF_TRY(ensure(cs.ce_mpk, n * sizeof(uint2)));
+                PF_HIP(pf::launch_ce_mag_rows(y, as<uint32_t>(cs.ce_mrow),
+                                              even ? as<uint2>(cs.ce_mpk) : nullptr, n,
+                                              p.n_mag, s));
                 cs.ce_nmag = p.n_mag;
             }
             break;
@@ -1009,6 +1032,7 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
     g.n_blocks = static_cast<uint64_t>(cs.n_curves) * static_cast<uint64_t>(g.tblocks);
     g.out = as<float>(cs.out);
     g.has_pdt = has_pdt ? 1 : 0;
+    g.tmax = time_bound(cs, p);
     if (g.n_blocks > 0xffffffffull)
         return fail(PF_E_LIMIT, "grid of %llu workgroups exceeds one launch",
                     static_cast<unsigned long long>(g.n_blocks));
@@ -1023,7 +1047,8 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
     hipError_t le = hipSuccess;
     switch (p.algo) {
         case PF_ALGO_CE: {
-            pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext};
+            pf::CeArgs a{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext,
+                         ce_packed_cols(cs, p)};
             // The atomic variant counts in 16-bit halves (half the LDS, twice
             // the waves per CU) whenever no count can exceed 65535.
             const bool can_pack = p.phase_ext == 1 && p.mag_ext == 1 &&
@@ -1063,7 +1088,8 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
             break;
         }
         case PF_ALGO_VN: {
-            pf::VnArgs a{{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext},
+            pf::VnArgs a{{as<uint32_t>(cs.ce_mrow), p.n_phase, p.n_mag, p.phase_ext, p.mag_ext,
+                          ce_packed_cols(cs, p)},
                          p.bandwidth, p.margin, as<float>(cs.vn_lut), as<int64_t>(cs.vn_lut_off)};
             // 16-bit packed counts when no count can exceed 65535 and every row
             // owns a whole word for the trace's per-row fraction (n_mag >= 3).
@@ -1399,6 +1425,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(ctx->outbuf.p), static_cast<uint64_t>(o_val),
         reinterpret_cast<uint64_t>(cs.out.p), reinterpret_cast<uint64_t>(cs.t64.p),
         reinterpret_cast<uint64_t>(cs.tt.p), reinterpret_cast<uint64_t>(cs.ce_mrow.p),
+        reinterpret_cast<uint64_t>(cs.ce_mpk.p),
         reinterpret_cast<uint64_t>(cs.aov_yy.p), reinterpret_cast<uint64_t>(cs.w.p),
         reinterpret_cast<uint64_t>(cs.fpw_wy.p), reinterpret_cast<uint64_t>(cs.bls_yw.p),
         reinterpret_cast<uint64_t>(cs.wsum_ok ? cs.wsum.p : nullptr),
@@ -1411,6 +1438,7 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         reinterpret_cast<uint64_t>(cs.mhf_pt.p), reinterpret_cast<uint64_t>(cs.mhf_sums.p),
         static_cast<uint64_t>(mhf_max_k(p.max_harmonics)),
         static_cast<uint64_t>(p.algo == PF_ALGO_LS ? ls_fast_mode() : 0),
+        reinterpret_cast<uint64_t>(time_bound(cs, p)),
         reinterpret_cast<uint64_t>(ws.idx), reinterpret_cast<uint64_t>(ws.val)};
     for (const Bucket& b : buckets) {
         key.push_back(static_cast<uint64_t>(b.r0));
@@ -1793,6 +1821,7 @@ int pf_mf_features(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const 
     g.n_blocks = static_cast<uint64_t>(n_curves);
     g.out = nullptr;
     g.has_pdt = 0;
+    g.tmax = time_bound(cs, p);
     if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
     pf::MfArgs a{as<float>(cs.w), as<float>(cs.fpw_wy), as<float>(cs.mf_wyy), mags(cs, p),
                  mf_templates(ctx, num_bins), num_bins, as<float>(ctx->periods),
@@ -1850,6 +1879,7 @@ int pf_mhf_per_k(pf_ctx* ctx, int64_t n_curves, const int64_t* offsets, const fl
     g.n_blocks = static_cast<uint64_t>(n_curves);
     g.out = nullptr;
     g.has_pdt = ctx->has_pdt ? 1 : 0;
+    g.tmax = nullptr;  // MHF folds phases for its sums, not bins
     if (g.n_blocks > 0xffffffffull) return fail(PF_E_LIMIT, "too many curves for one launch");
     pf::MhfArgs a{};
     PF_TRY(mhf_args(ctx, cs, K, 1, as<float>(ctx->outbuf), &a));
@@ -1959,10 +1989,14 @@ int pf_peaks_noalloc(pf_ctx* ctx, void* stream, const pf_params* params, int64_t
     };
     const uint64_t o_off = region((C + 1) * 8), o_per = region(n_periods * 4ull),
                    o_pdt = region(n_pdts * 4ull), o_t64 = region(N * 8),
-                   o_tt = has_pdt ? region(N * 8) : 0, o_yp = prep ? region(N * 4) : 0;
+                   o_tt = has_pdt ? region(N * 8) : 0, o_yp = prep ? region(N * 4) : 0,
+                   o_tmax = region(C * 4);
     uint64_t o_a = 0, o_b = 0, o_c = 0;  // the statistic's own arrays
     switch (p.algo) {
-        case PF_ALGO_CE: o_a = region(N * 4); break;                      // magnitude rows
+        case PF_ALGO_CE:  // magnitude rows; packed columns (even n_mag)
+            o_a = region(N * 4);
+            if (p.n_mag % 2 == 0) o_b = region(N * 8);
+            break;
         case PF_ALGO_AOV: o_a = region(N * 4); o_c = region(C * 4); break;  // y*y, mean
         case PF_ALGO_FPW: o_a = region(N * 4); o_b = region(N * 4); break;  // w, w*y
         case PF_ALGO_BLS:                                                  // w, yw, mean
@@ -2021,10 +2055,14 @@ int pf_peaks_noalloc(pf_ctx* ctx, void* stream, const pf_params* params, int64_t
     cs.y = fixed(const_cast<float*>(y), N * 4);
     if (err) cs.e = fixed(const_cast<float*>(err), N * 4);
     cs.t64 = fixed(base + o_t64, N * 8);
+    cs.tmax = fixed(base + o_tmax, C * 4);
     if (has_pdt) cs.tt = fixed(base + o_tt, N * 8);
     if (prep) cs.yp = fixed(base + o_yp, N * 4);
     switch (p.algo) {
-        case PF_ALGO_CE: cs.ce_mrow = fixed(base + o_a, N * 4); break;
+        case PF_ALGO_CE:
+            cs.ce_mrow = fixed(base + o_a, N * 4);
+            if (p.n_mag % 2 == 0) cs.ce_mpk = fixed(base + o_b, N * 8);
+            break;
         case PF_ALGO_AOV:
             cs.aov_yy = fixed(base + o_a, N * 4);
             cs.aov_avg = fixed(base + o_c, C * 4);

@@ -289,6 +289,25 @@ def test_device_division_identities():
     assert "fold_bad=0 bin_bad=0 fast_bad=0" in r.stdout
 
 
+def test_fast_phase_bins_equal_the_reference_bins():
+    """The fast phase bin of the binned kernels (periodfind_amd/csrc/pf_fastbin.hpp,
+    compiled here for the host): wherever it reports a certain bin, that bin
+    equals the reference's (fold.rs then ce.rs:52 / fpw.rs:48), on random
+    times and periods, times within a few ulps of every bin edge, and special
+    values.  The check has teeth: an eps 16 times too small, or a cycle term
+    64 times too small, gives hundreds of wrong bins on the same cases."""
+    src = os.path.join(HERE, "exactness", "check_fastbin.cpp")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "check_fastbin")
+        subprocess.run(["g++", "-O2", "-ffp-contract=off", "-o", exe, src], check=True)
+        r = subprocess.run([exe, "4000000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("bad=0")
+    fast = int(r.stdout.split("fast=")[1].split()[0])
+    cases = int(r.stdout.split("cases=")[1].split()[0])
+    assert fast > 0.99 * cases  # the exact branch stays rare
+
+
 def test_device_logf_is_the_c_library_logf():
     """The logf the CE and VN kernels use (periodfind_amd/csrc/pf_logf.hpp,
     compiled here for the host) equals the C library's logf, which the
