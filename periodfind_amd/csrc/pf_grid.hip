@@ -43,16 +43,62 @@ __device__ __forceinline__ LaneTrial lane_trial(uint64_t n_blocks, int64_t tbloc
     return lt;
 }
 
-// Persistent-launch work queue: item = rank * tblocks + trial block, where
-// rank orders curves longest first (so the longest items start first and
-// the tail is short).  Claims are guided: a wave takes remaining / (2 * waves)
-// items (at least 1, at most the claim bound of the bucket it is in),
-// estimated from the counter value it saw last, so claims are large while the
-// queue is full and single items at the end, with few atomics on the shared
-// counter.  `seen` is wave-uniform.
+// Persistent-launch work queue, one part per XCD.  Ranks order curves
+// longest first (so the longest items start first and the tail is short);
+// rank r belongs to part r % 8, so each part keeps that order and gets an
+// even share.  A wave claims from the part of its own XCD (gfx950 deals
+// workgroup i to XCD i % 8) and, once that is empty, from the others in
+// turn.  The waves of one XCD thus work on a few curves at a time, whose
+// points stay in that XCD's L2: with one queue for all, consecutive claims
+// went to every XCD and the resident waves spread over hundreds of curves
+// (CE on 10,000 ragged curves fetched 152 GiB from HBM, against 0.7 GiB for
+// one plain launch).  Within a part, items are numbered virtually: the
+// parts' items are laid end to end, part p's rank j being virtual rank
+// base(p) + j, item = virtual rank * tblocks + trial block, so a claim is a
+// contiguous range (item_rank maps it back to the rank).  Claims are guided:
+// a wave takes remaining / (2 * waves per part) items (at least 1, at most
+// the claim bound of the bucket it is in), estimated from the counter value
+// it saw last, so claims are large while the part is full and single items
+// at the end, with few atomics on each counter.  The state is wave-uniform.
+constexpr int kQueueParts = kQueuePartsHost;
+
 struct Claim {
     int64_t begin, end;
 };
+
+struct QueueState {
+    int part;      // the part being drained
+    int64_t seen;  // its items known to be claimed
+    int tried;     // parts found empty in a row
+};
+
+__device__ __forceinline__ QueueState queue_start() {
+    return QueueState{static_cast<int>(blockIdx.x % kQueueParts), 0, 0};
+}
+
+// Ranks of the parts: the first nr % 8 parts hold one more.
+struct QueueParts {
+    int64_t q, r;  // nr = 8 q + r
+    __device__ int64_t ranks(int p) const { return q + (p < r ? 1 : 0); }
+    __device__ int64_t base(int p) const { return p * q + (p < r ? p : r); }
+    // virtual rank -> rank
+    __device__ int64_t rank(int64_t v) const {
+        const int64_t big = r * (q + 1);
+        const int64_t p = v < big ? v / (q + 1) : r + (v - big) / (q > 0 ? q : 1);
+        return (v - base(static_cast<int>(p))) * kQueueParts + p;
+    }
+};
+
+__device__ __forceinline__ QueueParts queue_parts(const WorkQueue& q, const GridArgs& g) {
+    const int64_t nr = q.total / g.tblocks;
+    return QueueParts{nr / kQueueParts, nr % kQueueParts};
+}
+
+// The rank of a (virtual) queue item.
+__device__ __forceinline__ int64_t item_rank(const WorkQueue& q, const GridArgs& g,
+                                             int64_t item) {
+    return queue_parts(q, g).rank(item / g.tblocks);
+}
 
 // Binning variant and claim bound of a rank's bucket (buckets are contiguous
 // rank ranges; wave-uniform).  Constant indices only, so the queue stays in
@@ -73,26 +119,45 @@ __device__ __forceinline__ int rank_claim(const WorkQueue& q, int64_t rank) {
     return v;
 }
 
+// The next claim, or {total, total} once every part is empty (each wave
+// then has found all 8 parts empty: every item was claimed exactly once, by
+// the atomics).
 __device__ __forceinline__ Claim next_claim(const WorkQueue& q, const GridArgs& g,
-                                            int64_t& seen) {
-    const int64_t bound = rank_claim(q, seen / g.tblocks);
-    int64_t c = (q.total - seen) / (2 * static_cast<int64_t>(gridDim.x));
-    c = c < 1 ? 1 : (c > bound ? bound : c);
-    unsigned long long v = 0;
-    if (threadIdx.x == 0) v = atomicAdd(q.counter, static_cast<unsigned long long>(c));
-    v = __shfl(v, 0);
-    const int64_t b = static_cast<int64_t>(v);
-    seen = b + c;
-    return Claim{b, b + c < q.total ? b + c : q.total};
+                                            QueueState& st) {
+    const QueueParts qp = queue_parts(q, g);
+    const int64_t waves = gridDim.x / kQueueParts > 0 ? gridDim.x / kQueueParts : 1;
+    while (st.tried < kQueueParts) {
+        const int p = st.part;
+        const int64_t total = qp.ranks(p) * g.tblocks;
+        if (st.seen < total) {
+            const int64_t vbase = qp.base(p) * g.tblocks;
+            const int64_t bound = rank_claim(q, qp.rank((vbase + st.seen) / g.tblocks));
+            int64_t c = (total - st.seen) / (2 * waves);
+            c = c < 1 ? 1 : (c > bound ? bound : c);
+            unsigned long long v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(q.counter + p, static_cast<unsigned long long>(c));
+            v = __shfl(v, 0);
+            const int64_t b = static_cast<int64_t>(v);
+            st.seen = b + c;
+            if (b < total) {
+                st.tried = 0;
+                return Claim{vbase + b, vbase + (b + c < total ? b + c : total)};
+            }
+        }
+        st.part = (p + 1) % kQueueParts;
+        st.seen = 0;
+        st.tried += 1;
+    }
+    return Claim{q.total, q.total};
 }
 
 __device__ __forceinline__ LaneTrial queue_trial(const WorkQueue& q, const GridArgs& g,
                                                  int64_t item,
                                                  int lane = static_cast<int>(threadIdx.x)) {
     LaneTrial lt;
-    const int64_t rank = item / g.tblocks;
-    const int64_t tb = item - rank * g.tblocks;
-    lt.curve = q.order[rank];
+    const int64_t vr = item / g.tblocks;
+    const int64_t tb = item - vr * g.tblocks;
+    lt.curve = q.order[queue_parts(q, g).rank(vr)];
     const int64_t tr = tb * kWave + lane;
     lt.active = tr < g.T;
     lt.trial = lt.active ? tr : g.T - 1;
@@ -630,9 +695,11 @@ __global__ __launch_bounds__(kWave) void ce_kernel(GridArgs g, CeArgs a) {
 }
 
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
-template <bool HAS_PDT, bool EXT1>
+// PACKED: every bucket takes the packed atomic variant, the only code path
+// compiled in (115 instead of 120 VGPRs, 25 instead of 63 KB of code).
+template <bool HAS_PDT, bool EXT1, bool PACKED = false>
 __global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it);
@@ -640,7 +707,11 @@ __global__ __launch_bounds__(kWave) void ce_persistent(GridArgs g, CeArgs a, Wor
                 ce_block<HAS_PDT, false, 1>(g, a, lt);
                 continue;
             }
-            switch (rank_group(q, it / g.tblocks)) {
+            if constexpr (PACKED) {
+                ce_block<HAS_PDT, EXT1, -1>(g, a, lt);
+                continue;
+            }
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case -1: ce_block<HAS_PDT, EXT1, -1>(g, a, lt); break;
                 case 0: ce_block<HAS_PDT, EXT1, 0>(g, a, lt); break;
                 case 2: ce_block<HAS_PDT, EXT1, 2>(g, a, lt); break;
@@ -822,7 +893,7 @@ __global__ __launch_bounds__(kWave) void vn_kernel(GridArgs g, VnArgs a) {
 
 template <bool HAS_PDT, bool EXT1, int MAXM>
 __global__ __launch_bounds__(kWave) void vn_persistent(GridArgs g, VnArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it);
@@ -830,7 +901,7 @@ __global__ __launch_bounds__(kWave) void vn_persistent(GridArgs g, VnArgs a, Wor
                 vn_block<HAS_PDT, false, 1, MAXM>(g, a, lt);
                 continue;
             }
-            if (rank_group(q, it / g.tblocks) < 0) vn_block<HAS_PDT, EXT1, -1, MAXM>(g, a, lt);
+            if (rank_group(q, item_rank(q, g, it)) < 0) vn_block<HAS_PDT, EXT1, -1, MAXM>(g, a, lt);
             else vn_block<HAS_PDT, EXT1, 0, MAXM>(g, a, lt);
         }
 }
@@ -939,7 +1010,7 @@ __global__ __launch_bounds__(kWave) void aov_kernel(GridArgs g, AovArgs a) {
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
 template <bool HAS_PDT, bool EXT1>
 __global__ __launch_bounds__(kWave) void aov_persistent(GridArgs g, AovArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it);
@@ -947,7 +1018,7 @@ __global__ __launch_bounds__(kWave) void aov_persistent(GridArgs g, AovArgs a, W
                 aov_block<HAS_PDT, false, 1>(g, a, lt);
                 continue;
             }
-            switch (rank_group(q, it / g.tblocks)) {
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case 2: aov_block<HAS_PDT, EXT1, 2>(g, a, lt); break;
                 case 4: aov_block<HAS_PDT, EXT1, 4>(g, a, lt); break;
                 default: aov_block<HAS_PDT, EXT1, 1>(g, a, lt); break;
@@ -1029,7 +1100,7 @@ __global__ __launch_bounds__(kWave) void ls_kernel(GridArgs g, LsArgs a) {
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
 template <bool HAS_PDT, bool FAST>
 __global__ __launch_bounds__(kWave) void ls_persistent(GridArgs g, LsArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it)
             ls_block<HAS_PDT, FAST>(g, a, queue_trial(q, g, it));
@@ -1126,11 +1197,11 @@ __global__ __launch_bounds__(kWave) void fpw_kernel(GridArgs g, FpwArgs a) {
 // Persistent: each wave pulls (curve, 64-trial block) items from the queue.
 template <bool HAS_PDT>
 __global__ __launch_bounds__(kWave) void fpw_persistent(GridArgs g, FpwArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it);
-            switch (rank_group(q, it / g.tblocks)) {
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case 2: fpw_block<HAS_PDT, 2>(g, a, lt); break;
                 case 4: fpw_block<HAS_PDT, 4>(g, a, lt); break;
                 default: fpw_block<HAS_PDT, 1>(g, a, lt); break;
@@ -1329,11 +1400,11 @@ __global__ __launch_bounds__(kWave) void mf_kernel(GridArgs g, MfArgs a) {
 
 template <bool HAS_PDT>
 __global__ __launch_bounds__(kWave) void mf_persistent(GridArgs g, MfArgs a, WorkQueue q) {
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it);
-            switch (rank_group(q, it / g.tblocks)) {
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case 2: mf_block<HAS_PDT, 2>(g, a, lt); break;
                 case 4: mf_block<HAS_PDT, 4>(g, a, lt); break;
                 default: mf_block<HAS_PDT, 1>(g, a, lt); break;
@@ -1754,7 +1825,7 @@ template <bool HAS_PDT, int K>
 __global__ __launch_bounds__(kWave) void mhf_persistent(GridArgs g, MhfArgs a, WorkQueue q) {
     extern __shared__ double2 tab[];
     if constexpr (K > 0) mhf_fill_table(tab);
-    int64_t seen = 0;
+    QueueState seen = queue_start();
     for (Claim cl = next_claim(q, g, seen); cl.begin < q.total; cl = next_claim(q, g, seen))
         for (int64_t it = cl.begin; it < cl.end; ++it)
             mhf_block<HAS_PDT, K>(g, a, queue_trial(q, g, it), tab);
@@ -2121,7 +2192,7 @@ __global__ __launch_bounds__(kWave * kBlsWaves) void bls_persistent(GridArgs g, 
     __shared__ int64_t claim[2];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
-    int64_t seen = 0;  // used by wave 0 only
+    QueueState seen = queue_start();  // used by wave 0 only
     for (;;) {
         if (wave == 0) {
             const Claim cl = next_claim(q, g, seen);
@@ -2136,7 +2207,7 @@ __global__ __launch_bounds__(kWave * kBlsWaves) void bls_persistent(GridArgs g, 
         if (begin >= q.total) break;
         for (int64_t it = begin; it < end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it, lane);
-            switch (rank_group(q, it / g.tblocks)) {
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case 2: bls_block<HAS_PDT, 2, kBlsWaves>(g, a, lt, lane, wave); break;
                 case 4: bls_block<HAS_PDT, 4, kBlsWaves>(g, a, lt, lane, wave); break;
                 default: bls_block<HAS_PDT, 1, kBlsWaves>(g, a, lt, lane, wave); break;
@@ -2371,7 +2442,7 @@ __attribute__((amdgpu_waves_per_eu(2 * kBlsEwWaves))) void bls_ew_persistent(Gri
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     // (wave-uniform: the search indexes registers by p = wave + 2k)
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-    int64_t seen = 0;  // used by wave 0 only
+    QueueState seen = queue_start();  // used by wave 0 only
     for (;;) {
         if (wave == 0) {
             const Claim cl = next_claim(q, g, seen);
@@ -2386,7 +2457,7 @@ __attribute__((amdgpu_waves_per_eu(2 * kBlsEwWaves))) void bls_ew_persistent(Gri
         if (begin >= q.total) break;
         for (int64_t it = begin; it < end; ++it) {
             const LaneTrial lt = queue_trial(q, g, it, lane);
-            switch (rank_group(q, it / g.tblocks)) {
+            switch (rank_group(q, item_rank(q, g, it))) {
                 case 2: bls_block_ew<HAS_PDT, 2, kBlsEwWaves>(g, a, lt, lane, wave); break;
                 case 4: bls_block_ew<HAS_PDT, 4, kBlsEwWaves>(g, a, lt, lane, wave); break;
                 default: bls_block_ew<HAS_PDT, 1, kBlsEwWaves>(g, a, lt, lane, wave); break;
@@ -2431,11 +2502,13 @@ void launch_persistent(K kernel, const GridArgs& g, const Args& a, size_t lds,
 }
 
 // Plain launches take the binning variant `grp`; persistent launches carry
-// one per length bucket in the queue.
+// one per length bucket in the queue (CE: grp < 0 when every bucket's is
+// the packed one).
 template <bool P, bool E>
 void ce_g(const GridArgs& g, const CeArgs& a, size_t lds, int grp, const WorkQueue* q,
           hipStream_t s) {
-    if (q) launch_persistent(ce_persistent<P, E>, g, a, lds, *q, s);
+    if (q && E && grp < 0) launch_persistent(ce_persistent<P, E, true>, g, a, lds, *q, s);
+    else if (q) launch_persistent(ce_persistent<P, E>, g, a, lds, *q, s);
     else if (!E) launch_grid_kernel(ce_kernel<P, false, 1>, g, a, lds, s);
     else if (grp < 0) launch_grid_kernel(ce_kernel<P, E, -1>, g, a, lds, s);
     else if (grp == 0) launch_grid_kernel(ce_kernel<P, E, 0>, g, a, lds, s);

@@ -7,6 +7,7 @@
 namespace pf {
 
 constexpr int kWaveHost = 64;  // wave64: trials per workgroup
+constexpr int kQueuePartsHost = 8;  // persistent-queue parts, one per XCD
 
 // Shared by every grid kernel: the curve set (CSR) and the trial grid.
 struct GridArgs {
@@ -32,7 +33,8 @@ struct GridArgs {
 constexpr int kQueueBuckets = 8;
 
 struct WorkQueue {
-    unsigned long long* counter;  // next unclaimed item (zeroed before launch)
+    unsigned long long* counter;  // [kQueuePartsHost] next unclaimed item of each
+                                  // part (zeroed before launch)
     const int64_t* order;         // rank -> curve index, longest curve first
     int64_t total;                // n_ranks * tblocks
     int n_buckets;                // 1..kQueueBuckets

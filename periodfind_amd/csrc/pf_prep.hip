@@ -29,21 +29,28 @@ __global__ void widen_times_kernel(const float* __restrict__ t, double* __restri
     if (tt) tt[i] = d * d;  // exact: 24x24-bit product
 }
 
-// Largest |t| of each curve, one wave per curve; a NaN time counts as +inf,
-// so the curve's fast phase bins are all replaced by exact ones
-// (pf_fastbin.hpp).
-__global__ __launch_bounds__(64) void time_bound_kernel(const int64_t* __restrict__ off,
-                                                        const float* __restrict__ t,
-                                                        float* __restrict__ tmax) {
+// Largest |t| of each curve, one 256-thread block per curve; a NaN time
+// counts as +inf, so the curve's fast phase bins are all replaced by exact
+// ones (pf_fastbin.hpp).
+constexpr int kBoundThreads = 256;
+
+__global__ __launch_bounds__(kBoundThreads) void time_bound_kernel(
+    const int64_t* __restrict__ off, const float* __restrict__ t, float* __restrict__ tmax) {
+    __shared__ float part[kBoundThreads / 64];
     const int64_t c = blockIdx.x;
     const int64_t a = off[c], b = off[c + 1];
     float m = 0.0f;
-    for (int64_t i = a + threadIdx.x; i < b; i += 64) {
+    for (int64_t i = a + threadIdx.x; i < b; i += kBoundThreads) {
         const float v = fabsf(t[i]);
         m = fmaxf(m, v == v ? v : __builtin_inff());
     }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if (threadIdx.x == 0) tmax[c] = m;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBoundThreads / 64; ++w) m = fmaxf(m, part[w]);
+        tmax[c] = m;
+    }
 }
 
 // `(m / fl(1/n_mag)) as usize` (rust/src/ce.rs:53; saturating, NaN -> 0),
@@ -492,7 +499,8 @@ hipError_t launch_widen_times(const float* t, double* t64, double* tt, int64_t n
 hipError_t launch_time_bound(const int64_t* off, const float* t, float* tmax, int64_t n_curves,
                              hipStream_t s) {
     if (n_curves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(time_bound_kernel, dim3(static_cast<uint32_t>(n_curves)), dim3(64), 0, s,
+    hipLaunchKernelGGL(time_bound_kernel, dim3(static_cast<uint32_t>(n_curves)),
+                       dim3(kBoundThreads), 0, s,
                        off, t, tmax);
     return hipGetLastError();
 }

@@ -1062,7 +1062,8 @@ int launch_grid(pf_ctx* ctx, CurveSet& cs, const Params& p, int64_t np, int64_t 
                     if (qq.bucket_group[b] == 0 && can_pack) qq.bucket_group[b] = -1;
                     if (qq.bucket_group[b] >= 0) all_packed = false;
                 }
-                le = pf::launch_ce(g, a, all_packed ? lds_pack : lds_full, 0, &qq, s);
+                le = pf::launch_ce(g, a, all_packed ? lds_pack : lds_full, all_packed ? -1 : 0,
+                                   &qq, s);
             } else {
                 const int grp = (group == 0 && can_pack) ? -1 : group;
                 le = pf::launch_ce(g, a, grp < 0 ? lds_pack : lds_full, grp, nullptr, s);
@@ -1452,7 +1453,8 @@ int ragged_peaks_on(pf_ctx* ctx, CurveSet& cs, const Params& p, const float* per
         unsigned long long* counters = as<unsigned long long>(cs.counter);
         PF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int rc = PF_OK;
-        if (pf::launch_zero_u64(counters, 1, s) != hipSuccess)
+        // one counter per queue part (XCD)
+        if (pf::launch_zero_u64(counters, pf::kQueuePartsHost, s) != hipSuccess)
             rc = fail(PF_E_HIP, "counter reset launch failed during capture");
         if (rc == PF_OK) {
             pf::WorkQueue q{};
