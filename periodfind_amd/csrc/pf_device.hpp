@@ -190,12 +190,19 @@ __device__ __forceinline__ void fold_bins(const double (&t)[K], const double (&t
 #pragma unroll
     for (int j = 0; j < K; ++j) y[j] = __builtin_fma(y[j], br.c, br.eps);
     __builtin_amdgcn_sched_barrier(0);
-    double lo = 1.0;
+    // The minimum as a pairwise tree: depth log2(K) instead of a chain of K
+    // dependent f64 operations.
+    double fr[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         k[j] = static_cast<uint32_t>(y[j]);
-        lo = __builtin_fmin(lo, __builtin_amdgcn_fract(y[j]));
+        fr[j] = __builtin_amdgcn_fract(y[j]);
     }
+#pragma unroll
+    for (int w = 1; w < K; w *= 2)
+#pragma unroll
+        for (int j = 0; j + w < K; j += 2 * w) fr[j] = __builtin_fmin(fr[j], fr[j + w]);
+    const double lo = fr[0];
     if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(lo > br.eps2)) != 0, 0)) {
         // y again (the same operations), so it need not stay live.
 #pragma unroll

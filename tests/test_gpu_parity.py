@@ -569,6 +569,36 @@ def test_ragged_persistent_graph_peaks(ctx, algo):
     cs.close()
 
 
+@pytest.mark.parametrize("n_curves", [1, 3, 7, 8, 9, 17])
+@pytest.mark.parametrize("algo", ["ce", "aov", "bls"])
+def test_ragged_queue_parts(ctx, algo, n_curves):
+    """The persistent queue's per-XCD parts (rank r in part r % 8): curve
+    counts below, at and just above the number of parts, where some parts
+    are empty or one longer than the others.  Every trial block is claimed
+    exactly once: the ragged peaks equal the plain launch's and the oracle's."""
+    from periodfind_amd import capi
+
+    print(f"ragged queue parts {algo} n_curves={n_curves}", flush=True)
+    rng = np.random.default_rng(100 + n_curves)
+    lens = list(np.round(np.exp(rng.uniform(np.log(16), np.log(1500), n_curves))).astype(int))
+    t, m, e = _curves(lens, seed=200 + n_curves)
+    m = _prep(algo, m)
+    P = (1.0 / np.linspace(0.05, 5.0, 300)).astype(np.float32)  # 10 trial blocks per curve
+    D = np.array([0.0, 1e-5], np.float32)
+    prm = PARAMS[algo] if algo != "aov" else dict(n_phase=15)
+    cs = capi.CurveSet(ctx, t, m, errs=e if algo in ("fpw", "bls") else None)
+    ri, rv = cs.peaks(algo, P, D, n_peaks=8, min_distance=1, ragged=False, **prm)
+    for rep in range(2):
+        gi, gv = cs.peaks(algo, P, D, n_peaks=8, min_distance=1, ragged=True, **prm)
+        np.testing.assert_array_equal(gi, ri)
+        np.testing.assert_array_equal(gv, rv)
+    grid = _oracle(algo, t, m, e, P, D, **PARAMS[algo])
+    oi, ov = O.find_top_peaks_batched(grid, 8, 1, use_max=algo != "ce")
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gv, ov)
+    cs.close()
+
+
 @pytest.mark.parametrize("algo", ["ce", "bls"])
 def test_heavy_shape_subsample(ctx, algo):
     """The heavy workload's shape: 65,536 points over a 3000-day baseline,
