@@ -66,6 +66,18 @@ struct Claim {
     int64_t begin, end;
 };
 
+// A wave-uniform 64-bit value, read from the wave's first active lane into
+// scalar registers.  What is derived from a claim (curve, point offsets) is
+// then visibly uniform, and the point stream takes scalar loads: a value
+// passed through a lane shuffle or LDS stays a vector register, and every
+// load indexed by it becomes a 64-lane vector load of one address.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+    const uint64_t u = static_cast<uint64_t>(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 struct QueueState {
     int part;      // the part being drained
     int64_t seen;  // its items known to be claimed
@@ -134,10 +146,11 @@ __device__ __forceinline__ Claim next_claim(const WorkQueue& q, const GridArgs& 
             const int64_t bound = rank_claim(q, qp.rank((vbase + st.seen) / g.tblocks));
             int64_t c = (total - st.seen) / (2 * waves);
             c = c < 1 ? 1 : (c > bound ? bound : c);
+            // Lane 0 claims (every caller reaches here with the whole wave
+            // active, so the first active lane is lane 0).
             unsigned long long v = 0;
             if (threadIdx.x == 0) v = atomicAdd(q.counter + p, static_cast<unsigned long long>(c));
-            v = __shfl(v, 0);
-            const int64_t b = static_cast<int64_t>(v);
+            const int64_t b = wave_uniform(static_cast<int64_t>(v));
             st.seen = b + c;
             if (b < total) {
                 st.tried = 0;
@@ -2202,7 +2215,7 @@ __global__ __launch_bounds__(kWave * kBlsWaves) void bls_persistent(GridArgs g, 
             }
         }
         __syncthreads();
-        const int64_t begin = claim[0], end = claim[1];
+        const int64_t begin = wave_uniform(claim[0]), end = wave_uniform(claim[1]);
         __syncthreads();  // every wave holds the claim before wave 0 takes the next
         if (begin >= q.total) break;
         for (int64_t it = begin; it < end; ++it) {
@@ -2452,7 +2465,7 @@ __attribute__((amdgpu_waves_per_eu(2 * kBlsEwWaves))) void bls_ew_persistent(Gri
             }
         }
         __syncthreads();
-        const int64_t begin = claim[0], end = claim[1];
+        const int64_t begin = wave_uniform(claim[0]), end = wave_uniform(claim[1]);
         __syncthreads();  // every wave holds the claim before wave 0 takes the next
         if (begin >= q.total) break;
         for (int64_t it = begin; it < end; ++it) {
