@@ -39,3 +39,15 @@ def test_shipped_table_is_the_rule_applied_to_the_shipped_report():
     # The long-curve CE claim is no longer the L2-thrashing 64.
     ce = {lo: c for a, lo, g, c in got if a == "ce"}
     assert ce[4096] == 16
+
+
+def test_from_report_cli_writes_the_shipped_entries(tmp_path):
+    out = tmp_path / "tuning.txt"
+    rc = tune.main(["--from-report",
+                    os.path.join(ROOT, "profiles", "tuning_report_gfx950.json"),
+                    "--out", str(out)])
+    assert rc == 0
+    body = [l for l in out.read_text().splitlines() if l and not l.startswith("#")]
+    shipped = [l.strip() for l in open(os.path.join(ROOT, "periodfind_amd", "tuning_gfx950.txt"))
+               if l.strip() and not l.startswith("#")]
+    assert body == shipped
